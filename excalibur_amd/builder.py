@@ -1,0 +1,73 @@
+"""Builds libexcalibur_amd.so (gfx950 kernels + C ABI + host runtime) and the peer executable.
+
+Every translation unit is compiled by hipcc for gfx950 with FMA contraction off (the bit-exact
+recurrences rely on it) and packed-FP32 instructions disabled; objects go to
+excalibur_amd/_build/, the library and the peer executable next to this file.
+"""
+import concurrent.futures as cf
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+OBJ = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "libexcalibur_amd.so")
+PEER = os.path.join(HERE, "excalibur_peer")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+COMMON = [
+    "-std=c++17", "-O3", "-fPIC", "-ffp-contract=off", "-fno-fast-math",
+    "-Wall", "-Wno-unused-function", "-Wno-sign-compare", "-Wno-unused-variable",
+    "-I", os.path.join(ROOT, "include"), "-I", CSRC,
+]
+DEVICE = ["--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+          "-munsafe-fp-atomics"]
+
+# sources of the shared library; the peer executable adds peer_main.cpp
+LIB_SOURCES = sorted(f for f in os.listdir(CSRC) if f.endswith((".hip", ".cpp")) and f != "peer_main.cpp")
+
+
+def _compile(src):
+    path = os.path.join(CSRC, src)
+    out = os.path.join(OBJ, src + ".o")
+    cmd = [HIPCC] + COMMON + ["-c", path, "-o", out]
+    if src.endswith(".hip"):
+        cmd[1:1] = ["-x", "hip"] + DEVICE
+    else:
+        cmd[1:1] = ["-x", "hip"] + DEVICE  # host runtime files call HIP APIs and may hold kernels
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"compile failed: {src}\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    return out
+
+
+def build(verbose=False, jobs=None):
+    os.makedirs(OBJ, exist_ok=True)
+    jobs = jobs or min(8, os.cpu_count() or 4)
+    srcs = LIB_SOURCES + (["peer_main.cpp"] if os.path.exists(os.path.join(CSRC, "peer_main.cpp")) else [])
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        objs = dict(zip(srcs, ex.map(_compile, srcs)))
+    lib_objs = [objs[s] for s in LIB_SOURCES]
+    tmp = LIB + ".tmp"
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + lib_objs + ["-lpthread"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"link failed\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, LIB)
+    if "peer_main.cpp" in objs:
+        tmp = PEER + ".tmp"
+        cmd = [HIPCC, "--offload-arch=gfx950", "-o", tmp, objs["peer_main.cpp"], "-L", HERE,
+               "-lexcalibur_amd", "-Wl,-rpath,$ORIGIN", "-lpthread"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"link failed\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+        os.replace(tmp, PEER)
+    if verbose:
+        print("built", LIB)
+
+
+if __name__ == "__main__":
+    build(verbose=True)
+    sys.exit(0)

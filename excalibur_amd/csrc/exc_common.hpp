@@ -1,0 +1,51 @@
+// Shared helpers for the gfx950 kernels and the C ABI of libexcalibur_amd.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+
+#include "excalibur_hip.h"
+
+#define EXC_CHECK_HIP(expr)                                                                  \
+  do {                                                                                       \
+    hipError_t _e = (expr);                                                                  \
+    if (_e != hipSuccess) {                                                                  \
+      exc_detail::set_last_error(#expr, _e, __FILE__, __LINE__);                             \
+      return EXC_ERR_HIP;                                                                    \
+    }                                                                                        \
+  } while (0)
+
+#define EXC_REQUIRE(cond, code, msg)                                                         \
+  do {                                                                                       \
+    if (!(cond)) {                                                                           \
+      exc_detail::set_last_message(msg, __FILE__, __LINE__);                                 \
+      return (code);                                                                         \
+    }                                                                                        \
+  } while (0)
+
+namespace exc_detail {
+void set_last_error(const char* what, hipError_t e, const char* file, int line);
+void set_last_message(const char* what, const char* file, int line);
+inline hipStream_t stream(exc_stream_t s) { return reinterpret_cast<hipStream_t>(s); }
+inline unsigned ceil_div(long a, long b) { return (unsigned)((a + b - 1) / b); }
+}  // namespace exc_detail
+
+// Wavefront helpers (CDNA: 64 lanes).
+constexpr int kWave = 64;
+
+__device__ inline double wave_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+  return v;
+}
+
+__device__ inline float wave_sum(float v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+  return v;
+}
+
+__device__ inline double wave_max(double v) {
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, kWave));
+  return v;
+}
