@@ -1,0 +1,576 @@
+// DecentralizedClient on the GPU: the stage order of client.cpp:447-1213 (meta and seeds, local
+// QR, Givens masks, re-partition, global QR, Ralha bidiagonalization, bidiagonal SVD, combine and
+// unmask), every matrix stage through libexcalibur_amd's C ABI on device-resident data, the peer
+// exchanges on host TCP (net.cpp).
+#include "client.hpp"
+
+#include <hip/hip_runtime.h>
+#include <sys/stat.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "excalibur_hip.h"
+#include "net.hpp"
+
+namespace exc {
+
+void ck(int rc, const char* what) {
+  if (rc != EXC_OK) throw std::runtime_error(std::string(what) + ": " + exc_last_error());
+}
+
+void ckh(hipError_t e, const char* what) {
+  if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// Device buffer of doubles (or bytes), zero-initialized.
+struct DBuf {
+  double* p = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  ~DBuf() { release(); }
+  void alloc(size_t count, hipStream_t s) {
+    release();
+    n = count;
+    void* q = nullptr;
+    ck(exc_malloc(&q, std::max<size_t>(count, 1) * sizeof(double)), "device allocation");
+    p = (double*)q;
+    ckh(hipMemsetAsync(p, 0, std::max<size_t>(count, 1) * sizeof(double), s), "memset");
+  }
+  void release() {
+    if (p) exc_free(p);
+    p = nullptr;
+    n = 0;
+  }
+};
+
+// Pinned host staging buffer.
+struct HBuf {
+  double* p = nullptr;
+  size_t n = 0;
+  ~HBuf() {
+    if (p) exc_host_free(p);
+  }
+  void alloc(size_t count) {
+    if (p) exc_host_free(p);
+    void* q = nullptr;
+    ck(exc_host_alloc(&q, std::max<size_t>(count, 1) * sizeof(double)), "pinned allocation");
+    p = (double*)q;
+    n = count;
+    memset(p, 0, std::max<size_t>(count, 1) * sizeof(double));
+  }
+};
+
+struct ClientDevice {
+  hipStream_t s = nullptr;
+  DBuf X;  // raw data (m x local_n); Q_i^T of the local QR in place afterwards
+  DBuf masked, P, Vt, tau_g, Qg, F, U, alpha, beta, house, tau_h, cs, bu, bv, sigma, flip, part, FinalU, tmp,
+      FinalVT, msg, g, scal, work;
+  ~ClientDevice() {
+    if (s) hipStreamDestroy(s);
+  }
+};
+
+}  // namespace exc
+
+using exc::ck;
+using exc::ckh;
+using exc::DBuf;
+using exc::HBuf;
+using exc::now_seconds;
+
+namespace {
+
+void upload_plan(long K, long passes, int mask, unsigned seed, DBuf& pairs_cs_dev, hipStream_t s) {
+  const long half = exc_givens_half(K);
+  std::vector<int32_t> pairs((size_t)passes * half * 2);
+  std::vector<double> cs((size_t)passes * half * 2);
+  ck(exc_givens_plan(K, passes, mask, seed, pairs.data(), cs.data()), "givens plan");
+  // one buffer: pairs (as raw int32) then c/s
+  const size_t pair_words = (pairs.size() * sizeof(int32_t) + 7) / 8;
+  pairs_cs_dev.alloc(pair_words + cs.size(), s);
+  ckh(hipMemcpyAsync(pairs_cs_dev.p, pairs.data(), pairs.size() * sizeof(int32_t), hipMemcpyHostToDevice, s), "plan");
+  ckh(hipMemcpyAsync(pairs_cs_dev.p + pair_words, cs.data(), cs.size() * sizeof(double), hipMemcpyHostToDevice, s),
+      "plan");
+  ckh(hipStreamSynchronize(s), "plan upload");
+}
+
+// block_shift_mask(m, n, 32, x, mask, is_left, seed) for K <= 10000 (bbtsvd.cpp:533-546).
+void givens_mask(double* d_x, long m, long n, int mask, int is_left, unsigned seed, hipStream_t s) {
+  const long K = is_left ? m : n;
+  if (K <= 1) return;
+  if (K > 10000)
+    throw std::runtime_error("Givens masks with more than 10000 rotated lines use the virtual-matrix path, "
+                             "which this build does not implement");
+  DBuf plan;
+  upload_plan(K, 32, mask, seed, plan, s);
+  const long half = exc_givens_half(K);
+  const size_t pair_words = ((size_t)32 * half * 2 * sizeof(int32_t) + 7) / 8;
+  ck(exc_givens_apply(d_x, m, n, (const int32_t*)plan.p, plan.p + pair_words, 32, mask, is_left, s), "givens apply");
+  ckh(hipStreamSynchronize(s), "givens");
+}
+
+}  // namespace
+
+DecentralizedClient::DecentralizedClient(int cid, int nclient, int mode, size_t topk, size_t drow, size_t dcol,
+                                         int port, std::string listen_, std::string dpath_, bool memmap,
+                                         bool evaluate_, std::string log_dir_, unsigned int seed, int opt)
+    : DecentralizedClient(cid, nclient, mode, topk, drow, dcol, port, listen_, dpath_, memmap, evaluate_, log_dir_,
+                          seed, opt, ClientOptions()) {}
+
+DecentralizedClient::DecentralizedClient(int cid, int nclient, int mode, size_t topk, size_t drow, size_t dcol,
+                                         int port, std::string listen_, std::string dpath_, bool memmap,
+                                         bool evaluate_, std::string log_dir_, unsigned int seed, int opt,
+                                         const ClientOptions& options)
+    : svd_mode(mode),
+      top_k((int)topk),
+      client_id(cid),
+      num_client(nclient),
+      m(drow),
+      local_n(dcol),
+      opt_control(opt),
+      evaluate(evaluate_),
+      is_memmap(memmap),
+      listen(listen_),
+      dpath(dpath_),
+      log_dir(log_dir_),
+      base_server_port(port),
+      opt_(options),
+      seed_(seed) {
+  if (evaluate) opt_.dump_results = true;
+  mkdir(log_dir.c_str(), 0755);
+  log("Parameters: cid=" + std::to_string(cid) + " num_client=" + std::to_string(nclient) + " mode=" +
+      std::to_string(mode) + " topk=" + std::to_string((long)top_k) + " m=" + std::to_string(m) + " local_n=" +
+      std::to_string(local_n) + " port=" + std::to_string(port) + " listen=" + listen + " seed=" +
+      std::to_string(seed) + " opt=" + std::to_string(opt) + " form=" + opt_.form);
+  dev_.reset(new exc::ClientDevice());
+  ckh(hipStreamCreateWithFlags(&dev_->s, hipStreamNonBlocking), "stream");
+  load_data();
+  init_server_and_client();
+  run();
+}
+
+DecentralizedClient::~DecentralizedClient() = default;
+
+void DecentralizedClient::log(const std::string& line) {
+  std::ofstream f(log_dir + "/Client" + std::to_string(client_id) + ".log", std::ios::app);
+  f << line << "\n";
+}
+
+void DecentralizedClient::load_data() {
+  const size_t count = m * local_n;
+  hipStream_t s = dev_->s;
+  dev_->X.alloc(count, s);
+  if (opt_.data) {
+    ckh(hipMemcpyAsync(dev_->X.p, opt_.data, count * sizeof(double), hipMemcpyHostToDevice, s), "upload");
+  } else {
+    const std::string path = dpath + "/Client" + std::to_string(client_id) + ".mat";
+    FILE* f = fopen(path.c_str(), "rb");
+    if (!f) throw std::runtime_error("cannot open " + path);
+    HBuf h;
+    h.alloc(count);
+    const size_t got = fread(h.p, sizeof(double), count, f);
+    fclose(f);
+    if (got != count) throw std::runtime_error(path + " holds fewer than m * local_n doubles");
+    ckh(hipMemcpyAsync(dev_->X.p, h.p, count * sizeof(double), hipMemcpyHostToDevice, s), "upload");
+    ckh(hipStreamSynchronize(s), "upload");
+  }
+  ckh(hipStreamSynchronize(s), "upload");
+}
+
+void DecentralizedClient::init_server_and_client() {
+  net_.reset(new exc::PeerNet(client_id, num_client, listen, base_server_port, opt_.recv_timeout_s));
+  net_->connect_all(std::max(60.0, opt_.recv_timeout_s));
+}
+
+void DecentralizedClient::run() {
+  hipStream_t s = dev_->s;
+  exc::ClientDevice& D = *dev_;
+  exc::PeerNet& net = *net_;
+  const int P = num_client, cid = client_id;
+  auto sync = [&] { ckh(hipStreamSynchronize(s), "stream sync"); };
+  // srand(seed) of client.cpp:196. The reference makes no rand() call between the constructor and
+  // this point, but the HIP runtime's initialization (stream, allocations, load) may touch the C
+  // library's generator, so the seed is set here, after it, to keep the reference's draws.
+  srand(seed_);
+  log("All Clients Connected, DecFedSVD Begin");
+  const double t_run = now_seconds();
+  double t0 = t_run;
+
+  bool dense_mask_opt = true;  // client.cpp:480-488
+  if (opt_control == 0 || opt_control == 2) dense_mask_opt = false;
+  if (!dense_mask_opt) throw std::runtime_error("opt 0/2 (dense random orthogonal masks) is not supported");
+  if (svd_mode == 4) throw std::runtime_error("mode 4 (linear regression) is not supported");
+  if (opt_.form != "fp64") throw std::runtime_error("form '" + opt_.form + "' is not available in this build");
+  const bool compute_u = svd_mode != 3;
+  const bool compute_v = svd_mode != 2;
+
+  // 1) meta and seeds (client.cpp:494-531)
+  ClientMeta cm{m, local_n, cid, 0};
+  if (cid == 0) {
+    public_seed = (unsigned)rand();
+    cm.shared_seed = public_seed;
+  }
+  net.broadcast(&cm, sizeof cm);
+  const double thr = 5.0;
+  auto reduce_n = [&](size_t ln) { return ((double)(ln / m)) > thr ? std::min(ln, m) : ln; };
+  global_n = local_n;
+  reduced_global_n = reduce_n(local_n);
+  for (int j = 0; j < P; j++) {
+    if (j == cid) continue;
+    ClientMeta rm;
+    net.receive_into(j, &rm, sizeof rm);
+    local_ns[rm.cid] = rm.local_n;
+    reduced_local_ns[rm.cid] = reduce_n(rm.local_n);
+    global_n += rm.local_n;
+    reduced_global_n += reduced_local_ns[rm.cid];
+    if (rm.cid == 0) public_seed = rm.shared_seed;
+  }
+  local_ns[cid] = local_n;
+  reduced_local_ns[cid] = reduce_n(local_n);
+  for (int i = 0; i <= P; i++) {
+    n_pos[i] = 0;
+    reduced_n_pos[i] = 0;
+    for (int j = 0; j < std::min(i, P); j++) {
+      n_pos[i] += local_ns[j];
+      reduced_n_pos[i] += reduced_local_ns[j];
+    }
+  }
+  private_seed = (unsigned)rand();
+  time_init = now_seconds() - t0;
+  if (m > global_n) throw std::runtime_error("the tall-and-skinny branch (m > global_n) is not supported");
+  if (m >= global_n && compute_v)
+    throw std::runtime_error("m == global_n takes the reference's tall-and-skinny combine branch, not supported");
+  if (m < 3) throw std::runtime_error("m must be at least 3");
+
+  // 2) local QR (client.cpp:572-589)
+  const size_t rl = reduced_local_ns[cid];
+  const bool lqr = m < local_n && ((double)(local_n / m)) > thr;
+  double* masked = D.X.p;
+  if (lqr) {
+    t0 = now_seconds();
+    D.masked.alloc(m * m, s);
+    const size_t wb = exc_qr_t_workspace((int)m, (long)local_n, 1);
+    D.work.alloc((wb + 7) / 8, s);
+    ck(exc_qr_t((int)m, (long)local_n, D.X.p, (long)local_n, D.masked.p, compute_v ? D.X.p : nullptr, (long)local_n,
+                nullptr, 0, nullptr, nullptr, 0, D.work.p, D.work.n * 8, s),
+       "local QR");
+    sync();
+    masked = D.masked.p;
+    time_local_qr = now_seconds() - t0;
+    log("Local QR Costs(1) (short-and-wide) " + std::to_string(time_local_qr));
+  }
+
+  // 3) masks (client.cpp:592-605)
+  t0 = now_seconds();
+  givens_mask(masked, (long)m, (long)rl, 1, 1, public_seed, s);
+  givens_mask(masked, (long)m, (long)rl, 1, 0, private_seed, s);
+  time_apply_mask = now_seconds() - t0;
+  log("Masking Costs(1) (short-and-wide) " + std::to_string(time_apply_mask));
+
+  // 4) re-partition (client.cpp:613-703)
+  std::vector<size_t> m_part(P, m / P), m_part_pos(P + 1, 0);
+  for (int i = 0; i < (int)(m % P); i++) m_part[i] += 1;
+  for (int i = 0; i < P; i++) m_part_pos[i + 1] = m_part_pos[i] + m_part[i];
+  const size_t c = m_part[cid];
+  const size_t R = reduced_global_n;
+  t0 = now_seconds();
+  {
+    HBuf hm, hr;
+    hm.alloc(m * rl);
+    ckh(hipMemcpyAsync(hm.p, masked, m * rl * sizeof(double), hipMemcpyDeviceToHost, s), "d2h");
+    sync();
+    for (int j = 0; j < P; j++)
+      if (j != cid) net.send_to(j, hm.p + m_part_pos[j] * rl, m_part[j] * rl * sizeof(double));
+    hr.alloc(c * R);
+    for (int j = 0; j < P; j++) {
+      const size_t rj = reduced_local_ns[j], off = reduced_n_pos[j];
+      if (j == cid) {
+        for (size_t r = 0; r < c; r++) memcpy(hr.p + r * R + off, hm.p + (m_part_pos[cid] + r) * rl, rj * 8);
+      } else {
+        std::vector<char> blk = net.receive(j);
+        if (blk.size() != c * rj * 8) throw std::runtime_error("re-partition block of unexpected size");
+        const double* b = (const double*)blk.data();
+        for (size_t r = 0; r < c; r++) memcpy(hr.p + r * R + off, b + r * rj, rj * 8);
+      }
+    }
+    D.P.alloc(c * R, s);
+    ckh(hipMemcpyAsync(D.P.p, hr.p, c * R * sizeof(double), hipMemcpyHostToDevice, s), "h2d");
+    sync();
+  }
+  if (!lqr) D.X.release();  // the masked raw data has been re-partitioned
+  if (lqr) D.masked.release();
+  time_exchange_low_dim = now_seconds() - t0;
+  net.add_comm_seconds(time_exchange_low_dim);
+
+  // 5) global QR (client.cpp:706-798)
+  t0 = now_seconds();
+  D.F.alloc(m * c, s);
+  if (R > m) {
+    D.Vt.alloc(m * R, s);
+    D.tau_g.alloc(m, s);
+    DBuf rt, msgd;
+    for (int i = 0; i < P; i++) {
+      const long ci = (long)m_part[i], pos = (long)m_part_pos[i], L = (long)R - pos;
+      const long len = exc_reflector_message_len(ci, (long)R, pos);
+      std::vector<int> grp;
+      if (compute_v) for (int j = 0; j < P; j++) grp.push_back(j);
+      else for (int j = i; j < P; j++) grp.push_back(j);
+      if (i == cid) {
+        rt.alloc((size_t)ci * ci, s);
+        const size_t wb = exc_qr_t_workspace((int)ci, L, 0);
+        D.work.alloc((wb + 7) / 8, s);
+        ck(exc_qr_t((int)ci, L, D.P.p + pos, (long)R, rt.p, nullptr, 0, D.Vt.p + pos * (long)R + pos, (long)R,
+                    D.tau_g.p + pos, nullptr, 1, D.work.p, D.work.n * 8, s),
+           "global QR panel");
+        ck(exc_copy2d(ci, ci, rt.p, ci, D.P.p + pos, (long)R, s), "panel R");
+        ck(exc_fill2d(ci, L - ci, 0.0, D.P.p + pos + ci, (long)R, s), "panel zero");
+        msgd.alloc(len, s);
+        ck(exc_pack_reflectors(ci, (long)R, pos, D.Vt.p, (long)R, D.tau_g.p, msgd.p, s), "pack");
+        std::vector<char> hmsg(len * sizeof(double));
+        ckh(hipMemcpyAsync(hmsg.data(), msgd.p, len * sizeof(double), hipMemcpyDeviceToHost, s), "d2h");
+        sync();
+        if ((int)grp.size() > 1) net.all_gather(hmsg, len * sizeof(double), grp, cid);
+      } else if (compute_v || cid > i) {
+        std::vector<char> hmsg;
+        net.all_gather(hmsg, len * sizeof(double), grp, i);
+        if ((long)hmsg.size() != len * 8) throw std::runtime_error("global QR message of unexpected size");
+        msgd.alloc(len, s);
+        ckh(hipMemcpyAsync(msgd.p, hmsg.data(), len * sizeof(double), hipMemcpyHostToDevice, s), "h2d");
+        ck(exc_unpack_reflectors(ci, (long)R, pos, msgd.p, D.Vt.p, (long)R, D.tau_g.p, s), "unpack");
+        if (cid > i) {
+          const size_t wb = exc_wy_workspace((int)ci, L, (long)c);
+          D.work.alloc((wb + 7) / 8, s);
+          ck(exc_apply_q_right((long)c, L, (int)ci, D.P.p + pos, (long)R, D.Vt.p + pos * (long)R + pos, (long)R,
+                               D.tau_g.p + pos, D.work.p, D.work.n * 8, s),
+             "apply panel reflectors");
+        }
+        sync();
+      }
+    }
+    if (compute_v) {
+      D.Qg.alloc(m * R, s);
+      const size_t wb = exc_wy_workspace((int)m, (long)R, (long)m);
+      D.work.alloc((wb + 7) / 8, s);
+      ck(exc_orgqr_t((int)m, (long)R, D.Vt.p, (long)R, D.tau_g.p, D.Qg.p, (long)R, D.work.p, D.work.n * 8, s),
+         "global Q");
+    }
+    ck(exc_transpose((long)c, (long)m, D.P.p, (long)R, D.F.p, (long)c, s), "shrink");
+    sync();
+    D.Vt.release();
+  } else {
+    ck(exc_transpose((long)c, (long)R, D.P.p, (long)R, D.F.p, (long)c, s), "transpose");
+    sync();
+  }
+  D.P.release();
+  time_global_qr = now_seconds() - t0;
+
+  // 6) Ralha bidiagonalization (client.cpp:800-910) + lower to upper (947-954)
+  t0 = now_seconds();
+  D.U.alloc(m * c, s);
+  D.alpha.alloc(m, s);
+  D.beta.alloc(m, s);
+  D.house.alloc(m * m, s);
+  D.tau_h.alloc(m, s);
+  D.cs.alloc(2 * m, s);
+  D.msg.alloc(m + 3, s);
+  D.g.alloc(m + 3, s);
+  D.scal.alloc(4, s);
+  sync();
+  {
+    HBuf hb;
+    hb.alloc(m + 3);
+    double ar = 0.0;
+    auto allreduce_dev = [&](double* dsrc, double* ddst, long n) {
+      ckh(hipMemcpyAsync(hb.p, dsrc, n * sizeof(double), hipMemcpyDeviceToHost, s), "d2h");
+      sync();
+      const double ta = now_seconds();
+      net.ring_all_reduce_sum(hb.p, n);
+      ar += now_seconds() - ta;
+      ckh(hipMemcpyAsync(ddst, hb.p, n * sizeof(double), hipMemcpyHostToDevice, s), "h2d");
+    };
+    ck(exc_ralha_tail_update(-1, (int)c, D.F.p, D.U.p, nullptr, D.beta.p, D.scal.p, s), "ralha init");
+    allreduce_dev(D.scal.p, D.scal.p + 1, 1);
+    ck(exc_ralha_tail_scale(0, (int)c, D.F.p, D.U.p, D.scal.p + 1, D.alpha.p, s), "ralha init");
+    for (int i = 0; i + 2 < (int)m; i++) {
+      ck(exc_ralha_msg(i, (int)m, (int)c, D.F.p, D.U.p, D.msg.p, s), "ralha msg");
+      allreduce_dev(D.msg.p, D.g.p, (long)m + 3 - i);
+      ck(exc_ralha_apply(i, (int)m, (int)c, D.F.p, D.U.p, D.g.p, D.house.p, D.tau_h.p, D.alpha.p, D.beta.p, s),
+         "ralha apply");
+    }
+    for (int i = (int)m - 2; i < (int)m; i++) {
+      ck(exc_ralha_dot(i, (int)c, D.F.p, D.U.p, D.scal.p, s), "ralha dot");
+      allreduce_dev(D.scal.p, D.scal.p + 1, 1);
+      ck(exc_ralha_tail_update(i, (int)c, D.F.p, D.U.p, D.scal.p + 1, D.beta.p, D.scal.p + 2, s), "ralha tail");
+      allreduce_dev(D.scal.p + 2, D.scal.p + 3, 1);
+      ck(exc_ralha_tail_scale(i, (int)c, D.F.p, D.U.p, D.scal.p + 3, D.alpha.p, s), "ralha tail");
+    }
+    ck(exc_lower_to_upper((int)m, D.alpha.p, D.beta.p, D.cs.p, s), "lower to upper");
+    sync();
+    time_ralha_allreduce = ar;
+  }
+  D.F.release();
+  time_bi_diag = now_seconds() - t0;
+  log("One-side Householder reflector Costs " + std::to_string(time_bi_diag));
+
+  // 7) bidiagonal SVD (client.cpp:963-996)
+  t0 = now_seconds();
+  {
+    DBuf b2;
+    b2.alloc(2, s);
+    DBuf cnt2;
+    cnt2.alloc(1, s);
+    ck(exc_bisection_bound((int)m, D.alpha.p, D.beta.p, b2.p, s), "bound");
+    ck(exc_sturm_count((int)m, D.alpha.p, D.beta.p, b2.p, 2, (int*)cnt2.p, s), "count");
+    int hc[2];
+    ckh(hipMemcpyAsync(hc, cnt2.p, sizeof hc, hipMemcpyDeviceToHost, s), "d2h");
+    sync();
+    sigma_count = hc[0] - hc[1];
+    log("Ori Sigma Count = " + std::to_string(sigma_count));
+    if (top_k > 0) sigma_count = std::min(top_k, sigma_count);
+    log("Adj Sigma Count = " + std::to_string(sigma_count));
+  }
+  const int k = sigma_count;
+  D.bu.alloc(m * (size_t)k, s);
+  D.bv.alloc(m * (size_t)k, s);
+  D.sigma.alloc(std::max(k, 1), s);
+  D.flip.alloc(std::max(k, 1), s);
+  {
+    const size_t wb = exc_bidiag_svd_workspace((int)m, k);
+    D.work.alloc((wb + 7) / 8, s);
+    const int rc = exc_bidiag_svd((int)m, D.alpha.p, D.beta.p, D.bu.p, D.bv.p, D.sigma.p, k, nullptr, D.work.p,
+                                  D.work.n * 8, s);
+    ck(rc, "bidiagonal SVD");
+    ck(exc_sign_normalize((int)m, k, D.bu.p, D.bv.p, D.flip.p, s), "sign rule");
+    sync();
+  }
+  time_bsvd = now_seconds() - t0;
+
+  // 8) combine and unmask (client.cpp:998-1199)
+  t0 = now_seconds();
+  if (compute_u) {
+    D.part.alloc((size_t)k * c, s);
+    ck(exc_dgemm(0, 0, k, (long)c, (long)m, 1.0, D.bv.p, (long)m, D.U.p, (long)c, 0.0, D.part.p, (long)c, nullptr, 0,
+                 s),
+       "part final U");
+    D.FinalU.alloc(m * (size_t)k, s);
+    ck(exc_transpose(k, (long)c, D.part.p, (long)c, D.FinalU.p + m_part_pos[cid] * (size_t)k, k, s), "final U");
+    // ring all-gather of the row blocks of FinalU (client.cpp:1062-1096)
+    const double tc = now_seconds();
+    HBuf hU;
+    hU.alloc(m * (size_t)k);
+    ckh(hipMemcpyAsync(hU.p + m_part_pos[cid] * k, D.FinalU.p + m_part_pos[cid] * k, c * (size_t)k * 8,
+                       hipMemcpyDeviceToHost, s),
+        "d2h");
+    sync();
+    const int next = (cid + 1) % P, front = (cid - 1 + P) % P;
+    for (int st = 0; st < P - 1; st++) {
+      int di = (cid - st + P) % P;
+      net.send_to(next, hU.p + m_part_pos[di] * k, m_part[di] * (size_t)k * 8);
+      di = (di - 1 + P) % P;
+      net.receive_into(front, hU.p + m_part_pos[di] * k, m_part[di] * (size_t)k * 8);
+    }
+    ckh(hipMemcpyAsync(D.FinalU.p, hU.p, m * (size_t)k * 8, hipMemcpyHostToDevice, s), "h2d");
+    sync();
+    net.add_comm_seconds(now_seconds() - tc);
+    D.part.release();
+    givens_mask(D.FinalU.p, (long)m, k, 0, 1, public_seed, s);
+  }
+  D.U.release();
+  if (compute_v) {
+    ck(exc_rot_back((int)m, k, D.bu.p, D.cs.p, s), "back rotation");
+    {
+      const size_t wb = exc_ormqr_house_workspace((int)m, k);
+      D.work.alloc((wb + 7) / 8, s);
+      ck(exc_ormqr_house_ln((int)m, k, D.house.p, D.tau_h.p, D.bu.p, k, D.work.p, D.work.n * 8, s), "ormqr house");
+    }
+    D.tmp.alloc((size_t)k * rl, s);
+    ck(exc_dgemm(1, 0, k, (long)rl, (long)m, 1.0, D.bu.p, k, D.Qg.p + reduced_n_pos[cid], (long)R, 0.0, D.tmp.p,
+                 (long)rl, nullptr, 0, s),
+       "combine V");
+    sync();
+    D.Qg.release();
+    givens_mask(D.tmp.p, k, (long)rl, 0, 0, private_seed, s);
+    if (lqr) {
+      D.FinalVT.alloc((size_t)k * local_n, s);
+      ck(exc_dgemm(0, 0, k, (long)local_n, (long)rl, 1.0, D.tmp.p, (long)rl, D.X.p, (long)local_n, 0.0, D.FinalVT.p,
+                   (long)local_n, nullptr, 0, s),
+         "final V");
+      sync();
+      D.X.release();
+      D.tmp.release();
+    }
+  }
+  sync();
+  time_combine_svd_remove_mask = now_seconds() - t0;
+  time_total = now_seconds() - t_run;
+  time_communication = net.comm_seconds();
+
+  log("Finish DecFedSVD");
+  log("DecFedSVD totally Costs " + std::to_string(time_total));
+  log("Overall Communication Cost " + std::to_string(time_communication));
+  log("time_init=" + std::to_string(time_init));
+  log("time_local_qr=" + std::to_string(time_local_qr));
+  log("time_global_qr=" + std::to_string(time_global_qr));
+  log("time_apply_mask=" + std::to_string(time_apply_mask));
+  log("time_exchange_low_dim=" + std::to_string(time_exchange_low_dim));
+  log("time_bi_diag=" + std::to_string(time_bi_diag));
+  log("time_bsvd=" + std::to_string(time_bsvd));
+  log("time_combine_svd_remove_mask=" + std::to_string(time_combine_svd_remove_mask));
+  log("time_ralha_allreduce=" + std::to_string(time_ralha_allreduce));
+
+  if (opt_.keep_results || opt_.dump_results) {
+    sigma.assign(k, 0.0);
+    ckh(hipMemcpyAsync(sigma.data(), D.sigma.p, k * 8, hipMemcpyDeviceToHost, s), "d2h");
+    if (compute_u) {
+      FinalU.assign(m * (size_t)k, 0.0);
+      ckh(hipMemcpyAsync(FinalU.data(), D.FinalU.p, FinalU.size() * 8, hipMemcpyDeviceToHost, s), "d2h");
+    }
+    if (compute_v) {
+      const double* src = lqr ? D.FinalVT.p : D.tmp.p;
+      FinalVT.assign((size_t)k * local_n, 0.0);
+      ckh(hipMemcpyAsync(FinalVT.data(), src, FinalVT.size() * 8, hipMemcpyDeviceToHost, s), "d2h");
+    }
+    sync();
+    if (opt_.dump_results) {
+      auto dump = [&](const std::string& name, const std::vector<double>& v) {
+        FILE* f = fopen((log_dir + "/Client" + std::to_string(cid) + "_" + name + ".bin").c_str(), "wb");
+        if (f) {
+          fwrite(v.data(), sizeof(double), v.size(), f);
+          fclose(f);
+        }
+      };
+      dump("sigma", sigma);
+      if (compute_u) dump("FinalU", FinalU);
+      if (compute_v) dump("FinalVT", FinalVT);
+      // the bidiagonal and its vectors, for stage-by-stage comparison
+      std::vector<double> a(m), b(m - 1), u(m * (size_t)k), v(m * (size_t)k);
+      ckh(hipMemcpyAsync(a.data(), D.alpha.p, m * 8, hipMemcpyDeviceToHost, s), "d2h");
+      ckh(hipMemcpyAsync(b.data(), D.beta.p, (m - 1) * 8, hipMemcpyDeviceToHost, s), "d2h");
+      ckh(hipMemcpyAsync(v.data(), D.bv.p, v.size() * 8, hipMemcpyDeviceToHost, s), "d2h");
+      sync();
+      dump("alpha", a);
+      dump("beta", b);
+      dump("bidiag_v", v);
+    }
+  }
+  char json[1024];
+  snprintf(json, sizeof json,
+           "{\"cid\": %d, \"sigma_count\": %d, \"total\": %.6f, \"communication\": %.6f, \"time_init\": %.6f, "
+           "\"time_local_qr\": %.6f, \"time_global_qr\": %.6f, \"time_apply_mask\": %.6f, "
+           "\"time_exchange_low_dim\": %.6f, \"time_bi_diag\": %.6f, \"time_ralha_allreduce\": %.6f, "
+           "\"time_bsvd\": %.6f, \"time_combine_svd_remove_mask\": %.6f}",
+           cid, k, time_total, time_communication, time_init, time_local_qr, time_global_qr, time_apply_mask,
+           time_exchange_low_dim, time_bi_diag, time_ralha_allreduce, time_bsvd, time_combine_svd_remove_mask);
+  printf("EXC_RESULT %s\n", json);
+  fflush(stdout);
+}

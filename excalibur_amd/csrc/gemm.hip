@@ -1,0 +1,298 @@
+// Dense GEMM on the gfx950 matrix cores: C = alpha * op(A) op(B) + beta * C, row-major, FP64
+// (v_mfma_f64_16x16x4_f64) and FP32 (v_mfma_f32_16x16x4_f32).
+//
+// Workgroup tile 128 x 128, four wavefronts in a 2 x 2 arrangement, each wavefront a 64 x 64
+// block of 4 x 4 MFMA tiles; K is staged through LDS in chunks of 16 with register
+// double-buffering of the next chunk. Options used by the factorizations: split-K (partial tiles
+// summed in a fixed order by a second kernel, so results do not depend on scheduling), skipping
+// the zero K range of a triangular operand, and computing only the lower tiles of a symmetric
+// product (mirrored into the upper half by the reduction).
+#include "exc_common.hpp"
+#include "gemm.hpp"
+
+namespace {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+template <typename T>
+struct Mfma;
+template <>
+struct Mfma<double> {
+  typedef d4 acc_t;
+  __device__ static inline acc_t step(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  // result element v of lane l sits in row (l/16) + 4v, column l%16
+  __device__ static inline int row(int lk, int v) { return lk + 4 * v; }
+};
+template <>
+struct Mfma<float> {
+  typedef f4 acc_t;
+  __device__ static inline acc_t step(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  // result element v of lane l sits in row 4(l/16) + v, column l%16
+  __device__ static inline int row(int lk, int v) { return 4 * lk + v; }
+};
+
+constexpr int BM = 128, BN = 128, BK = 16, NT = 256;
+
+struct GemmArgs {
+  int ta, tb;
+  long M, N, K;
+  const void* A;
+  long lda;
+  const void* B;
+  long ldb;
+  void* C;
+  long ldc;
+  double alpha, beta;
+  int tri;        // EXC_TRI_* restriction of the K range per tile
+  int lower_only; // compute only tiles with tile_row >= tile_col
+  long kchunk;    // split-K chunk (multiple of BK); 0 = no split
+  void* part;     // split-K partials: splits x M x N (type T)
+};
+
+// K range [k0, k1) that can contribute to output rows [i0, i0+BM) / columns [j0, j0+BN).
+__device__ inline void k_range(const GemmArgs& g, long i0, long j0, long& k0, long& k1) {
+  k0 = 0;
+  k1 = g.K;
+  switch (g.tri) {
+    case EXC_TRI_A_LOWER: k1 = min(k1, i0 + BM); break;  // op(A)[i][k] = 0 for k > i
+    case EXC_TRI_A_UPPER: k0 = max(k0, i0); break;       // op(A)[i][k] = 0 for k < i
+    case EXC_TRI_B_LOWER: k0 = max(k0, j0); break;       // op(B)[k][j] = 0 for k < j
+    case EXC_TRI_B_UPPER: k1 = min(k1, j0 + BN); break;  // op(B)[k][j] = 0 for k > j
+    default: break;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(NT) void k_gemm(GemmArgs g) {
+  using acc_t = typename Mfma<T>::acc_t;
+  __shared__ T As[2][BK][BM + 4];
+  __shared__ T Bs[2][BK][BN + 4];
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6, lane = tid & 63;
+  const long tile_m = blockIdx.y, tile_n = blockIdx.x;
+  if (g.lower_only && tile_n > tile_m) return;
+  const long i0 = tile_m * BM, j0 = tile_n * BN;
+  long k0, k1;
+  k_range(g, i0, j0, k0, k1);
+  if (g.kchunk > 0) {
+    const long ks = (long)blockIdx.z * g.kchunk;
+    k0 = max(k0, ks);
+    k1 = min(k1, ks + g.kchunk);
+  }
+  const T* __restrict__ A = (const T*)g.A;
+  const T* __restrict__ B = (const T*)g.B;
+  const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+  acc_t acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int b = 0; b < 4; b++) acc[a][b] = acc_t{0, 0, 0, 0};
+
+  // global -> register staging: 8 elements of A and 8 of B per thread per chunk
+  T ra[8], rb[8];
+  auto load_chunk = [&](long kb) {
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const int e = tid + q * NT;  // 0..2047
+      // A tile element (i in 0..127, k in 0..15)
+      int ii, kk;
+      if (!g.ta) { ii = e >> 4; kk = e & 15; }   // rows of A contiguous in k
+      else { kk = e >> 7; ii = e & 127; }        // A stored K x M: contiguous in i
+      const long gi = i0 + ii, gk = kb + kk;
+      T v = T(0);
+      if (gi < g.M && gk < k1 && gk >= k0) v = g.ta ? A[gk * g.lda + gi] : A[gi * g.lda + gk];
+      if ((g.tri == EXC_TRI_A_LOWER && gk > gi) || (g.tri == EXC_TRI_A_UPPER && gk < gi)) v = T(0);
+      ra[q] = v;
+      int jj, kb2;
+      if (!g.tb) { kb2 = e >> 7; jj = e & 127; }  // B stored K x N: contiguous in j
+      else { jj = e >> 4; kb2 = e & 15; }         // B stored N x K: contiguous in k
+      const long gj = j0 + jj, gk2 = kb + kb2;
+      T w = T(0);
+      if (gj < g.N && gk2 < k1 && gk2 >= k0) w = g.tb ? B[gj * g.ldb + gk2] : B[gk2 * g.ldb + gj];
+      if ((g.tri == EXC_TRI_B_LOWER && gk2 < gj) || (g.tri == EXC_TRI_B_UPPER && gk2 > gj)) w = T(0);
+      rb[q] = w;
+    }
+  };
+  auto store_chunk = [&](int buf) {
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const int e = tid + q * NT;
+      int ii, kk;
+      if (!g.ta) { ii = e >> 4; kk = e & 15; }
+      else { kk = e >> 7; ii = e & 127; }
+      As[buf][kk][ii] = ra[q];
+      int jj, kb2;
+      if (!g.tb) { kb2 = e >> 7; jj = e & 127; }
+      else { jj = e >> 4; kb2 = e & 15; }
+      Bs[buf][kb2][jj] = rb[q];
+    }
+  };
+
+  const long kstart = (k0 / BK) * BK;
+  int buf = 0;
+  if (kstart < k1) {
+    load_chunk(kstart);
+    store_chunk(0);
+  }
+  __syncthreads();
+  const int li = lane & 15, lk = lane >> 4;
+  for (long kb = kstart; kb < k1; kb += BK) {
+    const bool more = kb + BK < k1;
+    if (more) load_chunk(kb + BK);
+#pragma unroll
+    for (int ks = 0; ks < BK; ks += 4) {
+      T af[4], bf[4];
+#pragma unroll
+      for (int a = 0; a < 4; a++) af[a] = As[buf][ks + lk][wm + a * 16 + li];
+#pragma unroll
+      for (int b = 0; b < 4; b++) bf[b] = Bs[buf][ks + lk][wn + b * 16 + li];
+#pragma unroll
+      for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) acc[a][b] = Mfma<T>::step(af[a], bf[b], acc[a][b]);
+    }
+    if (more) {
+      store_chunk(buf ^ 1);
+      __syncthreads();
+      buf ^= 1;
+    }
+  }
+  // epilogue: operands were supplied as A[lane%16][lane/16], B[lane/16][lane%16]
+  T* C = g.kchunk > 0 ? (T*)g.part + (size_t)blockIdx.z * g.M * g.N : (T*)g.C;
+  const long ldc = g.kchunk > 0 ? g.N : g.ldc;
+  const bool direct = g.kchunk == 0;
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int b = 0; b < 4; b++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const long gi = i0 + wm + a * 16 + Mfma<T>::row(lk, v);
+        const long gj = j0 + wn + b * 16 + li;
+        if (gi < g.M && gj < g.N) {
+          const T val = acc[a][b][v];
+          if (direct) {
+            T out = (T)g.alpha * val;
+            if (g.beta != 0.0) out += (T)g.beta * C[gi * ldc + gj];
+            C[gi * ldc + gj] = out;
+          } else {
+            C[gi * ldc + gj] = val;
+          }
+        }
+      }
+}
+
+// Sums split-K partials in split order, applies alpha/beta, and mirrors lower tiles when asked.
+template <typename T>
+__global__ __launch_bounds__(256) void k_splitk_reduce(GemmArgs g, int splits) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long total = g.M * g.N;
+  if (idx >= total) return;
+  const long i = idx / g.N, j = idx - i * g.N;
+  if (g.lower_only && (j / BN) > (i / BM)) return;
+  const T* P = (const T*)g.part;
+  T s = T(0);
+  for (int z = 0; z < splits; z++) s += P[(size_t)z * total + idx];
+  T* C = (T*)g.C;
+  T out = (T)g.alpha * s;
+  if (g.beta != 0.0) out += (T)g.beta * C[i * g.ldc + j];
+  C[i * g.ldc + j] = out;
+}
+
+// Copies the lower triangle onto the upper one (square C).
+template <typename T>
+__global__ __launch_bounds__(256) void k_mirror_lower(long n, T* C, long ldc) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n * n) return;
+  const long i = idx / n, j = idx - i * n;
+  if (j > i) C[i * ldc + j] = C[j * ldc + i];
+}
+
+template <typename T>
+int gemm_impl(int ta, int tb, long M, long N, long K, double alpha, const T* A, long lda, const T* B, long ldb,
+              double beta, T* C, long ldc, const ExcGemmOpts& o, hipStream_t s) {
+  EXC_REQUIRE(M >= 0 && N >= 0 && K >= 0 && C, EXC_ERR_ARG, "exc_gemm: bad shape");
+  if (M == 0 || N == 0) return EXC_OK;
+  GemmArgs g{};
+  g.ta = ta; g.tb = tb; g.M = M; g.N = N; g.K = K;
+  g.A = A; g.lda = lda; g.B = B; g.ldb = ldb; g.C = C; g.ldc = ldc;
+  g.alpha = alpha; g.beta = beta; g.tri = o.tri; g.lower_only = o.lower_only;
+  const long tm = (M + BM - 1) / BM, tn = (N + BN - 1) / BN;
+  long tiles = o.lower_only ? tm * (tm + 1) / 2 : tm * tn;
+  int splits = 1;
+  if (o.work && K > 4 * BK) {
+    // split K until about two workgroups per CU
+    long want = (512 + tiles - 1) / tiles;
+    long maxs = K / (4 * BK);
+    long sp = std::min(want, maxs);
+    size_t need = (size_t)sp * M * N * sizeof(T);
+    while (sp > 1 && need > o.work_bytes) { sp--; need = (size_t)sp * M * N * sizeof(T); }
+    splits = (int)std::max(1L, sp);
+  }
+  if (K == 0) splits = 1;
+  if (splits > 1) {
+    long chunk = (K + splits - 1) / splits;
+    chunk = ((chunk + BK - 1) / BK) * BK;
+    splits = (int)((K + chunk - 1) / chunk);
+    g.kchunk = chunk;
+    g.part = o.work;
+  }
+  dim3 grid((unsigned)tn, (unsigned)tm, (unsigned)splits);
+  hipLaunchKernelGGL(k_gemm<T>, grid, dim3(NT), 0, s, g);
+  EXC_CHECK_HIP(hipGetLastError());
+  if (splits > 1) {
+    hipLaunchKernelGGL(k_splitk_reduce<T>, dim3(exc_detail::ceil_div(M * N, 256)), dim3(256), 0, s, g, splits);
+    EXC_CHECK_HIP(hipGetLastError());
+  }
+  if (o.lower_only && o.mirror) {
+    hipLaunchKernelGGL(k_mirror_lower<T>, dim3(exc_detail::ceil_div(M * M, 256)), dim3(256), 0, s, M, C, ldc);
+    EXC_CHECK_HIP(hipGetLastError());
+  }
+  return EXC_OK;
+}
+
+}  // namespace
+
+int exc_gemm_opts(int ta, int tb, long M, long N, long K, double alpha, const double* A, long lda, const double* B,
+                  long ldb, double beta, double* C, long ldc, const ExcGemmOpts& o, hipStream_t s) {
+  return gemm_impl<double>(ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, o, s);
+}
+
+int exc_sgemm_opts(int ta, int tb, long M, long N, long K, double alpha, const float* A, long lda, const float* B,
+                   long ldb, double beta, float* C, long ldc, const ExcGemmOpts& o, hipStream_t s) {
+  return gemm_impl<float>(ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, o, s);
+}
+
+extern "C" {
+
+size_t exc_gemm_workspace(long M, long N, long K) {
+  (void)K;
+  return (size_t)8 * M * N * sizeof(double);
+}
+
+int exc_dgemm(int trans_a, int trans_b, long M, long N, long K, double alpha, const double* d_a, long lda,
+              const double* d_b, long ldb, double beta, double* d_c, long ldc, void* d_work, size_t work_bytes,
+              exc_stream_t stream) {
+  ExcGemmOpts o;
+  o.work = d_work;
+  o.work_bytes = work_bytes;
+  return gemm_impl<double>(trans_a, trans_b, M, N, K, alpha, d_a, lda, d_b, ldb, beta, d_c, ldc, o,
+                           exc_detail::stream(stream));
+}
+
+int exc_sgemm(int trans_a, int trans_b, long M, long N, long K, double alpha, const float* d_a, long lda,
+              const float* d_b, long ldb, double beta, float* d_c, long ldc, void* d_work, size_t work_bytes,
+              exc_stream_t stream) {
+  ExcGemmOpts o;
+  o.work = d_work;
+  o.work_bytes = work_bytes;
+  return gemm_impl<float>(trans_a, trans_b, M, N, K, alpha, d_a, lda, d_b, ldb, beta, d_c, ldc, o,
+                          exc_detail::stream(stream));
+}
+
+}  // extern "C"

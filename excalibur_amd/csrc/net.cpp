@@ -1,0 +1,297 @@
+// Host TCP transport between peer processes (client.cpp:19-411 behaviour, rewritten): one
+// listening socket per peer, one outgoing connection to every other peer, a reader thread per
+// incoming connection that queues whole frames, and the reference's collectives on top.
+#include "net.hpp"
+
+#include <arpa/inet.h>
+#include <netinet/in.h>
+#include <netinet/tcp.h>
+#include <sys/socket.h>
+#include <unistd.h>
+
+#include <chrono>
+#include <cstring>
+#include <stdexcept>
+
+namespace exc {
+
+double now_seconds() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+namespace {
+
+constexpr size_t kMaxChunk = size_t(1) << 30;  // client.cpp:13
+
+void write_all(int fd, const void* p, size_t n) {
+  const char* c = (const char*)p;
+  while (n > 0) {
+    ssize_t k = ::send(fd, c, std::min(n, kMaxChunk), MSG_NOSIGNAL);
+    if (k < 0) {
+      if (errno == EINTR) continue;
+      throw std::runtime_error(std::string("send failed: ") + strerror(errno));
+    }
+    c += k;
+    n -= (size_t)k;
+  }
+}
+
+bool read_all(int fd, void* p, size_t n) {
+  char* c = (char*)p;
+  while (n > 0) {
+    ssize_t k = ::recv(fd, c, std::min(n, kMaxChunk), MSG_WAITALL);
+    if (k == 0) return false;
+    if (k < 0) {
+      if (errno == EINTR) continue;
+      return false;
+    }
+    c += k;
+    n -= (size_t)k;
+  }
+  return true;
+}
+
+void tune(int fd) {
+  int one = 1;
+  setsockopt(fd, IPPROTO_TCP, TCP_NODELAY, &one, sizeof one);
+  int buf = 8 << 20;
+  setsockopt(fd, SOL_SOCKET, SO_SNDBUF, &buf, sizeof buf);
+  setsockopt(fd, SOL_SOCKET, SO_RCVBUF, &buf, sizeof buf);
+}
+
+std::string peer_ip(const std::string& base, int j) {
+  size_t dot = base.rfind('.');
+  int last = std::stoi(base.substr(dot + 1));
+  return base.substr(0, dot + 1) + std::to_string(last + j);
+}
+
+}  // namespace
+
+PeerNet::PeerNet(int cid, int P, const std::string& listen, int base_port, double timeout_s)
+    : cid_(cid), P_(P), listen_(listen), base_port_(base_port), timeout_s_(timeout_s) {
+  out_fd_.assign(P, -1);
+  for (int i = 0; i < P; i++) inbox_.push_back(new Inbox());
+}
+
+PeerNet::~PeerNet() {
+  stop_ = true;
+  for (int fd : out_fd_)
+    if (fd >= 0) ::shutdown(fd, SHUT_RDWR), ::close(fd);
+  if (listen_fd_ >= 0) ::shutdown(listen_fd_, SHUT_RDWR), ::close(listen_fd_);
+  {
+    std::lock_guard<std::mutex> g(mu_);
+    for (int fd : in_fds_) ::shutdown(fd, SHUT_RDWR);
+  }
+  if (acceptor_.joinable()) acceptor_.join();
+  for (auto& t : readers_)
+    if (t.joinable()) t.join();
+  {
+    std::lock_guard<std::mutex> g(mu_);
+    for (int fd : in_fds_) ::close(fd);
+  }
+  for (auto* b : inbox_) delete b;
+}
+
+void PeerNet::accept_loop() {
+  for (int k = 0; k < P_ - 1 && !stop_; k++) {
+    sockaddr_in a{};
+    socklen_t len = sizeof a;
+    int fd = ::accept(listen_fd_, (sockaddr*)&a, &len);
+    if (fd < 0) {
+      if (stop_) return;
+      k--;
+      continue;
+    }
+    tune(fd);
+    int from = -1;
+    if (!read_all(fd, &from, sizeof from) || from < 0 || from >= P_) {
+      ::close(fd);
+      k--;
+      continue;
+    }
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      in_fds_.push_back(fd);
+      readers_.emplace_back(&PeerNet::reader_loop, this, fd, from);
+    }
+    connected_in_++;
+  }
+}
+
+void PeerNet::reader_loop(int fd, int from) {
+  Inbox& box = *inbox_[from];
+  while (!stop_) {
+    uint64_t size = 0;
+    if (!read_all(fd, &size, sizeof size)) break;
+    std::vector<char> msg(size);
+    if (size && !read_all(fd, msg.data(), size)) break;
+    {
+      std::lock_guard<std::mutex> g(box.mu);
+      box.q.push_back(std::move(msg));
+      box.n++;
+    }
+    box.cv.notify_one();
+  }
+  std::lock_guard<std::mutex> g(box.mu);
+  box.closed = true;
+  box.cv.notify_all();
+}
+
+void PeerNet::connect_all(double timeout_s) {
+  listen_fd_ = ::socket(AF_INET, SOCK_STREAM, 0);
+  if (listen_fd_ < 0) throw std::runtime_error("socket failed");
+  int one = 1;
+  setsockopt(listen_fd_, SOL_SOCKET, SO_REUSEADDR, &one, sizeof one);
+  sockaddr_in a{};
+  a.sin_family = AF_INET;
+  a.sin_port = htons(base_port_ + cid_);
+  a.sin_addr.s_addr = INADDR_ANY;
+  if (::bind(listen_fd_, (sockaddr*)&a, sizeof a) != 0)
+    throw std::runtime_error("bind failed on port " + std::to_string(base_port_ + cid_) + ": " + strerror(errno));
+  if (::listen(listen_fd_, 128) != 0) throw std::runtime_error("listen failed");
+  acceptor_ = std::thread(&PeerNet::accept_loop, this);
+  const double t0 = now_seconds();
+  for (int j = 0; j < P_; j++) {
+    if (j == cid_) continue;
+    sockaddr_in s{};
+    s.sin_family = AF_INET;
+    s.sin_port = htons(base_port_ + j);
+    s.sin_addr.s_addr = inet_addr(peer_ip(listen_, j).c_str());
+    while (true) {
+      int fd = ::socket(AF_INET, SOCK_STREAM, 0);
+      if (::connect(fd, (sockaddr*)&s, sizeof s) == 0) {
+        tune(fd);
+        out_fd_[j] = fd;
+        write_all(fd, &cid_, sizeof cid_);
+        break;
+      }
+      ::close(fd);
+      if (now_seconds() - t0 > timeout_s) throw std::runtime_error("could not connect to peer " + std::to_string(j));
+      usleep(2000);
+    }
+  }
+  while (connected_in_.load() < P_ - 1) {
+    if (now_seconds() - t0 > timeout_s) throw std::runtime_error("peers did not connect in time");
+    usleep(200);
+  }
+}
+
+void PeerNet::send_to(int peer, const void* buf, size_t bytes) {
+  uint64_t size = bytes;
+  write_all(out_fd_[peer], &size, sizeof size);
+  if (bytes) write_all(out_fd_[peer], buf, bytes);
+}
+
+std::vector<char> PeerNet::receive(int peer) {
+  Inbox& box = *inbox_[peer];
+  // short spin first: per-column all-reduce messages arrive within microseconds
+  const double t0 = now_seconds();
+  while (box.n.load(std::memory_order_acquire) == 0 && now_seconds() - t0 < 2e-4) {
+  }
+  std::unique_lock<std::mutex> lk(box.mu);
+  auto ready = [&] { return !box.q.empty() || box.closed; };
+  if (timeout_s_ > 0) {
+    if (!box.cv.wait_for(lk, std::chrono::duration<double>(timeout_s_), ready))
+      throw std::runtime_error("receive from peer " + std::to_string(peer) + " timed out");
+  } else {
+    box.cv.wait(lk, ready);
+  }
+  if (box.q.empty()) throw std::runtime_error("peer " + std::to_string(peer) + " disconnected");
+  std::vector<char> m = std::move(box.q.front());
+  box.q.pop_front();
+  box.n--;
+  return m;
+}
+
+void PeerNet::receive_into(int peer, void* buf, size_t bytes) {
+  std::vector<char> m = receive(peer);
+  if (m.size() != bytes)
+    throw std::runtime_error("message from peer " + std::to_string(peer) + " has " + std::to_string(m.size()) +
+                             " bytes, expected " + std::to_string(bytes));
+  if (bytes) memcpy(buf, m.data(), bytes);
+}
+
+void PeerNet::broadcast(const void* buf, size_t bytes) {
+  for (int j = 0; j < P_; j++)
+    if (j != cid_) send_to(j, buf, bytes);
+}
+
+void PeerNet::ring_all_reduce_sum(double* buf, long n) {
+  const double t0 = now_seconds();
+  if (n < (long)P_ * 10000) {
+    broadcast(buf, n * sizeof(double));
+    for (int j = 0; j < P_; j++) {
+      if (j == cid_) continue;
+      std::vector<char> m = receive(j);
+      const double* x = (const double*)m.data();
+      for (long q = 0; q < n; q++) buf[q] = buf[q] + x[q];
+    }
+  } else {
+    std::vector<long> split(P_ + 1, 0);
+    for (int i = 0; i < P_; i++) split[i + 1] = n / P_;
+    for (int i = 1; i <= (int)(n % P_); i++) split[i] += 1;
+    for (int i = 2; i <= P_; i++) split[i] += split[i - 1];
+    const int next = (cid_ + 1) % P_, front = (cid_ - 1 + P_) % P_;
+    for (int s = 0; s < P_ - 1; s++) {
+      int di = (cid_ - s + P_) % P_;
+      send_to(next, buf + split[di], (split[di + 1] - split[di]) * sizeof(double));
+      std::vector<char> m = receive(front);
+      di = (di - 1 + P_) % P_;
+      const double* x = (const double*)m.data();
+      for (long q = 0; q < split[di + 1] - split[di]; q++) buf[split[di] + q] = buf[split[di] + q] + x[q];
+    }
+    for (int s = 0; s < P_ - 1; s++) {
+      int di = (cid_ + 1 - s + P_) % P_;
+      send_to(next, buf + split[di], (split[di + 1] - split[di]) * sizeof(double));
+      std::vector<char> m = receive(front);
+      di = (di - 1 + P_) % P_;
+      memcpy(buf + split[di], m.data(), (split[di + 1] - split[di]) * sizeof(double));
+    }
+  }
+  comm_s_ += now_seconds() - t0;
+}
+
+void PeerNet::all_gather(std::vector<char>& buf, size_t bytes, const std::vector<int>& grp, int src) {
+  const double t0 = now_seconds();
+  const int nc = (int)grp.size();
+  if (bytes <= (size_t)nc || nc == 2) {
+    if (cid_ == src) {
+      for (int j : grp)
+        if (j != cid_) send_to(j, buf.data(), bytes);
+    } else {
+      buf = receive(src);
+    }
+  } else {
+    std::vector<size_t> split(nc, 0);
+    for (int i = 0; i < nc - 1; i++) split[i + 1] = bytes / (nc - 1);
+    for (int i = 1; i <= (int)(bytes % (nc - 1)); i++) split[i] += 1;
+    for (int i = 2; i <= nc - 1; i++) split[i] += split[i - 1];
+    if (cid_ != src) buf.assign(bytes, 0);
+    int fake = 0, pre = -1, nxt = -1, counter = 0;
+    for (int i = 0; i < nc; i++) {
+      if (grp[i] == src) continue;
+      if (cid_ == src) {
+        send_to(grp[i], buf.data() + split[counter], split[counter + 1] - split[counter]);
+      } else if (cid_ == grp[i]) {
+        fake = counter;
+        pre = grp[(i - 1 + nc) % nc] != src ? grp[(i - 1 + nc) % nc] : grp[(i - 2 + nc) % nc];
+        nxt = grp[(i + 1) % nc] != src ? grp[(i + 1) % nc] : grp[(i + 2) % nc];
+        std::vector<char> m = receive(src);
+        memcpy(buf.data() + split[counter], m.data(), split[counter + 1] - split[counter]);
+      }
+      counter++;
+    }
+    if (cid_ != src) {
+      for (int s = 0; s < nc - 2; s++) {
+        int di = (fake - s + nc - 1) % (nc - 1);
+        send_to(nxt, buf.data() + split[di], split[di + 1] - split[di]);
+        std::vector<char> m = receive(pre);
+        di = (di - 1 + nc - 1) % (nc - 1);
+        memcpy(buf.data() + split[di], m.data(), split[di + 1] - split[di]);
+      }
+    }
+  }
+  comm_s_ += now_seconds() - t0;
+}
+
+}  // namespace exc

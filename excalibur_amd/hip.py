@@ -50,6 +50,36 @@ SIGNATURES = {
     "exc_givens_plan": (_I, [_L, _L, _I, _U, _vp, _vp]),
     "exc_givens_apply": (_I, [_dp, _L, _L, _dp, _dp, _L, _I, _I, _vp]),
     "exc_givens_apply_f32": (_I, [_dp, _L, _L, _dp, _dp, _L, _I, _I, _vp]),
+    "exc_gemm_workspace": (_S, [_L, _L, _L]),
+    "exc_dgemm": (_I, [_I, _I, _L, _L, _L, _D, _dp, _L, _dp, _L, _D, _dp, _L, _dp, _S, _vp]),
+    "exc_sgemm": (_I, [_I, _I, _L, _L, _L, _D, _dp, _L, _dp, _L, _D, _dp, _L, _dp, _S, _vp]),
+    "exc_fact_workspace": (_S, [_I, _L]),
+    "exc_potrf_lower": (_I, [_I, _dp, _L, _dp, _dp, _S, _vp]),
+    "exc_trsm_left": (_I, [_I, _I, _I, _L, _dp, _L, _dp, _L, _dp, _S, _vp]),
+    "exc_trsm_right": (_I, [_I, _I, _I, _L, _dp, _L, _dp, _L, _dp, _S, _vp]),
+    "exc_lu_sign": (_I, [_I, _dp, _L, _dp, _I, _dp, _S, _vp]),
+    "exc_qr_t_workspace": (_S, [_I, _L, _I]),
+    "exc_qr_t": (_I, [_I, _L, _dp, _L, _dp, _dp, _L, _dp, _L, _dp, _dp, _I, _dp, _S, _vp]),
+    "exc_wy_workspace": (_S, [_I, _L, _L]),
+    "exc_apply_q_right": (_I, [_L, _L, _I, _dp, _L, _dp, _L, _dp, _dp, _S, _vp]),
+    "exc_orgqr_t": (_I, [_I, _L, _dp, _L, _dp, _dp, _L, _dp, _S, _vp]),
+    "exc_ormqr_house_workspace": (_S, [_I, _L]),
+    "exc_ormqr_house_ln": (_I, [_I, _L, _dp, _dp, _dp, _L, _dp, _S, _vp]),
+    "exc_ralha_msg": (_I, [_I, _I, _I, _dp, _dp, _dp, _vp]),
+    "exc_ralha_apply": (_I, [_I, _I, _I, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _vp]),
+    "exc_ralha_dot": (_I, [_I, _I, _dp, _dp, _dp, _vp]),
+    "exc_ralha_tail_update": (_I, [_I, _I, _dp, _dp, _dp, _dp, _dp, _vp]),
+    "exc_ralha_tail_scale": (_I, [_I, _I, _dp, _dp, _dp, _dp, _vp]),
+    "exc_lower_to_upper": (_I, [_I, _dp, _dp, _dp, _vp]),
+    "exc_sign_normalize": (_I, [_I, _I, _dp, _dp, _dp, _vp]),
+    "exc_rot_back": (_I, [_I, _I, _dp, _dp, _vp]),
+    "exc_bisection_bound": (_I, [_I, _dp, _dp, _dp, _vp]),
+    "exc_copy2d": (_I, [_L, _L, _dp, _L, _dp, _L, _vp]),
+    "exc_transpose": (_I, [_L, _L, _dp, _L, _dp, _L, _vp]),
+    "exc_fill2d": (_I, [_L, _L, _D, _dp, _L, _vp]),
+    "exc_reflector_message_len": (_L, [_L, _L, _L]),
+    "exc_pack_reflectors": (_I, [_L, _L, _L, _dp, _L, _dp, _dp, _vp]),
+    "exc_unpack_reflectors": (_I, [_L, _L, _L, _dp, _dp, _L, _dp, _vp]),
 }
 
 EXTRA_SIGNATURES = {}
@@ -220,3 +250,211 @@ def givens_apply(x, pairs, cs, mask, is_left, f32=False):
          1 if mask else 0, 1 if is_left else 0, None)
     sync()
     return dx.to_host()
+
+
+def gemm(A, B, ta=False, tb=False, alpha=1.0, beta=0.0, C=None, f32=False, split=True):
+    dt = np.float32 if f32 else np.float64
+    A = np.ascontiguousarray(A, dt); B = np.ascontiguousarray(B, dt)
+    M = A.shape[1] if ta else A.shape[0]
+    K = A.shape[0] if ta else A.shape[1]
+    N = B.shape[0] if tb else B.shape[1]
+    dA = DeviceArray.from_host(A); dB = DeviceArray.from_host(B)
+    dC = DeviceArray.from_host(np.ascontiguousarray(C, dt)) if C is not None else DeviceArray((M, N), dt).zero()
+    ws = workspace(_lib.exc_gemm_workspace(M, N, K)) if split else None
+    call("exc_sgemm" if f32 else "exc_dgemm", int(ta), int(tb), M, N, K, alpha, dA.ptr, A.shape[1], dB.ptr,
+         B.shape[1], beta, dC.ptr, N, ws.ptr if ws else None, ws.nbytes if ws else 0, None)
+    sync()
+    return dC.to_host()
+
+
+def potrf_lower(G):
+    G = np.ascontiguousarray(G, np.float64)
+    n = G.shape[0]
+    d = DeviceArray.from_host(G); info = DeviceArray((1,), np.int32).zero()
+    ws = workspace(_lib.exc_fact_workspace(n, n))
+    call("exc_potrf_lower", n, d.ptr, n, info.ptr, ws.ptr, ws.nbytes, None)
+    sync()
+    return d.to_host(), int(info.to_host()[0])
+
+
+def trsm_left(T, X, upper=False, unit=False):
+    T = np.ascontiguousarray(T, np.float64); X = np.ascontiguousarray(X, np.float64)
+    n, nrhs = X.shape
+    dT = DeviceArray.from_host(T); dX = DeviceArray.from_host(X)
+    ws = workspace(_lib.exc_fact_workspace(n, nrhs))
+    call("exc_trsm_left", int(upper), int(unit), n, nrhs, dT.ptr, n, dX.ptr, nrhs, ws.ptr, ws.nbytes, None)
+    sync()
+    return dX.to_host()
+
+
+def trsm_right(T, X, lower_t=False, unit=False):
+    T = np.ascontiguousarray(T, np.float64); X = np.ascontiguousarray(X, np.float64)
+    rows, n = X.shape
+    dT = DeviceArray.from_host(T); dX = DeviceArray.from_host(X)
+    ws = workspace(_lib.exc_fact_workspace(n, rows))
+    call("exc_trsm_right", int(lower_t), int(unit), n, rows, dT.ptr, n, dX.ptr, n, ws.ptr, ws.nbytes, None)
+    sync()
+    return dX.to_host()
+
+
+def lu_sign(A):
+    A = np.ascontiguousarray(A, np.float64)
+    n = A.shape[0]
+    dA = DeviceArray.from_host(A); ds = DeviceArray((n,))
+    ws = workspace(_lib.exc_fact_workspace(n, n))
+    call("exc_lu_sign", n, dA.ptr, n, ds.ptr, 0, ws.ptr, ws.nbytes, None)
+    sync()
+    return dA.to_host(), ds.to_host()
+
+
+def qr_t(X, want_q=True, want_v=False, square_last=True):
+    """QR of X^T (X: m x r). Returns dict rt, qt, vt, tau, sgn (numpy)."""
+    X = np.ascontiguousarray(X, np.float64)
+    m, r = X.shape
+    dX = DeviceArray.from_host(X)
+    drt = DeviceArray((m, m))
+    dqt = DeviceArray((m, r)) if want_q else None
+    dvt = DeviceArray((m, r)) if want_v else None
+    dtau = DeviceArray((m,)) if want_v else None
+    dsg = DeviceArray((m,))
+    ws = workspace(_lib.exc_qr_t_workspace(m, r, int(want_q)))
+    call("exc_qr_t", m, r, dX.ptr, r, drt.ptr, dqt.ptr if dqt else None, r, dvt.ptr if dvt else None, r,
+         dtau.ptr if dtau else None, dsg.ptr, int(square_last), ws.ptr, ws.nbytes, None)
+    sync()
+    out = dict(rt=drt.to_host(), sgn=dsg.to_host())
+    if dqt: out["qt"] = dqt.to_host()
+    if dvt: out["vt"] = dvt.to_host(); out["tau"] = dtau.to_host()
+    return out
+
+
+def apply_q_right(Mt, vt, tau):
+    Mt = np.ascontiguousarray(Mt, np.float64); vt = np.ascontiguousarray(vt, np.float64)
+    rows, L = Mt.shape
+    k = vt.shape[0]
+    dM = DeviceArray.from_host(Mt); dV = DeviceArray.from_host(vt); dT = DeviceArray.from_host(np.asarray(tau, np.float64))
+    ws = workspace(_lib.exc_wy_workspace(k, L, rows))
+    call("exc_apply_q_right", rows, L, k, dM.ptr, L, dV.ptr, L, dT.ptr, ws.ptr, ws.nbytes, None)
+    sync()
+    return dM.to_host()
+
+
+def orgqr_t(vt, tau):
+    vt = np.ascontiguousarray(vt, np.float64)
+    k, L = vt.shape
+    dV = DeviceArray.from_host(vt); dT = DeviceArray.from_host(np.asarray(tau, np.float64))
+    dQ = DeviceArray((k, L))
+    ws = workspace(_lib.exc_wy_workspace(k, L, k))
+    call("exc_orgqr_t", k, L, dV.ptr, L, dT.ptr, dQ.ptr, L, ws.ptr, ws.nbytes, None)
+    sync()
+    return dQ.to_host()
+
+
+def ormqr_house_ln(house, tau, C):
+    house = np.ascontiguousarray(house, np.float64); C = np.ascontiguousarray(C, np.float64)
+    m, n = C.shape
+    dH = DeviceArray.from_host(house); dT = DeviceArray.from_host(np.asarray(tau, np.float64))
+    dC = DeviceArray.from_host(C)
+    ws = workspace(_lib.exc_ormqr_house_workspace(m, n))
+    call("exc_ormqr_house_ln", m, n, dH.ptr, dT.ptr, dC.ptr, n, ws.ptr, ws.nbytes, None)
+    sync()
+    return dC.to_host()
+
+
+def ralha_peers(F_list):
+    """The overlapped Ralha loop over P simulated peers in this process: GPU kernels per peer,
+    the all-reduce done here in each peer's own order (own buffer, then the others by id)."""
+    P = len(F_list)
+    m = F_list[0].shape[0]
+    cs = [f.shape[1] for f in F_list]
+    dF = [DeviceArray.from_host(np.ascontiguousarray(f, np.float64)) for f in F_list]
+    dU = [DeviceArray((m, c)).zero() for c in cs]
+    da = [DeviceArray((m,)).zero() for _ in range(P)]
+    db = [DeviceArray((m,)).zero() for _ in range(P)]
+    dh = [DeviceArray((m, m)).zero() for _ in range(P)]
+    dt = [DeviceArray((m,)).zero() for _ in range(P)]
+    dmsg = [DeviceArray((m + 3,)).zero() for _ in range(P)]
+    dg = [DeviceArray((m + 3,)).zero() for _ in range(P)]
+    ds = [DeviceArray((4,)).zero() for _ in range(P)]
+
+    def allreduce(srcs, n):
+        vals = [a.to_host()[:n].copy() for a in srcs]
+        out = []
+        for p in range(P):
+            acc = vals[p].copy()
+            for q in range(P):
+                if q != p:
+                    acc = acc + vals[q]
+            out.append(acc)
+        return out
+
+    def put(dst, vec, off=0):
+        buf = dst.to_host()
+        buf[off:off + vec.size] = vec
+        call("exc_memcpy_h2d", dst.ptr, buf.ctypes.data, buf.nbytes, None)
+
+    for p in range(P):
+        call("exc_ralha_tail_update", -1, cs[p], dF[p].ptr, dU[p].ptr, None, db[p].ptr, ds[p].ptr, None)
+    sync()
+    red = allreduce(ds, 1)
+    for p in range(P):
+        put(ds[p], red[p], 1)
+        call("exc_ralha_tail_scale", 0, cs[p], dF[p].ptr, dU[p].ptr, ds[p].ptr + 8, da[p].ptr, None)
+    for i in range(m - 2):
+        for p in range(P):
+            call("exc_ralha_msg", i, m, cs[p], dF[p].ptr, dU[p].ptr, dmsg[p].ptr, None)
+        sync()
+        red = allreduce(dmsg, m + 3 - i)
+        for p in range(P):
+            put(dg[p], red[p])
+            call("exc_ralha_apply", i, m, cs[p], dF[p].ptr, dU[p].ptr, dg[p].ptr, dh[p].ptr, dt[p].ptr, da[p].ptr,
+                 db[p].ptr, None)
+        sync()
+    for i in range(m - 2, m):
+        for p in range(P):
+            call("exc_ralha_dot", i, cs[p], dF[p].ptr, dU[p].ptr, ds[p].ptr, None)
+        sync()
+        red = allreduce(ds, 1)
+        for p in range(P):
+            put(ds[p], red[p], 1)
+            call("exc_ralha_tail_update", i, cs[p], dF[p].ptr, dU[p].ptr, ds[p].ptr + 8, db[p].ptr, ds[p].ptr + 16,
+                 None)
+        sync()
+        red2 = [a.to_host()[2:3].copy() for a in ds]
+        for p in range(P):
+            acc = red2[p].copy()
+            for q in range(P):
+                if q != p:
+                    acc = acc + red2[q]
+            put(ds[p], acc, 3)
+            call("exc_ralha_tail_scale", i, cs[p], dF[p].ptr, dU[p].ptr, ds[p].ptr + 24, da[p].ptr, None)
+        sync()
+    return [dict(alpha=da[p].to_host(), beta=db[p].to_host()[: m - 1], house=dh[p].to_host(), tau=dt[p].to_host(),
+                 U=dU[p].to_host()) for p in range(P)]
+
+
+def lower_to_upper(alpha, beta):
+    m = len(alpha)
+    da = DeviceArray.from_host(np.asarray(alpha, np.float64))
+    db = DeviceArray.from_host(np.concatenate([np.asarray(beta, np.float64), [0.0]]))
+    dc = DeviceArray((2 * m,)).zero()
+    call("exc_lower_to_upper", m, da.ptr, db.ptr, dc.ptr, None)
+    sync()
+    return da.to_host(), db.to_host()[: m - 1], dc.to_host()[: 2 * (m - 1)]
+
+
+def sign_normalize(bu, bv):
+    bu = np.ascontiguousarray(bu, np.float64); bv = np.ascontiguousarray(bv, np.float64)
+    m, k = bu.shape
+    du = DeviceArray.from_host(bu); dv = DeviceArray.from_host(bv); df = DeviceArray((max(k, 1),))
+    call("exc_sign_normalize", m, k, du.ptr, dv.ptr, df.ptr, None)
+    sync()
+    return du.to_host(), dv.to_host()
+
+
+def rot_back(bu, cs):
+    bu = np.ascontiguousarray(bu, np.float64)
+    m, k = bu.shape
+    du = DeviceArray.from_host(bu); dc = DeviceArray.from_host(np.asarray(cs, np.float64))
+    call("exc_rot_back", m, k, du.ptr, dc.ptr, None)
+    sync()
+    return du.to_host()

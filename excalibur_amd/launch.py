@@ -1,0 +1,115 @@
+"""Runs the GPU DecentralizedClient as P peer processes on loopback (one excalibur_peer process per
+peer, the reference's main.cpp command line), and collects their timers and results.
+
+The peers share the one GPU; they talk over host TCP on 127.0.0.1..127.0.0.P at base_port + cid,
+the addressing of client.cpp:238-245.
+"""
+import json
+import os
+import socket
+import subprocess
+import tempfile
+import time
+
+import numpy as np
+
+from . import PEER_PATH
+
+
+def free_port(span):
+    """A base port with `span` consecutive free ports."""
+    for _ in range(50):
+        s = socket.socket()
+        s.bind(("127.0.0.1", 0))
+        base = s.getsockname()[1]
+        s.close()
+        if base + span >= 65000:
+            continue
+        ok = True
+        for p in range(base, base + span):
+            t = socket.socket()
+            try:
+                t.bind(("", p))
+            except OSError:
+                ok = False
+            finally:
+                t.close()
+            if ok is False:
+                break
+        if ok:
+            return base
+    raise RuntimeError("no free port range")
+
+
+def write_peer_files(xs, dpath):
+    """Client{i}.mat: raw float64, row-major m x local_n_i (client.cpp:577, generate_data.cpp:184)."""
+    os.makedirs(dpath, exist_ok=True)
+    for i, x in enumerate(xs):
+        np.ascontiguousarray(x, dtype=np.float64).tofile(os.path.join(dpath, f"Client{i}.mat"))
+
+
+def run_peers(xs, seed=0, mode=1, topk=-1, opt=3, form="fp64", dump=True, workdir=None, timeout=600,
+              recv_timeout=300.0, extra=None):
+    """Runs len(xs) peers on the column blocks xs (each m x local_n_i). Returns dict with per-peer
+    timers and, when dump, sigma / FinalU / FinalVT arrays."""
+    if not os.path.exists(PEER_PATH):
+        raise RuntimeError(f"{PEER_PATH} is not built")
+    P = len(xs)
+    m = xs[0].shape[0]
+    own = workdir is None
+    workdir = workdir or tempfile.mkdtemp(prefix="exc_peers_")
+    dpath = os.path.join(workdir, "data")
+    logdir = os.path.join(workdir, "logs")
+    os.makedirs(logdir, exist_ok=True)
+    write_peer_files(xs, dpath)
+    port = free_port(P + 1)
+    procs = []
+    for cid in range(P):
+        cmd = [PEER_PATH, "-l", "127.0.0.1", "-p", str(port), "-m", str(m), "-n", str(xs[cid].shape[1]),
+               "-d", dpath, "-g", logdir, "-k", str(topk), "-o", str(mode), "-t", str(P), "-c", str(cid),
+               "-e", "0", "-f", "0", "-s", str(seed), "-a", str(opt), "--form", form,
+               "--dump", "1" if dump else "0", "--recv-timeout", str(recv_timeout)] + list(extra or [])
+        out = open(os.path.join(logdir, f"peer{cid}.out"), "w")
+        procs.append((subprocess.Popen(cmd, stdout=out, stderr=subprocess.STDOUT), out))
+    t0 = time.time()
+    rcs = []
+    for p, out in procs:
+        left = max(1.0, timeout - (time.time() - t0))
+        try:
+            rcs.append(p.wait(timeout=left))
+        except subprocess.TimeoutExpired:
+            for q, _ in procs:
+                if q.poll() is None:
+                    q.kill()
+            rcs.append(-9)
+        out.close()
+    res = {"workdir": workdir, "rc": rcs, "peers": []}
+    for cid in range(P):
+        text = open(os.path.join(logdir, f"peer{cid}.out")).read()
+        info = None
+        for line in text.splitlines():
+            if line.startswith("EXC_RESULT "):
+                info = json.loads(line[len("EXC_RESULT "):])
+        peer = {"log": text, "result": info}
+        if dump and info is not None:
+            k = info["sigma_count"]
+            base = os.path.join(logdir, f"Client{cid}_")
+            peer["sigma"] = np.fromfile(base + "sigma.bin")
+            if os.path.exists(base + "FinalU.bin"):
+                peer["FinalU"] = np.fromfile(base + "FinalU.bin").reshape(m, k)
+            if os.path.exists(base + "FinalVT.bin"):
+                peer["FinalVT"] = np.fromfile(base + "FinalVT.bin").reshape(k, xs[cid].shape[1])
+            for name in ("alpha", "beta", "bidiag_v"):
+                if os.path.exists(base + name + ".bin"):
+                    peer[name] = np.fromfile(base + name + ".bin")
+        res["peers"].append(peer)
+    if any(rc != 0 for rc in rcs):
+        logs = "\n".join(p["log"][-2000:] for p in res["peers"])
+        raise RuntimeError(f"peer processes failed rc={rcs}\n{logs}")
+    if own:
+        for cid in range(P):
+            try:
+                os.remove(os.path.join(dpath, f"Client{cid}.mat"))
+            except OSError:
+                pass
+    return res

@@ -109,6 +109,107 @@ int exc_givens_apply(double* d_x, long m, long n, const int32_t* d_pairs, const 
 int exc_givens_apply_f32(float* d_x, long m, long n, const int32_t* d_pairs, const double* d_cs, long passes,
                          int mask, int is_left, exc_stream_t stream);
 
+/* ---------------------------------------------------------------- dense kernels (MFMA) */
+
+/* C = alpha op(A) op(B) + beta C, row-major, op = transpose when trans_* != 0. FP64 on
+ * v_mfma_f64_16x16x4_f64 (the reference's cblas_dgemm calls, client.cpp:1018, 1179, 1190);
+ * FP32 on v_mfma_f32_16x16x4_f32 for the mixed-precision form. d_work (nullable) enables
+ * split-K for short-and-deep products; exc_gemm_workspace() bounds what it may use. */
+size_t exc_gemm_workspace(long M, long N, long K);
+int exc_dgemm(int trans_a, int trans_b, long M, long N, long K, double alpha, const double* d_a, long lda,
+              const double* d_b, long ldb, double beta, double* d_c, long ldc, void* d_work, size_t work_bytes,
+              exc_stream_t stream);
+int exc_sgemm(int trans_a, int trans_b, long M, long N, long K, double alpha, const float* d_a, long lda,
+              const float* d_b, long ldb, double beta, float* d_c, long ldc, void* d_work, size_t work_bytes,
+              exc_stream_t stream);
+
+/* Blocked factorizations (FP64, row-major, in place). The workspace (exc_fact_workspace) holds
+ * split-K partials of the GEMM updates. */
+size_t exc_fact_workspace(int n, long wide);
+/* Lower Cholesky of a symmetric n x n matrix (lower triangle read, strict upper zeroed);
+ * *d_info = 0 or the 1-based index of the first non-positive pivot. */
+int exc_potrf_lower(int n, double* d_a, long lda, int* d_info, void* d_work, size_t work_bytes, exc_stream_t stream);
+/* X <- op(T)^-1 X (left; T lower, or upper when upper != 0; unit diagonal when unit != 0). */
+int exc_trsm_left(int upper, int unit, int n, long nrhs, const double* d_l, long ldl, double* d_x, long ldx,
+                  void* d_work, size_t work_bytes, exc_stream_t stream);
+/* X <- X U^-1 (right; U upper), or X <- X L^-T when lower_t != 0. */
+int exc_trsm_right(int lower_t, int unit, int n, long rows, const double* d_u, long ldu, double* d_x, long ldx,
+                   void* d_work, size_t work_bytes, exc_stream_t stream);
+/* LU without pivoting of A - S, S_jj = -sign(current pivot candidate) chosen column by column
+ * (the sign rule of Householder reconstruction); d_sgn[j] = S_jj. square_last: the last column
+ * takes +sign instead (LAPACK does not reflect the last column of a square matrix). */
+int exc_lu_sign(int n, double* d_a, long lda, double* d_sgn, int square_last, void* d_work, size_t work_bytes,
+                exc_stream_t stream);
+
+/* ---------------------------------------------------------------- Householder QR (FP64) */
+
+/* QR of X^T (X: m x r row-major, r >= m), i.e. of the column-major r x m matrix the reference
+ * passes to LAPACKE_dgeqrf (client.cpp:424, 715), with LAPACK's Householder convention. Outputs:
+ * d_rt (m x m, lower) = R^T; d_qt (nullable, m x r, may alias d_x) = Q^T (dorgqr, client.cpp:430,
+ * 785); d_vt (nullable, m x r) the reflectors one per row (unit entry at column j, zeros before);
+ * d_tau (nullable, m); d_sgn (nullable, m) the reconstruction signs. square_last: treat the last
+ * column of a square X^T as LAPACK does (tau = 0). */
+size_t exc_qr_t_workspace(int m, long r, int want_q);
+int exc_qr_t(int m, long r, const double* d_x, long ldx, double* d_rt, double* d_qt, long ldq, double* d_vt,
+             long ldv, double* d_tau, double* d_sgn, int square_last, void* d_work, size_t work_bytes,
+             exc_stream_t stream);
+/* Mt (rows x L) <- Mt Q with Q = H_0...H_{k-1} from reflector rows d_vt (k x L) and d_tau: the
+ * transposed form of LAPACKE_dormqr('L', 'T') at client.cpp:777. Compact WY on the MFMA GEMM,
+ * T^-1 = striu(V^T V) + diag(1/tau). */
+size_t exc_wy_workspace(int k, long L, long rows);
+int exc_apply_q_right(long rows, long L, int k, double* d_mt, long ldm, const double* d_vt, long ldv,
+                      const double* d_tau, void* d_work, size_t work_bytes, exc_stream_t stream);
+/* Q^T (k x L) = first k rows of (H_0...H_{k-1})^T: LAPACKE_dorgqr at client.cpp:785. */
+int exc_orgqr_t(int k, long L, const double* d_vt, long ldv, const double* d_tau, double* d_qt, long ldq,
+                void* d_work, size_t work_bytes, exc_stream_t stream);
+/* C (m x n) <- H_0...H_{m-1} C with reflector j in column j of the row-major m x m d_house
+ * (implied unit diagonal): LAPACKE_dormqr(RowMajor, 'L', 'N') at client.cpp:1176. */
+size_t exc_ormqr_house_workspace(int m, long n);
+int exc_ormqr_house_ln(int m, long n, const double* d_house, const double* d_tau, double* d_c, long ldc,
+                       void* d_work, size_t work_bytes, exc_stream_t stream);
+
+/* ---------------------------------------------------------------- Ralha bidiagonalization */
+
+/* Step i (0 <= i <= m-3) of the overlapped one-sided bidiagonalization (client.cpp:840-892) on
+ * this peer's F (m x c) and U (m x c). exc_ralha_msg writes the local message d_msg (m+3-i
+ * values: f_{i+1+r}.f_i, then f_i.u_{i-1}, u_{i-1}.u_{i-1}, f_i.f_i, u_{i-1}.f_i when i > 0);
+ * after the host all-reduces it over the peers, exc_ralha_apply takes the reduced d_g: dlarfg
+ * (LAPACK convention) into column i+1 of d_house (m x m) and d_tau[i+1], the rank-1 update of
+ * F[i+1:], beta_{i-1}, alpha_i and u_i. */
+int exc_ralha_msg(int i, int m, int c, const double* d_f, const double* d_u, double* d_msg, exc_stream_t stream);
+int exc_ralha_apply(int i, int m, int c, double* d_f, double* d_u, const double* d_g, double* d_house, double* d_tau,
+                    double* d_alpha, double* d_beta, exc_stream_t stream);
+/* The last two columns and the first (client.cpp:830-833, 897-905): d_out = f_i.u_{i-1};
+ * tail_update: f_i -= beta u_{i-1} (beta = reduced value, stored in d_beta[i-1]), d_out =
+ * ||f_i||^2 (i = -1: ||f_0||^2 only); tail_scale: alpha_i = sqrt(reduced), u_i = f_i / alpha_i. */
+int exc_ralha_dot(int i, int c, const double* d_f, const double* d_u, double* d_out, exc_stream_t stream);
+int exc_ralha_tail_update(int i, int c, double* d_f, const double* d_u, const double* d_breduced, double* d_beta,
+                          double* d_out, exc_stream_t stream);
+int exc_ralha_tail_scale(int i, int c, const double* d_f, double* d_u, const double* d_n2, double* d_alpha,
+                         exc_stream_t stream);
+/* Lower to upper bidiagonal by drotg (client.cpp:947-954); d_cs = (c_i, s_i) pairs, 2(m-1). */
+int exc_lower_to_upper(int m, double* d_alpha, double* d_beta, double* d_cs, exc_stream_t stream);
+/* client.cpp:987-995: columns of d_bu (m x k) made to have a positive largest entry, rows of
+ * d_bv (k x m) flipped with them; d_flip (k) receives the factors. */
+int exc_sign_normalize(int m, int k, double* d_bu, double* d_bv, double* d_flip, exc_stream_t stream);
+/* client.cpp:1127-1128: rows (i, i+1) of d_bu (m x k) rotated by (c_i, -s_i), i = m-2..0. */
+int exc_rot_back(int m, int k, double* d_bu, const double* d_cs, exc_stream_t stream);
+/* d_out2 = {||alpha||^2 + ||beta||^2, 0}: the bisection root bound of client.cpp:966 and the
+ * two shifts of its singular-value count. */
+int exc_bisection_bound(int n, const double* d_alpha, const double* d_beta, double* d_out2, exc_stream_t stream);
+
+/* ---------------------------------------------------------------- data movement */
+int exc_copy2d(long rows, long cols, const double* d_src, long lds, double* d_dst, long ldd, exc_stream_t stream);
+int exc_transpose(long rows, long cols, const double* d_src, long lds, double* d_dst, long ldd, exc_stream_t stream);
+int exc_fill2d(long rows, long cols, double value, double* d_dst, long ldd, exc_stream_t stream);
+/* Global-QR message of a peer panel (client.cpp:716-724): for each of its c reflectors
+ * j = pos..pos+c-1, tau_j then the R-1-j entries of v_j below the diagonal (rows of d_vt). */
+long exc_reflector_message_len(long c, long R, long pos);
+int exc_pack_reflectors(long c, long R, long pos, const double* d_vt, long ldv, const double* d_tau, double* d_out,
+                        exc_stream_t stream);
+int exc_unpack_reflectors(long c, long R, long pos, const double* d_in, double* d_vt, long ldv, double* d_tau,
+                          exc_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,175 @@
+"""GPU pipeline stages of DecentralizedClient::run() against the oracle (same inputs), and the
+multi-process drop-in run against the oracle's simulated run and numpy's SVD."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def banner(msg):
+    print(f"[gpu-pipeline] {msg}", flush=True)
+
+
+def test_local_qr_vs_oracle(hip, oracle):
+    banner("local QR 50 x 400")
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal((50, 400))
+    rt_o, qt_o = oracle.local_qr(x)
+    out = hip.qr_t(x, want_q=True)
+    assert np.max(np.abs(out["rt"] - rt_o)) <= 1e-12 * np.abs(rt_o).max()
+    assert np.max(np.abs(out["qt"] - qt_o)) <= 1e-12
+
+
+@pytest.mark.parametrize("P,m,widths", [(2, 40, [80, 80]), (3, 31, [40, 31, 50]), (2, 100, [500, 500])])
+def test_global_qr_vs_oracle(hip, oracle, P, m, widths):
+    """The sequential panel protocol of client.cpp:706-794 driven through the C ABI."""
+    banner(f"global QR P={P} m={m}")
+    rng = np.random.default_rng(m)
+    R = sum(widths)
+    mp = [m // P + (1 if i < m % P else 0) for i in range(P)]
+    pos = np.concatenate([[0], np.cumsum(mp)])
+    panels = [rng.standard_normal((mp[c], R)) for c in range(P)]
+    shr_o, qg_o, tau_o = oracle.global_qr([p.copy() for p in panels], m)
+    # GPU: each peer's panel in turn, reflectors exchanged as rows of Vt
+    Vt = np.zeros((m, R)); tau = np.zeros(m)
+    pans = [p.copy() for p in panels]
+    for i in range(P):
+        L = R - pos[i]
+        out = hip.qr_t(pans[i][:, pos[i]:].copy(), want_q=False, want_v=True, square_last=True)
+        pans[i][:, pos[i]:pos[i] + mp[i]] = out["rt"]
+        pans[i][:, pos[i] + mp[i]:] = 0.0
+        Vt[pos[i]:pos[i + 1], pos[i]:] = out["vt"]
+        tau[pos[i]:pos[i + 1]] = out["tau"]
+        for c in range(i + 1, P):
+            pans[c][:, pos[i]:] = hip.apply_q_right(pans[c][:, pos[i]:].copy(), out["vt"], out["tau"])
+    qgt = hip.orgqr_t(Vt, tau)
+    assert np.max(np.abs(tau - tau_o)) <= 1e-12
+    for c in range(P):
+        shrink = pans[c][:, :m].T
+        assert np.max(np.abs(shrink - shr_o[c])) <= 1e-12 * np.abs(shr_o[c]).max()
+    assert np.max(np.abs(qgt.T - qg_o)) <= 1e-12
+
+
+@pytest.mark.parametrize("P,m,cols", [(1, 12, [12]), (2, 60, [30, 30]), (3, 101, [34, 34, 33])])
+def test_ralha_vs_oracle(hip, oracle, P, m, cols):
+    banner(f"ralha P={P} m={m}")
+    rng = np.random.default_rng(m + P)
+    # an R factor of Gaussian data, split by columns: the matrix the reference feeds to Ralha
+    Rf = np.linalg.qr(rng.standard_normal((2 * m, m)))[1]
+    F = [Rf[:, sum(cols[:p]):sum(cols[:p + 1])].copy() for p in range(P)]
+    got = hip.ralha_peers(F)
+    want = oracle.ralha(F)
+    for p in range(P):
+        for key in ("alpha", "beta", "tau", "U"):
+            a, b = got[p][key], want[p][key]
+            assert np.max(np.abs(a - b)) <= 1e-11 * max(1.0, np.abs(b).max()), key
+        ho, hg = want[p]["house"], got[p]["house"]
+        # reflector columns (below the diagonal) of columns with tau != 0
+        for j in range(1, m - 1):
+            if want[p]["tau"][j] != 0:
+                assert np.max(np.abs(hg[j + 1:, j] - ho[j + 1:, j])) <= 1e-10
+
+
+def test_bidiag_helpers_vs_oracle(hip, oracle):
+    banner("lower->upper, sign rule, back rotation")
+    rng = np.random.default_rng(4)
+    m = 200
+    a = np.abs(rng.standard_normal(m)) + 0.1
+    b = rng.standard_normal(m - 1)
+    ag, bg, csg = hip.lower_to_upper(a, b)
+    ao, bo, cso = oracle.lower_to_upper(a, b)
+    assert np.max(np.abs(ag - ao)) <= 1e-14 * np.abs(ao).max()
+    assert np.max(np.abs(bg - bo)) <= 1e-14 * np.abs(ao).max()
+    assert np.max(np.abs(csg - cso)) <= 1e-14
+    bu = rng.standard_normal((m, 50)); bv = rng.standard_normal((50, m))
+    u1, v1 = hip.sign_normalize(bu, bv)
+    u2, v2 = oracle.sign_normalize(bu, bv)
+    assert np.array_equal(u1, u2) and np.array_equal(v1, v2)
+    r1 = hip.rot_back(bu, cso)
+    r2 = oracle.rot_back(bu, cso)
+    assert np.array_equal(r1, r2)
+
+
+def compare_runs(res, run, xs, tol_sigma=1e-12):
+    m = xs[0].shape[0]
+    k = run.count
+    X = np.hstack(xs)
+    s_np = np.linalg.svd(X, compute_uv=False)
+    for cid, peer in enumerate(res["peers"]):
+        sg = peer["sigma"]
+        so = run.get("sigma", cid)
+        assert sg.size == k
+        assert np.max(np.abs(sg - so)) <= tol_sigma * so[0]
+        assert np.max(np.abs(sg - s_np[:k])) <= tol_sigma * s_np[0]
+    U = res["peers"][0]["FinalU"]
+    assert all(np.array_equal(U, p["FinalU"]) for p in res["peers"]), "every peer holds the same FinalU"
+    VT = np.hstack([p["FinalVT"] for p in res["peers"]])
+    sg = res["peers"][0]["sigma"]
+    resid = np.linalg.norm(X - (U * sg) @ VT) / np.linalg.norm(X)
+    orth_u = np.abs(U.T @ U - np.eye(k)).max()
+    orth_v = np.abs(VT @ VT.T - np.eye(k)).max()
+    Uo = run.get("FinalU", 0).reshape(m, k)
+    VTo = np.hstack([run.get("FinalVT", c).reshape(k, x.shape[1]) for c, x in enumerate(xs)])
+    resid_o = np.linalg.norm(X - (Uo * run.get("sigma", 0)) @ VTo) / np.linalg.norm(X)
+    orth_uo = np.abs(Uo.T @ Uo - np.eye(k)).max()
+    # signs follow the reference's rules: vectors of well separated values agree with the oracle's
+    gaps = np.minimum(np.abs(np.diff(np.r_[np.inf, sg])), np.abs(np.diff(np.r_[sg, -np.inf])))
+    sep = gaps > 1e-6 * sg[0]
+    du = np.abs(U[:, sep] - Uo[:, sep]).max() if sep.any() else 0.0
+    p0 = res["peers"][0]
+    diag = {}
+    if "alpha" in p0:
+        diag["d_alpha"] = float(np.abs(p0["alpha"] - run.get("alpha", 0)).max())
+        diag["d_beta"] = float(np.abs(p0["beta"] - run.get("beta", 0)).max())
+        bvo = run.get("bidiag_v", 0).reshape(k, m)
+        bvg = p0["bidiag_v"].reshape(k, m)
+        flips = [j for j in range(k) if np.abs(bvg[j] + bvo[j]).max() < 1e-6]
+        diag["bv_flipped_rows"] = flips[:20]
+        diag["d_bv"] = float(np.abs(bvg - bvo).max())
+    colflip = [j for j in range(k) if np.abs(U[:, j] + Uo[:, j]).max() < 1e-6]
+    diag["U_flipped_cols"] = colflip[:20]
+    return dict(resid=resid, orth_u=orth_u, orth_v=orth_v, resid_oracle=resid_o, orth_u_oracle=orth_uo,
+                du_separated=du, **diag)
+
+
+def test_two_peer_run_config1(hip, oracle):
+    """A = 1000 x 100 N(0,1) seed 0 as 2 row blocks of 500 (reference sees A^T: m = 100)."""
+    from excalibur_amd.launch import run_peers
+    banner("2-peer run, 2 x 500 x 100")
+    rng = np.random.default_rng(0)
+    A = rng.standard_normal((1000, 100))
+    xs = [A[:500].T.copy(), A[500:].T.copy()]
+    res = run_peers(xs, seed=0)
+    run = oracle.Run(xs, [0, 0])
+    st = compare_runs(res, run, xs)
+    print(st)
+    assert st["resid"] <= 1e-12 and st["orth_u"] <= 1e-12 and st["orth_v"] <= 1e-12
+    assert st["du_separated"] <= 1e-9
+
+
+def test_two_peer_run_with_local_qr(hip, oracle):
+    """local_n / m > 5 triggers the local QR (client.cpp:572) and R = 2m the global QR."""
+    from excalibur_amd.launch import run_peers
+    banner("2-peer run with local QR, m = 60, local_n = 400")
+    rng = np.random.default_rng(5)
+    xs = [rng.standard_normal((60, 400)), rng.standard_normal((60, 400))]
+    res = run_peers(xs, seed=3)
+    run = oracle.Run(xs, [3, 3])
+    st = compare_runs(res, run, xs)
+    print(st)
+    assert st["resid"] <= 1e-12 and st["orth_u"] <= 1e-12 and st["orth_v"] <= 1e-12
+    assert st["du_separated"] <= 1e-9
+
+
+def test_three_peer_run_uneven(hip, oracle):
+    from excalibur_amd.launch import run_peers
+    banner("3-peer run, uneven widths")
+    rng = np.random.default_rng(9)
+    m = 47
+    xs = [rng.standard_normal((m, w)) for w in (300, 120, 500)]
+    res = run_peers(xs, seed=11)
+    run = oracle.Run(xs, [11, 11, 11])
+    st = compare_runs(res, run, xs)
+    print(st)
+    assert st["resid"] <= 1e-12 and st["orth_u"] <= 1e-12 and st["orth_v"] <= 1e-12
+    assert st["du_separated"] <= 1e-9
