@@ -32,6 +32,45 @@ void set_last_message(const char* what, const char* file, int line) {
 
 using exc_detail::stream;
 
+namespace exc_prof {
+
+namespace {
+struct Rec {
+  std::string name;
+  hipEvent_t a, b;
+  double flops, bytes;
+};
+std::mutex g_mu;
+bool g_on = false;
+std::vector<Rec> g_recs;
+std::vector<hipEvent_t> g_pool;
+}  // namespace
+
+bool enabled() { return g_on; }
+
+hipEvent_t mark(hipStream_t s) {
+  if (!g_on) return nullptr;
+  hipEvent_t e = nullptr;
+  {
+    std::lock_guard<std::mutex> g(g_mu);
+    if (!g_pool.empty()) {
+      e = g_pool.back();
+      g_pool.pop_back();
+    }
+  }
+  if (!e && hipEventCreate(&e) != hipSuccess) return nullptr;
+  hipEventRecord(e, s);
+  return e;
+}
+
+void book(const char* name, hipEvent_t a, hipEvent_t b, double flops, double bytes) {
+  if (!a || !b) return;
+  std::lock_guard<std::mutex> g(g_mu);
+  g_recs.push_back(Rec{name, a, b, flops, bytes});
+}
+
+}  // namespace exc_prof
+
 extern "C" {
 
 const char* exc_last_error(void) { return exc_detail::g_last_error.c_str(); }
@@ -115,6 +154,51 @@ int exc_stream_destroy(exc_stream_t s) {
 
 int exc_stream_sync(exc_stream_t s) {
   EXC_CHECK_HIP(hipStreamSynchronize(stream(s)));
+  return EXC_OK;
+}
+
+int exc_prof_enable(int on) {
+  exc_prof::g_on = on != 0;
+  return EXC_OK;
+}
+
+// {"name": {"launches": n, "ms": total, "flops": total, "bytes": total}, ...}; clears the records.
+int exc_prof_report(char* buf, size_t cap) {
+  EXC_REQUIRE(buf && cap > 2, EXC_ERR_ARG, "exc_prof_report: bad buffer");
+  std::lock_guard<std::mutex> g(exc_prof::g_mu);
+  struct Agg {
+    long n = 0;
+    double ms = 0, flops = 0, bytes = 0;
+  };
+  std::vector<std::pair<std::string, Agg>> agg;
+  for (auto& r : exc_prof::g_recs) {
+    hipEventSynchronize(r.b);
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, r.a, r.b);
+    auto it = std::find_if(agg.begin(), agg.end(), [&](auto& p) { return p.first == r.name; });
+    if (it == agg.end()) {
+      agg.push_back({r.name, Agg()});
+      it = agg.end() - 1;
+    }
+    it->second.n++;
+    it->second.ms += ms;
+    it->second.flops += r.flops;
+    it->second.bytes += r.bytes;
+    exc_prof::g_pool.push_back(r.a);
+    exc_prof::g_pool.push_back(r.b);
+  }
+  exc_prof::g_recs.clear();
+  std::string out = "{";
+  for (size_t i = 0; i < agg.size(); i++) {
+    char tmp[512];
+    snprintf(tmp, sizeof tmp, "%s\"%s\": {\"launches\": %ld, \"ms\": %.6f, \"flops\": %.6e, \"bytes\": %.6e}",
+             i ? ", " : "", agg[i].first.c_str(), agg[i].second.n, agg[i].second.ms, agg[i].second.flops,
+             agg[i].second.bytes);
+    out += tmp;
+  }
+  out += "}";
+  EXC_REQUIRE(out.size() + 1 <= cap, EXC_ERR_SPACE, "exc_prof_report: buffer too small");
+  memcpy(buf, out.c_str(), out.size() + 1);
   return EXC_OK;
 }
 

@@ -49,3 +49,12 @@ __device__ inline double wave_max(double v) {
   for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, kWave));
   return v;
 }
+
+// Live kernel timing with HIP events on the launch stream (off unless exc_prof_enable(1)).
+namespace exc_prof {
+bool enabled();
+// Returns an event recorded on `s` now (nullptr when profiling is off).
+hipEvent_t mark(hipStream_t s);
+// Books the interval between two marks under `name` with its algorithmic flops and bytes.
+void book(const char* name, hipEvent_t start, hipEvent_t end, double flops, double bytes);
+}  // namespace exc_prof

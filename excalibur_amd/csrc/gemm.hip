@@ -243,8 +243,17 @@ int gemm_impl(int ta, int tb, long M, long N, long K, double alpha, const T* A, 
     g.part = o.work;
   }
   dim3 grid((unsigned)tn, (unsigned)tm, (unsigned)splits);
+  hipEvent_t e0 = exc_prof::mark(s);
   hipLaunchKernelGGL(k_gemm<T>, grid, dim3(NT), 0, s, g);
   EXC_CHECK_HIP(hipGetLastError());
+  if (e0) {
+    // useful flops: a triangular operand halves the products, a symmetric (lower-only) result too
+    double flops = 2.0 * (double)M * (double)N * (double)K;
+    if (o.tri != EXC_TRI_NONE) flops *= 0.5;
+    if (o.lower_only) flops *= 0.5;
+    const double bytes = (double)sizeof(T) * ((double)M * K + (double)K * N + (double)M * N * (beta != 0.0 ? 2 : 1));
+    exc_prof::book(sizeof(T) == 8 ? "gemm_f64" : "gemm_f32", e0, exc_prof::mark(s), flops, bytes);
+  }
   if (splits > 1) {
     hipLaunchKernelGGL(k_splitk_reduce<T>, dim3(exc_detail::ceil_div(M * N, 256)), dim3(256), 0, s, g, splits);
     EXC_CHECK_HIP(hipGetLastError());

@@ -120,6 +120,7 @@ int givens_launch(T* x, long m, long n, const int32_t* pairs, const double* cs, 
   if (w > 64) w = 64;
   const size_t lds = (size_t)w * line_bytes;
   const unsigned grid = exc_detail::ceil_div(lines, w);
+  hipEvent_t e0 = exc_prof::mark(s);
   if (is_left) {
     if (lds > 64 * 1024)
       EXC_CHECK_HIP(hipFuncSetAttribute((const void*)k_givens<T, true>,
@@ -134,6 +135,12 @@ int givens_launch(T* x, long m, long n, const int32_t* pairs, const double* cs, 
                        (int)passes, half, mask, odd_mode);
   }
   EXC_CHECK_HIP(hipGetLastError());
+  if (e0) {
+    // one read and one write of the matrix; 6 flops per element per rotation
+    const double bytes = 2.0 * sizeof(T) * (double)m * (double)n;
+    const double flops = 6.0 * (double)passes * half * (double)lines * 2.0 / 2.0 * 2.0 / 2.0;
+    exc_prof::book(sizeof(T) == 8 ? "givens_f64" : "givens_f32", e0, exc_prof::mark(s), flops, bytes);
+  }
   return EXC_OK;
 }
 

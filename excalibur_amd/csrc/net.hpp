@@ -44,6 +44,7 @@ class PeerNet {
 
   double comm_seconds() const { return comm_s_; }
   void add_comm_seconds(double s) { comm_s_ += s; }
+  void reset_comm() { comm_s_ = 0.0; }
 
  private:
   struct Inbox {

@@ -27,7 +27,8 @@ int main(int argc, char** argv) {
       {"seed", required_argument, nullptr, 's'},   {"opt", required_argument, nullptr, 'a'},
       {"help", no_argument, nullptr, 'h'},         {"form", required_argument, nullptr, 1000},
       {"dump", required_argument, nullptr, 1001},  {"recv-timeout", required_argument, nullptr, 1002},
-      {"refine-iters", required_argument, nullptr, 1003}, {nullptr, 0, nullptr, 0}};
+      {"refine-iters", required_argument, nullptr, 1003}, {"repeat", required_argument, nullptr, 1004},
+      {"profile", required_argument, nullptr, 1005},      {nullptr, 0, nullptr, 0}};
   int c, index = 0;
   while ((c = getopt_long(argc, argv, "l:p:m:n:k:o:d:g:t:c:e:f:s:a:h", long_options, &index)) != -1) {
     switch (c) {
@@ -53,6 +54,8 @@ int main(int argc, char** argv) {
       case 1001: options.dump_results = atoi(optarg) == 1; break;
       case 1002: options.recv_timeout_s = atof(optarg); break;
       case 1003: options.refine_iters = atoi(optarg); break;
+      case 1004: options.repeat = atoi(optarg); break;
+      case 1005: options.profile = atoi(optarg) == 1; break;
       case 'h':
         printf(" -l --listen base server IP\n -p --port base server port\n -m --drow rows\n -n --dcol local columns\n"
                " -d --dpath data dir\n -g --logdir log dir\n -k --topk top-k (-1 all)\n -o --mode 0..3\n"

@@ -38,6 +38,8 @@ SIGNATURES = {
     "exc_stream_create": (_I, [ctypes.POINTER(_vp)]),
     "exc_stream_destroy": (_I, [_vp]),
     "exc_stream_sync": (_I, [_vp]),
+    "exc_prof_enable": (_I, [_I]),
+    "exc_prof_report": (_I, [ctypes.c_char_p, _S]),
     "exc_sturm_count": (_I, [_I, _dp, _dp, _dp, _I, _dp, _vp]),
     "exc_bisection_workspace": (_S, [_I, _I]),
     "exc_bisection": (_I, [_I, _dp, _dp, _D, _D, _D, _I, _dp, _dp, _dp, _dp, _dp, _S, _vp]),

@@ -35,6 +35,8 @@ struct ClientOptions {
   std::string form = "fp64";      // "fp64" (plain) or "mixed" (FP32 bulk + FP64 refinement)
   int refine_iters = 2;           // refinement sweeps of the mixed form
   const double* data = nullptr;   // optional in-memory input (m x local_n row-major) instead of dpath
+  int repeat = 1;                 // run() this many times on the same input (benchmarking)
+  bool profile = false;           // time the MFMA GEMM / Givens launches with HIP events
 };
 
 class DecentralizedClient {

@@ -51,6 +51,11 @@ int exc_memset(void* d_dst, int value, size_t bytes, exc_stream_t stream);
 int exc_stream_create(exc_stream_t* stream);
 int exc_stream_destroy(exc_stream_t stream);
 int exc_stream_sync(exc_stream_t stream);
+/* Live timing of the MFMA GEMM and Givens launches with HIP events on their own stream:
+ * exc_prof_enable(1) starts recording; exc_prof_report writes a JSON object
+ * {"kernel": {"launches", "ms", "flops", "bytes"}} (algorithmic flops and bytes) and clears. */
+int exc_prof_enable(int on);
+int exc_prof_report(char* buf, size_t cap);
 
 /* ---------------------------------------------------------------- bidiagonal SVD (FP64) */
 
