@@ -400,13 +400,12 @@ void DecentralizedClient::run() {
       ck(exc_orgqr_t((int)m, (long)R, D.Vt.p, (long)R, D.tau_g.p, D.Qg.p, (long)R, D.work.p, D.work.n * 8, s),
          "global Q");
     }
-    ck(exc_transpose((long)c, (long)m, D.P.p, (long)R, D.F.p, (long)c, s), "shrink");
     sync();
     D.Vt.release();
-  } else {
-    ck(exc_transpose((long)c, (long)R, D.P.p, (long)R, D.F.p, (long)c, s), "transpose");
-    sync();
   }
+  // F = R[:, own] (m x c) kept column-major for the Ralha loop: rows of P (c x R) are its columns
+  ck(exc_copy2d((long)c, (long)m, D.P.p, (long)R, D.F.p, (long)m, s), "shrink");
+  sync();
   D.P.release();
   time_global_qr = now_seconds() - t0;
 
@@ -434,21 +433,32 @@ void DecentralizedClient::run() {
       ar += now_seconds() - ta;
       ckh(hipMemcpyAsync(ddst, hb.p, n * sizeof(double), hipMemcpyHostToDevice, s), "h2d");
     };
-    ck(exc_ralha_tail_update(-1, (int)c, D.F.p, D.U.p, nullptr, D.beta.p, D.scal.p, s), "ralha init");
+    const int mi = (int)m, ci = (int)c;
+    ck(exc_ralha_tail_update(-1, mi, ci, D.F.p, D.U.p, nullptr, D.beta.p, D.scal.p, s), "ralha init");
     allreduce_dev(D.scal.p, D.scal.p + 1, 1);
-    ck(exc_ralha_tail_scale(0, (int)c, D.F.p, D.U.p, D.scal.p + 1, D.alpha.p, s), "ralha init");
-    for (int i = 0; i + 2 < (int)m; i++) {
-      ck(exc_ralha_msg(i, (int)m, (int)c, D.F.p, D.U.p, D.msg.p, s), "ralha msg");
-      allreduce_dev(D.msg.p, D.g.p, (long)m + 3 - i);
-      ck(exc_ralha_apply(i, (int)m, (int)c, D.F.p, D.U.p, D.g.p, D.house.p, D.tau_h.p, D.alpha.p, D.beta.p, s),
-         "ralha apply");
+    ck(exc_ralha_tail_scale(0, mi, ci, D.F.p, D.U.p, D.scal.p + 1, D.alpha.p, s), "ralha init");
+    const size_t rwb = exc_ralha_workspace(mi, ci);
+    D.work.alloc((rwb + 7) / 8, s);
+    // the step kernels write the local message straight into pinned host memory
+    ck(exc_ralha_step(-1, mi, ci, D.F.p, D.U.p, nullptr, D.house.p, D.tau_h.p, D.alpha.p, D.beta.p, hb.p, D.work.p,
+                      D.work.n * 8, s),
+       "ralha step");
+    for (int i = 0; i + 2 < mi; i++) {
+      sync();
+      const double ta = now_seconds();
+      net.ring_all_reduce_sum(hb.p, (long)m + 3 - i);
+      ar += now_seconds() - ta;
+      ckh(hipMemcpyAsync(D.g.p, hb.p, ((long)m + 3 - i) * sizeof(double), hipMemcpyHostToDevice, s), "h2d");
+      ck(exc_ralha_step(i, mi, ci, D.F.p, D.U.p, D.g.p, D.house.p, D.tau_h.p, D.alpha.p, D.beta.p, hb.p, D.work.p,
+                        D.work.n * 8, s),
+         "ralha step");
     }
-    for (int i = (int)m - 2; i < (int)m; i++) {
-      ck(exc_ralha_dot(i, (int)c, D.F.p, D.U.p, D.scal.p, s), "ralha dot");
+    for (int i = mi - 2; i < mi; i++) {
+      ck(exc_ralha_dot(i, mi, ci, D.F.p, D.U.p, D.scal.p, s), "ralha dot");
       allreduce_dev(D.scal.p, D.scal.p + 1, 1);
-      ck(exc_ralha_tail_update(i, (int)c, D.F.p, D.U.p, D.scal.p + 1, D.beta.p, D.scal.p + 2, s), "ralha tail");
+      ck(exc_ralha_tail_update(i, mi, ci, D.F.p, D.U.p, D.scal.p + 1, D.beta.p, D.scal.p + 2, s), "ralha tail");
       allreduce_dev(D.scal.p + 2, D.scal.p + 3, 1);
-      ck(exc_ralha_tail_scale(i, (int)c, D.F.p, D.U.p, D.scal.p + 3, D.alpha.p, s), "ralha tail");
+      ck(exc_ralha_tail_scale(i, mi, ci, D.F.p, D.U.p, D.scal.p + 3, D.alpha.p, s), "ralha tail");
     }
     ck(exc_lower_to_upper((int)m, D.alpha.p, D.beta.p, D.cs.p, s), "lower to upper");
     sync();
@@ -494,12 +504,11 @@ void DecentralizedClient::run() {
   // 8) combine and unmask (client.cpp:998-1199)
   t0 = now_seconds();
   if (compute_u) {
-    D.part.alloc((size_t)k * c, s);
-    ck(exc_dgemm(0, 0, k, (long)c, (long)m, 1.0, D.bv.p, (long)m, D.U.p, (long)c, 0.0, D.part.p, (long)c, nullptr, 0,
-                 s),
-       "part final U");
+    // this peer's rows of FinalU: (bv U)^T = U^T bv^T, with U^T stored row-major (c x m)
     D.FinalU.alloc(m * (size_t)k, s);
-    ck(exc_transpose(k, (long)c, D.part.p, (long)c, D.FinalU.p + m_part_pos[cid] * (size_t)k, k, s), "final U");
+    ck(exc_dgemm(0, 1, (long)c, k, (long)m, 1.0, D.U.p, (long)m, D.bv.p, (long)m, 0.0,
+                 D.FinalU.p + m_part_pos[cid] * (size_t)k, k, nullptr, 0, s),
+       "part final U");
     // ring all-gather of the row blocks of FinalU (client.cpp:1062-1096)
     const double tc = now_seconds();
     HBuf& hU = D.h_U;
@@ -518,7 +527,6 @@ void DecentralizedClient::run() {
     ckh(hipMemcpyAsync(D.FinalU.p, hU.p, m * (size_t)k * 8, hipMemcpyHostToDevice, s), "h2d");
     sync();
     net.add_comm_seconds(now_seconds() - tc);
-    D.part.release();
     givens_mask(D.FinalU.p, (long)m, k, 0, 1, public_seed, D.plan, s);
   }
   D.U.release();

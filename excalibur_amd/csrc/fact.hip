@@ -84,53 +84,68 @@ __global__ __launch_bounds__(256) void k_getf2_sign(int kb, double* __restrict__
   }
 }
 
-// X_b <- op(L_bb)^-1 X_b for one diagonal block, one lane per right-hand side. The block of X
-// (kb x 64) lives in LDS, the triangle is read with wave-uniform addresses (scalar loads).
+// X_b <- op(L_bb)^-1 X_b for one diagonal block (kb <= NB), one lane per right-hand side: the
+// column lives in registers and the triangle is read with wave-uniform addresses (scalar loads),
+// so the substitution runs at VALU rate with no LDS traffic.
 template <bool UPPER, bool UNIT>
 __global__ __launch_bounds__(64) void k_trsm_left_blk(int kb, const double* __restrict__ L, long ldl,
                                                       double* __restrict__ X, long ldx, long nrhs) {
-  __shared__ double T[NB][64];
-  const int c = threadIdx.x;
-  const long col = (long)blockIdx.x * 64 + c;
-  const bool live = col < nrhs;
-  for (int i = 0; i < kb; i++) T[i][c] = live ? X[(long)i * ldx + col] : 0.0;
+  const long col = (long)blockIdx.x * 64 + threadIdx.x;
+  if (col >= nrhs) return;
+  double x[NB];
+#pragma unroll
+  for (int i = 0; i < NB; i++) x[i] = i < kb ? X[(long)i * ldx + col] : 0.0;
   if (!UPPER) {
-    for (int i = 0; i < kb; i++) {
-      double acc = T[i][c];
-      for (int k = 0; k < i; k++) acc -= L[(long)i * ldl + k] * T[k][c];
-      T[i][c] = UNIT ? acc : acc / L[(long)i * ldl + i];
+#pragma unroll
+    for (int i = 0; i < NB; i++) {
+      if (i < kb) {
+        double acc = x[i];
+#pragma unroll
+        for (int k = 0; k < i; k++) acc -= L[(long)i * ldl + k] * x[k];
+        x[i] = UNIT ? acc : acc / L[(long)i * ldl + i];
+      }
     }
   } else {
-    for (int i = kb - 1; i >= 0; i--) {
-      double acc = T[i][c];
-      for (int k = i + 1; k < kb; k++) acc -= L[(long)i * ldl + k] * T[k][c];
-      T[i][c] = UNIT ? acc : acc / L[(long)i * ldl + i];
+#pragma unroll
+    for (int i = NB - 1; i >= 0; i--) {
+      if (i < kb) {
+        double acc = x[i];
+#pragma unroll
+        for (int k = i + 1; k < NB; k++)
+          if (k < kb) acc -= L[(long)i * ldl + k] * x[k];
+        x[i] = UNIT ? acc : acc / L[(long)i * ldl + i];
+      }
     }
   }
-  if (live)
-    for (int i = 0; i < kb; i++) X[(long)i * ldx + col] = T[i][c];
+#pragma unroll
+  for (int i = 0; i < NB; i++)
+    if (i < kb) X[(long)i * ldx + col] = x[i];
 }
 
-// X_b <- X_b U_bb^-1 (or X_b L_bb^-T) for one diagonal block, one lane per row.
+// X_b <- X_b U_bb^-1 (or X_b L_bb^-T) for one diagonal block, one lane per row (in registers).
 template <bool LOWER_T, bool UNIT>
 __global__ __launch_bounds__(64) void k_trsm_right_blk(int kb, const double* __restrict__ U, long ldu,
                                                        double* __restrict__ X, long ldx, long rows) {
-  __shared__ double T[NB][65];
-  const int r = threadIdx.x;
-  const long row = (long)blockIdx.x * 64 + r;
-  const bool live = row < rows;
-  for (int j = 0; j < kb; j++) T[j][r] = live ? X[row * ldx + j] : 0.0;
-  for (int j = 0; j < kb; j++) {
-    double acc = T[j][r];
-    for (int k = 0; k < j; k++) {
-      const double ukj = LOWER_T ? U[(long)j * ldu + k] : U[(long)k * ldu + j];
-      acc -= T[k][r] * ukj;
+  const long row = (long)blockIdx.x * 64 + threadIdx.x;
+  if (row >= rows) return;
+  double x[NB];
+#pragma unroll
+  for (int j = 0; j < NB; j++) x[j] = j < kb ? X[row * ldx + j] : 0.0;
+#pragma unroll
+  for (int j = 0; j < NB; j++) {
+    if (j < kb) {
+      double acc = x[j];
+#pragma unroll
+      for (int k = 0; k < j; k++) {
+        const double ukj = LOWER_T ? U[(long)j * ldu + k] : U[(long)k * ldu + j];
+        acc -= x[k] * ukj;
+      }
+      x[j] = UNIT ? acc : acc / U[(long)j * ldu + j];
     }
-    const double ujj = U[(long)j * ldu + j];
-    T[j][r] = UNIT ? acc : acc / ujj;
   }
-  if (live)
-    for (int j = 0; j < kb; j++) X[row * ldx + j] = T[j][r];
+#pragma unroll
+  for (int j = 0; j < NB; j++)
+    if (j < kb) X[row * ldx + j] = x[j];
 }
 
 __global__ __launch_bounds__(256) void k_zero_strict_upper(int n, double* A, long lda) {

@@ -7,7 +7,7 @@
 
 namespace fact {
 
-constexpr int NB = 128;  // diagonal block size (held in LDS)
+constexpr int NB = 64;  // diagonal block size (LDS for the factorizations, registers for the solves)
 
 // In-place lower Cholesky of the symmetric n x n matrix A (lower triangle read); the strict
 // upper triangle is zeroed. info (device int, nullable) receives 0 or the 1-based index of the

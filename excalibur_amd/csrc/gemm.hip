@@ -67,11 +67,34 @@ __device__ inline void k_range(const GemmArgs& g, long i0, long j0, long& k0, lo
   }
 }
 
-template <typename T>
-__global__ __launch_bounds__(NT) void k_gemm(GemmArgs g) {
+// One k-chunk of A and B (BM x BK and BK x BN) staged through registers into LDS. The per-thread
+// element coordinates do not change between chunks: for a non-transposed A every thread owns one
+// k column (tid % 16) of 8 rows 16 apart; for a transposed A one row (tid % 128) of 8 k steps 2
+// apart (and likewise for B), so each chunk is 8 loads at a fixed stride from one base pointer.
+template <typename T, int TA, int TB>
+struct Stager {
+  const T* pa;  // base of this thread's A elements at k = 0
+  const T* pb;
+  long sa, sb;  // stride between the thread's 8 elements
+  long ka, kb;  // stride of one k step
+  int ai, ak, bj, bk;  // LDS coordinates of element q = 0 (q advances ai or ak by 16 / 2)
+  __device__ Stager(const GemmArgs& g, long i0, long j0, int tid) {
+    if (!TA) { ai = tid >> 4; ak = tid & 15; pa = (const T*)g.A + (i0 + ai) * g.lda + ak; sa = 16 * g.lda; ka = 1; }
+    else { ak = tid >> 7; ai = tid & 127; pa = (const T*)g.A + (long)ak * g.lda + i0 + ai; sa = 2 * g.lda; ka = g.lda; }
+    if (!TB) { bk = tid >> 7; bj = tid & 127; pb = (const T*)g.B + (long)bk * g.ldb + j0 + bj; sb = 2 * g.ldb; kb = g.ldb; }
+    else { bj = tid >> 4; bk = tid & 15; pb = (const T*)g.B + (j0 + bj) * g.ldb + bk; sb = 16 * g.ldb; kb = 1; }
+  }
+  __device__ inline int a_i(int q) const { return TA ? ai : ai + 16 * q; }
+  __device__ inline int a_k(int q) const { return TA ? ak + 2 * q : ak; }
+  __device__ inline int b_j(int q) const { return TB ? bj + 16 * q : bj; }
+  __device__ inline int b_k(int q) const { return TB ? bk : bk + 2 * q; }
+};
+
+template <typename T, int TA, int TB>
+__global__ __launch_bounds__(NT, 2) void k_gemm(GemmArgs g) {
   using acc_t = typename Mfma<T>::acc_t;
-  __shared__ T As[2][BK][BM + 4];
-  __shared__ T Bs[2][BK][BN + 4];
+  __shared__ T As[2][BK][BM + 1];
+  __shared__ T Bs[2][BK][BN + 1];
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
   const long tile_m = blockIdx.y, tile_n = blockIdx.x;
@@ -84,66 +107,61 @@ __global__ __launch_bounds__(NT) void k_gemm(GemmArgs g) {
     k0 = max(k0, ks);
     k1 = min(k1, ks + g.kchunk);
   }
-  const T* __restrict__ A = (const T*)g.A;
-  const T* __restrict__ B = (const T*)g.B;
+  const Stager<T, TA, TB> st(g, i0, j0, tid);
+  // interior: no bounds, no triangle, whole chunks
+  const bool interior = i0 + BM <= g.M && j0 + BN <= g.N && g.tri == EXC_TRI_NONE && (k0 % BK) == 0 &&
+                        ((k1 - k0) % BK) == 0;
   const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
   acc_t acc[4][4];
 #pragma unroll
   for (int a = 0; a < 4; a++)
 #pragma unroll
     for (int b = 0; b < 4; b++) acc[a][b] = acc_t{0, 0, 0, 0};
-
-  // global -> register staging: 8 elements of A and 8 of B per thread per chunk
   T ra[8], rb[8];
-  auto load_chunk = [&](long kb) {
+  auto load_fast = [&](long kbase) {
+    const T* pa = st.pa + kbase * st.ka;
+    const T* pb = st.pb + kbase * st.kb;
+#pragma unroll
+    for (int q = 0; q < 8; q++) ra[q] = pa[q * st.sa];
+#pragma unroll
+    for (int q = 0; q < 8; q++) rb[q] = pb[q * st.sb];
+  };
+  auto load_checked = [&](long kbase) {
 #pragma unroll
     for (int q = 0; q < 8; q++) {
-      const int e = tid + q * NT;  // 0..2047
-      // A tile element (i in 0..127, k in 0..15)
-      int ii, kk;
-      if (!g.ta) { ii = e >> 4; kk = e & 15; }   // rows of A contiguous in k
-      else { kk = e >> 7; ii = e & 127; }        // A stored K x M: contiguous in i
-      const long gi = i0 + ii, gk = kb + kk;
+      const long gi = i0 + st.a_i(q), gk = kbase + st.a_k(q);
       T v = T(0);
-      if (gi < g.M && gk < k1 && gk >= k0) v = g.ta ? A[gk * g.lda + gi] : A[gi * g.lda + gk];
+      if (gi < g.M && gk >= k0 && gk < k1) v = st.pa[kbase * st.ka + q * st.sa];
       if ((g.tri == EXC_TRI_A_LOWER && gk > gi) || (g.tri == EXC_TRI_A_UPPER && gk < gi)) v = T(0);
       ra[q] = v;
-      int jj, kb2;
-      if (!g.tb) { kb2 = e >> 7; jj = e & 127; }  // B stored K x N: contiguous in j
-      else { jj = e >> 4; kb2 = e & 15; }         // B stored N x K: contiguous in k
-      const long gj = j0 + jj, gk2 = kb + kb2;
+      const long gj = j0 + st.b_j(q), gk2 = kbase + st.b_k(q);
       T w = T(0);
-      if (gj < g.N && gk2 < k1 && gk2 >= k0) w = g.tb ? B[gj * g.ldb + gk2] : B[gk2 * g.ldb + gj];
+      if (gj < g.N && gk2 >= k0 && gk2 < k1) w = st.pb[kbase * st.kb + q * st.sb];
       if ((g.tri == EXC_TRI_B_LOWER && gk2 < gj) || (g.tri == EXC_TRI_B_UPPER && gk2 > gj)) w = T(0);
       rb[q] = w;
     }
   };
   auto store_chunk = [&](int buf) {
 #pragma unroll
-    for (int q = 0; q < 8; q++) {
-      const int e = tid + q * NT;
-      int ii, kk;
-      if (!g.ta) { ii = e >> 4; kk = e & 15; }
-      else { kk = e >> 7; ii = e & 127; }
-      As[buf][kk][ii] = ra[q];
-      int jj, kb2;
-      if (!g.tb) { kb2 = e >> 7; jj = e & 127; }
-      else { jj = e >> 4; kb2 = e & 15; }
-      Bs[buf][kb2][jj] = rb[q];
-    }
+    for (int q = 0; q < 8; q++) As[buf][st.a_k(q)][st.a_i(q)] = ra[q];
+#pragma unroll
+    for (int q = 0; q < 8; q++) Bs[buf][st.b_k(q)][st.b_j(q)] = rb[q];
   };
-
   const long kstart = (k0 / BK) * BK;
   int buf = 0;
   if (kstart < k1) {
-    load_chunk(kstart);
+    if (interior) load_fast(kstart);
+    else load_checked(kstart);
     store_chunk(0);
   }
   __syncthreads();
   const int li = lane & 15, lk = lane >> 4;
   for (long kb = kstart; kb < k1; kb += BK) {
     const bool more = kb + BK < k1;
-    if (more) load_chunk(kb + BK);
+    if (more) {
+      if (interior) load_fast(kb + BK);
+      else load_checked(kb + BK);
+    }
 #pragma unroll
     for (int ks = 0; ks < BK; ks += 4) {
       T af[4], bf[4];
@@ -166,6 +184,7 @@ __global__ __launch_bounds__(NT) void k_gemm(GemmArgs g) {
   T* C = g.kchunk > 0 ? (T*)g.part + (size_t)blockIdx.z * g.M * g.N : (T*)g.C;
   const long ldc = g.kchunk > 0 ? g.N : g.ldc;
   const bool direct = g.kchunk == 0;
+  const T alpha = (T)g.alpha, beta = (T)g.beta;
 #pragma unroll
   for (int a = 0; a < 4; a++)
 #pragma unroll
@@ -177,8 +196,8 @@ __global__ __launch_bounds__(NT) void k_gemm(GemmArgs g) {
         if (gi < g.M && gj < g.N) {
           const T val = acc[a][b][v];
           if (direct) {
-            T out = (T)g.alpha * val;
-            if (g.beta != 0.0) out += (T)g.beta * C[gi * ldc + gj];
+            T out = alpha * val;
+            if (g.beta != 0.0) out += beta * C[gi * ldc + gj];
             C[gi * ldc + gj] = out;
           } else {
             C[gi * ldc + gj] = val;
@@ -244,7 +263,10 @@ int gemm_impl(int ta, int tb, long M, long N, long K, double alpha, const T* A, 
   }
   dim3 grid((unsigned)tn, (unsigned)tm, (unsigned)splits);
   hipEvent_t e0 = exc_prof::mark(s);
-  hipLaunchKernelGGL(k_gemm<T>, grid, dim3(NT), 0, s, g);
+  if (!ta && !tb) hipLaunchKernelGGL((k_gemm<T, 0, 0>), grid, dim3(NT), 0, s, g);
+  else if (!ta && tb) hipLaunchKernelGGL((k_gemm<T, 0, 1>), grid, dim3(NT), 0, s, g);
+  else if (ta && !tb) hipLaunchKernelGGL((k_gemm<T, 1, 0>), grid, dim3(NT), 0, s, g);
+  else hipLaunchKernelGGL((k_gemm<T, 1, 1>), grid, dim3(NT), 0, s, g);
   EXC_CHECK_HIP(hipGetLastError());
   if (e0) {
     // useful flops: a triangular operand halves the products, a symmetric (lower-only) result too

@@ -1,50 +1,19 @@
 // One-sided (Ralha) bidiagonalization step kernels (client.cpp:829-910) and the small pipeline
 // kernels around the bidiagonal SVD (client.cpp:947-954, 987-995, 1127-1128).
 //
-// F (m x c, row-major) holds this peer's share of the rows f_j; U (m x c) the u_j; the reductions
-// across peers happen on the host between the two kernels of a step:
-//   ralha_msg(i):   w_r = f_{i+1+r} . f_i (r < m-1-i) and four dots with u_{i-1}  -> message
-//   (host: all-reduce of the message over the peers)
-//   ralha_apply(i): dlarfg of the reduced w, F[i+1:] -= tau h (h^T F[i+1:]), f_i -= beta u_{i-1},
-//                   alpha_i from the reduced dots, u_i = f_i / alpha_i.
-// The message kernel is row-parallel (one wavefront per row, fixed-order reduction); the apply
-// kernel is column-parallel (a workgroup owns a slab of columns, so the rank-1 update needs no
-// cross-workgroup synchronization). Every workgroup recomputes the reflector from the reduced
-// message with the same fixed reduction order, so all of them see the same h and tau.
+// This peer's share of the rows f_j (the reference's row-major m x c F) is kept column-major,
+// Fc[col * m + j], so every column is contiguous; U likewise. One kernel per column step applies
+// step i column-locally (a workgroup owns four columns, one wavefront each, and recomputes the
+// reflector from the reduced message in a fixed order, so all workgroups see the same h and tau)
+// and then writes its partial sums of step i+1's message; a second kernel adds the partials in
+// workgroup order. The host all-reduces that message over the peers between steps.
 #include <cfloat>
 
 #include "exc_common.hpp"
 
 namespace {
 
-constexpr int kColsPerWg = 16;  // columns per apply workgroup
-constexpr int kRowGroups = 16;  // 16 x 16 = 256 threads
-
-// msg[r] = F[i+1+r] . F[i] for r < m-1-i; if i > 0, msg[m-1-i..m+2-i] = f_i.u_{i-1},
-// u_{i-1}.u_{i-1}, f_i.f_i, u_{i-1}.f_i (client.cpp:842-855).
-__global__ __launch_bounds__(256) void k_ralha_msg(int i, int m, int c, const double* __restrict__ F,
-                                                   const double* __restrict__ U, double* __restrict__ msg) {
-  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const int lane = threadIdx.x & 63;
-  const int nrows = m - 1 - i;
-  const int total = nrows + (i > 0 ? 4 : 0);
-  if (wave >= total) return;
-  const double *x, *y;
-  const double* fi = F + (long)i * c;
-  if (wave < nrows) {
-    x = F + (long)(i + 1 + wave) * c;
-    y = fi;
-  } else {
-    const double* ui = U + (long)(i - 1) * c;
-    const int q = wave - nrows;
-    x = q == 0 ? fi : (q == 1 ? ui : (q == 2 ? fi : ui));
-    y = q == 0 ? ui : (q == 1 ? ui : (q == 2 ? fi : fi));
-  }
-  double s = 0.0;
-  for (int k = lane; k < c; k += 64) s += x[k] * y[k];
-  s = wave_sum(s);
-  if (lane == 0) msg[wave] = s;
-}
+constexpr int kColsPerWg = 4;  // columns per step workgroup (one wavefront each)
 
 __device__ inline double dlapy2_dev(double x, double y) {
   const double xa = fabs(x), ya = fabs(y);
@@ -72,102 +41,127 @@ __device__ inline double block_sum(double v, double* red) {
   return out;
 }
 
-// The reduced message g (length m+3-i) -> reflector of LAPACK's dlarfg (n = m-1-i) and the
-// update of this workgroup's columns.
-__global__ __launch_bounds__(256) void k_ralha_apply(int i, int m, int c, double* __restrict__ F,
-                                                     double* __restrict__ U, const double* __restrict__ g,
-                                                     double* __restrict__ house, double* __restrict__ tau,
-                                                     double* __restrict__ alpha, double* __restrict__ beta) {
+// One column step of the overlapped Ralha loop on column-major F and U (Fc[col*m + row]):
+//  apply (i >= 0): the reduced message g of step i -> LAPACK dlarfg (n = m-1-i) into column i+1
+//    of house and tau[i+1]; F[i+1:, col] -= tau h (h . F[i+1:, col]) for this workgroup's columns
+//    (one wavefront per column); f_i -= beta_{i-1} u_{i-1}, alpha_i, u_i = f_i / alpha_i
+//    (client.cpp:858-891);
+//  message (j = i+1 <= m-3): this workgroup's share of step j's message (f_{j+1+r} . f_j and
+//    the four dots with u_{j-1}, client.cpp:842-855) into its row of `partial`.
+__global__ __launch_bounds__(256) void k_ralha_step(int i, int m, int c, double* __restrict__ Fc,
+                                                    double* __restrict__ Uc, const double* __restrict__ g,
+                                                    double* __restrict__ house, double* __restrict__ tau,
+                                                    double* __restrict__ alpha, double* __restrict__ beta,
+                                                    double* __restrict__ partial) {
+  extern __shared__ double hbuf[];  // m doubles
   __shared__ double red[8];
-  __shared__ double hbuf[4096];
-  __shared__ double part[kRowGroups][kColsPerWg];
-  __shared__ double vts[kColsPerWg];
-  const int n = m - 1 - i;  // reflector length
-  const int tid = threadIdx.x;
-  // dlarfg: alpha_h = g[0], x = g[1..n-1]
-  double ss = 0.0;
-  for (int r = 1 + tid; r < n; r += blockDim.x) ss += g[r] * g[r];
-  double xnorm = sqrt(block_sum(ss, red));
-  double alph = g[0];
-  double t = 0.0, scale = 0.0, bet = alph;
-  int knt = 0;
-  const double safmin = DBL_MIN / (DBL_EPSILON * 0.5);
-  if (n > 1 && xnorm != 0.0) {
-    bet = -copysign(dlapy2_dev(alph, xnorm), alph);
-    double xs = 1.0;  // scaling of x during the safmin loop
-    if (fabs(bet) < safmin) {
-      const double rs = 1.0 / safmin;
-      do {
-        knt++;
-        xs *= rs;
-        bet *= rs;
-        alph *= rs;
-      } while (fabs(bet) < safmin && knt < 20);
-      double s2 = 0.0;
-      for (int r = 1 + tid; r < n; r += blockDim.x) s2 += (g[r] * xs) * (g[r] * xs);
-      xnorm = sqrt(block_sum(s2, red));
-      bet = -copysign(dlapy2_dev(alph, xnorm), alph);
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int c0 = blockIdx.x * kColsPerWg;
+  if (i >= 0) {
+    const int n = m - 1 - i;
+    double ss = 0.0;
+    for (int r = 1 + tid; r < n; r += blockDim.x) ss += g[r] * g[r];
+    double xnorm = sqrt(block_sum(ss, red));
+    double alph = g[0], t = 0.0, scale = 0.0;
+    if (n > 1 && xnorm != 0.0) {
+      double bet = -copysign(dlapy2_dev(alph, xnorm), alph);
+      double xs = 1.0;
+      const double safmin = DBL_MIN / (DBL_EPSILON * 0.5);
+      if (fabs(bet) < safmin) {
+        const double rs = 1.0 / safmin;
+        int knt = 0;
+        do {
+          knt++;
+          xs *= rs;
+          bet *= rs;
+          alph *= rs;
+        } while (fabs(bet) < safmin && knt < 20);
+        double s2 = 0.0;
+        for (int r = 1 + tid; r < n; r += blockDim.x) s2 += (g[r] * xs) * (g[r] * xs);
+        xnorm = sqrt(block_sum(s2, red));
+        bet = -copysign(dlapy2_dev(alph, xnorm), alph);
+      }
+      t = (bet - alph) / bet;
+      scale = xs * (1.0 / (alph - bet));
     }
-    t = (bet - alph) / bet;
-    scale = xs * (1.0 / (alph - bet));
-  }
-  // h[0] = 1, h[r] = x_r * scale; a zero tau leaves F untouched (H = I)
-  const bool big = n <= 4096;
-  for (int r = tid; r < n && big; r += blockDim.x) hbuf[r] = r == 0 ? 1.0 : g[r] * scale;
-  __syncthreads();
-  auto h_at = [&](int r) -> double { return big ? hbuf[r] : (r == 0 ? 1.0 : g[r] * scale); };
-  if (blockIdx.x == 0) {
-    for (int r = tid; r < n; r += blockDim.x) house[(long)(i + 1 + r) * m + (i + 1)] = h_at(r);
-    if (tid == 0) {
-      tau[i + 1] = t;
-      if (i > 0) beta[i - 1] = g[m - i - 1];
+    for (int r = tid; r < n; r += blockDim.x) hbuf[r] = r == 0 ? 1.0 : g[r] * scale;
+    __syncthreads();
+    const double b = i > 0 ? g[m - i - 1] : 0.0;
+    const double a = i > 0 ? sqrt(g[m - i] * (b * b) + g[m - i + 1] - 2 * b * g[m - i + 2]) : 0.0;
+    if (blockIdx.x == 0) {
+      for (int r = tid; r < n; r += blockDim.x) house[(long)(i + 1 + r) * m + (i + 1)] = hbuf[r];
+      if (tid == 0) {
+        tau[i + 1] = t;
+        if (i > 0) {
+          beta[i - 1] = b;
+          alpha[i] = a;
+        }
+      }
     }
-  }
-  // vt[col] = sum_r F[i+1+r][col] h[r]; F[i+1+r][col] -= tau h[r] vt[col]
-  const int cl = tid % kColsPerWg, rg = tid / kColsPerWg;
-  const int col = blockIdx.x * kColsPerWg + cl;
-  const bool live = col < c;
-  double acc = 0.0;
-  if (live && t != 0.0)
-    for (int r = rg; r < n; r += kRowGroups) acc += F[(long)(i + 1 + r) * c + col] * h_at(r);
-  part[rg][cl] = acc;
-  __syncthreads();
-  if (tid < kColsPerWg) {
-    double v = 0.0;
-    for (int q = 0; q < kRowGroups; q++) v += part[q][tid];
-    vts[tid] = v;
-  }
-  __syncthreads();
-  if (live && t != 0.0) {
-    const double vc = vts[cl];
-    for (int r = rg; r < n; r += kRowGroups) F[(long)(i + 1 + r) * c + col] += vc * (-t * h_at(r));
-  }
-  // f_i -= beta_{i-1} u_{i-1}; alpha_i; u_i = f_i / alpha_i (client.cpp:886-890)
-  if (i > 0) {
-    const double b = g[m - i - 1];
-    const double a = sqrt(g[m - i] * (b * b) + g[m - i + 1] - 2 * b * g[m - i + 2]);
-    if (blockIdx.x == 0 && tid == 0) alpha[i] = a;
-    if (rg == 0 && live) {
-      const double f = F[(long)i * c + col] + (-b) * U[(long)(i - 1) * c + col];
-      F[(long)i * c + col] = f;
-      U[(long)i * c + col] = (1.0 / a) * f;
+    for (int q = wave; q < kColsPerWg; q += (int)(blockDim.x >> 6)) {
+      const int col = c0 + q;
+      if (col >= c) break;
+      double* f = Fc + (long)col * m;
+      if (t != 0.0) {
+        double s = 0.0;
+        for (int r = lane; r < n; r += 64) s += f[i + 1 + r] * hbuf[r];
+        s = wave_sum(s);
+        for (int r = lane; r < n; r += 64) f[i + 1 + r] += s * (-t * hbuf[r]);
+      }
+      if (i > 0 && lane == 0) {
+        const double fi = f[i] + (-b) * Uc[(long)col * m + i - 1];
+        f[i] = fi;
+        Uc[(long)col * m + i] = (1.0 / a) * fi;
+      }
     }
+    __syncthreads();
+  }
+  const int j = i + 1;
+  if (j > m - 3) return;
+  const int nrows = m - 1 - j;
+  const int c1 = min(c, c0 + kColsPerWg);
+  double* out = partial + (long)blockIdx.x * (m + 3);
+  for (int r = tid; r < nrows; r += blockDim.x) {
+    double s = 0.0;
+    for (int col = c0; col < c1; col++) s += Fc[(long)col * m + j + 1 + r] * Fc[(long)col * m + j];
+    out[r] = s;
+  }
+  if (j > 0 && tid < 4) {
+    double s = 0.0;
+    for (int col = c0; col < c1; col++) {
+      const double f = Fc[(long)col * m + j], u = Uc[(long)col * m + j - 1];
+      s += tid == 0 ? f * u : (tid == 1 ? u * u : (tid == 2 ? f * f : u * f));
+    }
+    out[nrows + tid] = s;
   }
 }
 
-// dot = f_i . u_{i-1} (one workgroup)
-__global__ __launch_bounds__(256) void k_ralha_dot(int i, int c, const double* __restrict__ F,
-                                                   const double* __restrict__ U, double* __restrict__ out) {
+// msg[r] = sum over workgroups (in order) of partial[wg][r]; zeros after the valid entries.
+__global__ __launch_bounds__(256) void k_ralha_reduce(int j, int m, int nwg, const double* __restrict__ partial,
+                                                      double* __restrict__ msg) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  const int len = m + 3 - j;
+  if (r >= len) return;
+  const int valid = (m - 1 - j) + (j > 0 ? 4 : 0);
+  double s = 0.0;
+  if (r < valid)
+    for (int w = 0; w < nwg; w++) s += partial[(long)w * (m + 3) + r];
+  msg[r] = s;
+}
+
+// dot = f_i . u_{i-1} over the columns (column-major F and U)
+__global__ __launch_bounds__(256) void k_ralha_dot(int i, int m, int c, const double* __restrict__ Fc,
+                                                   const double* __restrict__ Uc, double* __restrict__ out) {
   __shared__ double red[8];
   double s = 0.0;
-  for (int k = threadIdx.x; k < c; k += blockDim.x) s += F[(long)i * c + k] * U[(long)(i - 1) * c + k];
+  for (int k = threadIdx.x; k < c; k += blockDim.x) s += Fc[(long)k * m + i] * Uc[(long)k * m + i - 1];
   s = block_sum(s, red);
   if (threadIdx.x == 0) out[0] = s;
 }
 
-// f_i -= beta u_{i-1} (beta from the reduced value), then out = ||f_i||^2 (i = -1: just ||f_0||^2)
-__global__ __launch_bounds__(256) void k_ralha_tail_update(int i, int c, double* __restrict__ F,
-                                                           const double* __restrict__ U,
+// f_i -= beta u_{i-1} (beta = reduced value), then out = ||f_i||^2 (i = -1: ||f_0||^2 only)
+__global__ __launch_bounds__(256) void k_ralha_tail_update(int i, int m, int c, double* __restrict__ Fc,
+                                                           const double* __restrict__ Uc,
                                                            const double* __restrict__ breduced,
                                                            double* __restrict__ beta, double* __restrict__ out) {
   __shared__ double red[8];
@@ -175,10 +169,10 @@ __global__ __launch_bounds__(256) void k_ralha_tail_update(int i, int c, double*
   double s = 0.0;
   const double b = i > 0 ? breduced[0] : 0.0;
   for (int k = threadIdx.x; k < c; k += blockDim.x) {
-    double f = F[(long)row * c + k];
+    double f = Fc[(long)k * m + row];
     if (i > 0) {
-      f = f + (-b) * U[(long)(i - 1) * c + k];
-      F[(long)row * c + k] = f;
+      f = f + (-b) * Uc[(long)k * m + i - 1];
+      Fc[(long)k * m + row] = f;
     }
     s += f * f;
   }
@@ -190,14 +184,14 @@ __global__ __launch_bounds__(256) void k_ralha_tail_update(int i, int c, double*
 }
 
 // alpha_i = sqrt(reduced ||f_i||^2); u_i = f_i / alpha_i
-__global__ __launch_bounds__(256) void k_ralha_tail_scale(int i, int c, const double* __restrict__ F,
-                                                          double* __restrict__ U, const double* __restrict__ n2,
+__global__ __launch_bounds__(256) void k_ralha_tail_scale(int i, int m, int c, const double* __restrict__ Fc,
+                                                          double* __restrict__ Uc, const double* __restrict__ n2,
                                                           double* __restrict__ alpha) {
   const double a = sqrt(n2[0]);
   if (blockIdx.x == 0 && threadIdx.x == 0) alpha[i] = a;
   const double inv = 1.0 / a;
   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < c; k += gridDim.x * blockDim.x)
-    U[(long)i * c + k] = inv * F[(long)i * c + k];
+    Uc[(long)k * m + i] = inv * Fc[(long)k * m + i];
 }
 
 // Lower to upper bidiagonal with the reference BLAS drotg (client.cpp:947-954), one thread.
@@ -272,47 +266,57 @@ __global__ __launch_bounds__(256) void k_rot_back(int m, int k, double* __restri
 
 extern "C" {
 
-int exc_ralha_msg(int i, int m, int c, const double* d_f, const double* d_u, double* d_msg, exc_stream_t stream) {
-  EXC_REQUIRE(m >= 3 && i >= 0 && i <= m - 3 && c >= 1 && d_f && d_u && d_msg, EXC_ERR_ARG,
-              "exc_ralha_msg: bad arguments");
-  const int total = (m - 1 - i) + (i > 0 ? 4 : 0);
-  hipLaunchKernelGGL(k_ralha_msg, dim3(exc_detail::ceil_div((long)total * 64, 256)), dim3(256), 0,
-                     exc_detail::stream(stream), i, m, c, d_f, d_u, d_msg);
+size_t exc_ralha_workspace(int m, int c) {
+  return (size_t)exc_detail::ceil_div(c, kColsPerWg) * (m + 3) * sizeof(double) + 256;
+}
+
+int exc_ralha_step(int i, int m, int c, double* d_fc, double* d_uc, const double* d_g, double* d_house,
+                   double* d_tau, double* d_alpha, double* d_beta, double* d_msg, void* d_work, size_t work_bytes,
+                   exc_stream_t stream) {
+  EXC_REQUIRE(m >= 3 && i >= -1 && i <= m - 3 && c >= 1 && d_fc && d_uc && (i < 0 || d_g) && d_house && d_tau &&
+                  d_alpha && d_beta && d_msg && d_work,
+              EXC_ERR_ARG, "exc_ralha_step: bad arguments");
+  const unsigned nwg = exc_detail::ceil_div(c, kColsPerWg);
+  EXC_REQUIRE(work_bytes >= (size_t)nwg * (m + 3) * sizeof(double), EXC_ERR_SPACE,
+              "exc_ralha_step: workspace too small");
+  hipStream_t s = exc_detail::stream(stream);
+  double* partial = (double*)d_work;
+  const size_t lds = (size_t)m * sizeof(double);
+  EXC_REQUIRE(lds <= 160 * 1024, EXC_ERR_ARG, "exc_ralha_step: m above 20480 does not fit the reflector in LDS");
+  if (lds > 64 * 1024)
+    EXC_CHECK_HIP(hipFuncSetAttribute((const void*)k_ralha_step, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  hipLaunchKernelGGL(k_ralha_step, dim3(nwg), dim3(256), lds, s, i, m, c, d_fc, d_uc, d_g, d_house, d_tau, d_alpha,
+                     d_beta, partial);
+  EXC_CHECK_HIP(hipGetLastError());
+  if (i + 1 <= m - 3) {
+    hipLaunchKernelGGL(k_ralha_reduce, dim3(exc_detail::ceil_div(m + 3, 256)), dim3(256), 0, s, i + 1, m, (int)nwg,
+                       partial, d_msg);
+    EXC_CHECK_HIP(hipGetLastError());
+  }
+  return EXC_OK;
+}
+
+int exc_ralha_dot(int i, int m, int c, const double* d_fc, const double* d_uc, double* d_out, exc_stream_t stream) {
+  EXC_REQUIRE(i >= 1 && i < m && c >= 1, EXC_ERR_ARG, "exc_ralha_dot: bad arguments");
+  hipLaunchKernelGGL(k_ralha_dot, dim3(1), dim3(256), 0, exc_detail::stream(stream), i, m, c, d_fc, d_uc, d_out);
   EXC_CHECK_HIP(hipGetLastError());
   return EXC_OK;
 }
 
-int exc_ralha_apply(int i, int m, int c, double* d_f, double* d_u, const double* d_g, double* d_house, double* d_tau,
-                    double* d_alpha, double* d_beta, exc_stream_t stream) {
-  EXC_REQUIRE(m >= 3 && i >= 0 && i <= m - 3 && c >= 1 && d_f && d_u && d_g && d_house && d_tau && d_alpha && d_beta,
-              EXC_ERR_ARG, "exc_ralha_apply: bad arguments");
-  hipLaunchKernelGGL(k_ralha_apply, dim3(exc_detail::ceil_div(c, kColsPerWg)), dim3(256), 0,
-                     exc_detail::stream(stream), i, m, c, d_f, d_u, d_g, d_house, d_tau, d_alpha, d_beta);
-  EXC_CHECK_HIP(hipGetLastError());
-  return EXC_OK;
-}
-
-int exc_ralha_dot(int i, int c, const double* d_f, const double* d_u, double* d_out, exc_stream_t stream) {
-  EXC_REQUIRE(i >= 1 && c >= 1, EXC_ERR_ARG, "exc_ralha_dot: bad arguments");
-  hipLaunchKernelGGL(k_ralha_dot, dim3(1), dim3(256), 0, exc_detail::stream(stream), i, c, d_f, d_u, d_out);
-  EXC_CHECK_HIP(hipGetLastError());
-  return EXC_OK;
-}
-
-int exc_ralha_tail_update(int i, int c, double* d_f, const double* d_u, const double* d_breduced, double* d_beta,
-                          double* d_out, exc_stream_t stream) {
-  EXC_REQUIRE(i >= -1 && c >= 1, EXC_ERR_ARG, "exc_ralha_tail_update: bad arguments");
-  hipLaunchKernelGGL(k_ralha_tail_update, dim3(1), dim3(256), 0, exc_detail::stream(stream), i, c, d_f, d_u,
+int exc_ralha_tail_update(int i, int m, int c, double* d_fc, const double* d_uc, const double* d_breduced,
+                          double* d_beta, double* d_out, exc_stream_t stream) {
+  EXC_REQUIRE(i >= -1 && i < m && c >= 1, EXC_ERR_ARG, "exc_ralha_tail_update: bad arguments");
+  hipLaunchKernelGGL(k_ralha_tail_update, dim3(1), dim3(256), 0, exc_detail::stream(stream), i, m, c, d_fc, d_uc,
                      d_breduced, d_beta, d_out);
   EXC_CHECK_HIP(hipGetLastError());
   return EXC_OK;
 }
 
-int exc_ralha_tail_scale(int i, int c, const double* d_f, double* d_u, const double* d_n2, double* d_alpha,
+int exc_ralha_tail_scale(int i, int m, int c, const double* d_fc, double* d_uc, const double* d_n2, double* d_alpha,
                          exc_stream_t stream) {
-  EXC_REQUIRE(i >= 0 && c >= 1, EXC_ERR_ARG, "exc_ralha_tail_scale: bad arguments");
+  EXC_REQUIRE(i >= 0 && i < m && c >= 1, EXC_ERR_ARG, "exc_ralha_tail_scale: bad arguments");
   hipLaunchKernelGGL(k_ralha_tail_scale, dim3(exc_detail::ceil_div(c, 256)), dim3(256), 0,
-                     exc_detail::stream(stream), i, c, d_f, d_u, d_n2, d_alpha);
+                     exc_detail::stream(stream), i, m, c, d_fc, d_uc, d_n2, d_alpha);
   EXC_CHECK_HIP(hipGetLastError());
   return EXC_OK;
 }

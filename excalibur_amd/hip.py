@@ -67,11 +67,11 @@ SIGNATURES = {
     "exc_orgqr_t": (_I, [_I, _L, _dp, _L, _dp, _dp, _L, _dp, _S, _vp]),
     "exc_ormqr_house_workspace": (_S, [_I, _L]),
     "exc_ormqr_house_ln": (_I, [_I, _L, _dp, _dp, _dp, _L, _dp, _S, _vp]),
-    "exc_ralha_msg": (_I, [_I, _I, _I, _dp, _dp, _dp, _vp]),
-    "exc_ralha_apply": (_I, [_I, _I, _I, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _vp]),
-    "exc_ralha_dot": (_I, [_I, _I, _dp, _dp, _dp, _vp]),
-    "exc_ralha_tail_update": (_I, [_I, _I, _dp, _dp, _dp, _dp, _dp, _vp]),
-    "exc_ralha_tail_scale": (_I, [_I, _I, _dp, _dp, _dp, _dp, _vp]),
+    "exc_ralha_workspace": (_S, [_I, _I]),
+    "exc_ralha_step": (_I, [_I, _I, _I, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _vp, _S, _vp]),
+    "exc_ralha_dot": (_I, [_I, _I, _I, _dp, _dp, _dp, _vp]),
+    "exc_ralha_tail_update": (_I, [_I, _I, _I, _dp, _dp, _dp, _dp, _dp, _vp]),
+    "exc_ralha_tail_scale": (_I, [_I, _I, _I, _dp, _dp, _dp, _dp, _vp]),
     "exc_lower_to_upper": (_I, [_I, _dp, _dp, _dp, _vp]),
     "exc_sign_normalize": (_I, [_I, _I, _dp, _dp, _dp, _vp]),
     "exc_rot_back": (_I, [_I, _I, _dp, _dp, _vp]),
@@ -364,12 +364,13 @@ def ormqr_house_ln(house, tau, C):
 
 def ralha_peers(F_list):
     """The overlapped Ralha loop over P simulated peers in this process: GPU kernels per peer,
-    the all-reduce done here in each peer's own order (own buffer, then the others by id)."""
+    the all-reduce done here in each peer's own order (own buffer, then the others by id).
+    F_list[p] is peer p's m x c_p share of the rows f_j; U is returned in the same orientation."""
     P = len(F_list)
     m = F_list[0].shape[0]
     cs = [f.shape[1] for f in F_list]
-    dF = [DeviceArray.from_host(np.ascontiguousarray(f, np.float64)) for f in F_list]
-    dU = [DeviceArray((m, c)).zero() for c in cs]
+    dF = [DeviceArray.from_host(np.ascontiguousarray(np.asarray(f, np.float64).T)) for f in F_list]
+    dU = [DeviceArray((c, m)).zero() for c in cs]
     da = [DeviceArray((m,)).zero() for _ in range(P)]
     db = [DeviceArray((m,)).zero() for _ in range(P)]
     dh = [DeviceArray((m, m)).zero() for _ in range(P)]
@@ -377,6 +378,7 @@ def ralha_peers(F_list):
     dmsg = [DeviceArray((m + 3,)).zero() for _ in range(P)]
     dg = [DeviceArray((m + 3,)).zero() for _ in range(P)]
     ds = [DeviceArray((4,)).zero() for _ in range(P)]
+    ws = [workspace(_lib.exc_ralha_workspace(m, c)) for c in cs]
 
     def allreduce(srcs, n):
         vals = [a.to_host()[:n].copy() for a in srcs]
@@ -394,32 +396,34 @@ def ralha_peers(F_list):
         buf[off:off + vec.size] = vec
         call("exc_memcpy_h2d", dst.ptr, buf.ctypes.data, buf.nbytes, None)
 
+    def step(i, p):
+        call("exc_ralha_step", i, m, cs[p], dF[p].ptr, dU[p].ptr, dg[p].ptr, dh[p].ptr, dt[p].ptr, da[p].ptr,
+             db[p].ptr, dmsg[p].ptr, ws[p].ptr, ws[p].nbytes, None)
+
     for p in range(P):
-        call("exc_ralha_tail_update", -1, cs[p], dF[p].ptr, dU[p].ptr, None, db[p].ptr, ds[p].ptr, None)
+        call("exc_ralha_tail_update", -1, m, cs[p], dF[p].ptr, dU[p].ptr, None, db[p].ptr, ds[p].ptr, None)
     sync()
     red = allreduce(ds, 1)
     for p in range(P):
         put(ds[p], red[p], 1)
-        call("exc_ralha_tail_scale", 0, cs[p], dF[p].ptr, dU[p].ptr, ds[p].ptr + 8, da[p].ptr, None)
+        call("exc_ralha_tail_scale", 0, m, cs[p], dF[p].ptr, dU[p].ptr, ds[p].ptr + 8, da[p].ptr, None)
+        step(-1, p)
     for i in range(m - 2):
-        for p in range(P):
-            call("exc_ralha_msg", i, m, cs[p], dF[p].ptr, dU[p].ptr, dmsg[p].ptr, None)
         sync()
         red = allreduce(dmsg, m + 3 - i)
         for p in range(P):
             put(dg[p], red[p])
-            call("exc_ralha_apply", i, m, cs[p], dF[p].ptr, dU[p].ptr, dg[p].ptr, dh[p].ptr, dt[p].ptr, da[p].ptr,
-                 db[p].ptr, None)
-        sync()
+            step(i, p)
+    sync()
     for i in range(m - 2, m):
         for p in range(P):
-            call("exc_ralha_dot", i, cs[p], dF[p].ptr, dU[p].ptr, ds[p].ptr, None)
+            call("exc_ralha_dot", i, m, cs[p], dF[p].ptr, dU[p].ptr, ds[p].ptr, None)
         sync()
         red = allreduce(ds, 1)
         for p in range(P):
             put(ds[p], red[p], 1)
-            call("exc_ralha_tail_update", i, cs[p], dF[p].ptr, dU[p].ptr, ds[p].ptr + 8, db[p].ptr, ds[p].ptr + 16,
-                 None)
+            call("exc_ralha_tail_update", i, m, cs[p], dF[p].ptr, dU[p].ptr, ds[p].ptr + 8, db[p].ptr,
+                 ds[p].ptr + 16, None)
         sync()
         red2 = [a.to_host()[2:3].copy() for a in ds]
         for p in range(P):
@@ -428,10 +432,10 @@ def ralha_peers(F_list):
                 if q != p:
                     acc = acc + red2[q]
             put(ds[p], acc, 3)
-            call("exc_ralha_tail_scale", i, cs[p], dF[p].ptr, dU[p].ptr, ds[p].ptr + 24, da[p].ptr, None)
+            call("exc_ralha_tail_scale", i, m, cs[p], dF[p].ptr, dU[p].ptr, ds[p].ptr + 24, da[p].ptr, None)
         sync()
     return [dict(alpha=da[p].to_host(), beta=db[p].to_host()[: m - 1], house=dh[p].to_host(), tau=dt[p].to_host(),
-                 U=dU[p].to_host()) for p in range(P)]
+                 U=np.ascontiguousarray(dU[p].to_host().T)) for p in range(P)]
 
 
 def lower_to_upper(alpha, beta):

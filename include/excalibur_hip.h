@@ -175,22 +175,29 @@ int exc_ormqr_house_ln(int m, long n, const double* d_house, const double* d_tau
 
 /* ---------------------------------------------------------------- Ralha bidiagonalization */
 
-/* Step i (0 <= i <= m-3) of the overlapped one-sided bidiagonalization (client.cpp:840-892) on
- * this peer's F (m x c) and U (m x c). exc_ralha_msg writes the local message d_msg (m+3-i
- * values: f_{i+1+r}.f_i, then f_i.u_{i-1}, u_{i-1}.u_{i-1}, f_i.f_i, u_{i-1}.f_i when i > 0);
- * after the host all-reduces it over the peers, exc_ralha_apply takes the reduced d_g: dlarfg
- * (LAPACK convention) into column i+1 of d_house (m x m) and d_tau[i+1], the rank-1 update of
- * F[i+1:], beta_{i-1}, alpha_i and u_i. */
-int exc_ralha_msg(int i, int m, int c, const double* d_f, const double* d_u, double* d_msg, exc_stream_t stream);
-int exc_ralha_apply(int i, int m, int c, double* d_f, double* d_u, const double* d_g, double* d_house, double* d_tau,
-                    double* d_alpha, double* d_beta, exc_stream_t stream);
+/* One-sided (Ralha) bidiagonalization of this peer's share F (m x c) of the rows f_j
+ * (client.cpp:800-910). F and U are stored column-major here: d_fc[col * m + j] = F[j][col]
+ * (a row-major c x m array), likewise d_uc.
+ *
+ * exc_ralha_step(i) does the column work of step i (0 <= i <= m-3; client.cpp:858-891): from
+ * the all-reduced message d_g of step i, dlarfg (LAPACK convention) into column i+1 of d_house
+ * (m x m row-major) and d_tau[i+1], F[i+1:] -= tau h (h^T F[i+1:]), f_i -= beta_{i-1} u_{i-1},
+ * alpha_i and u_i. It then writes this peer's message for step j = i+1 when j <= m-3
+ * (client.cpp:842-855): m+3-j values, f_{j+1+r}.f_j for r < m-1-j, then f_j.u_{j-1},
+ * u_{j-1}.u_{j-1}, f_j.f_j, u_{j-1}.f_j when j > 0, zeros after. i = -1 only writes the message
+ * of step 0. d_msg may be pinned host memory (exc_host_alloc); it is complete once the stream
+ * has reached that point. The workspace is exc_ralha_workspace(m, c) bytes. */
+size_t exc_ralha_workspace(int m, int c);
+int exc_ralha_step(int i, int m, int c, double* d_fc, double* d_uc, const double* d_g, double* d_house,
+                   double* d_tau, double* d_alpha, double* d_beta, double* d_msg, void* d_work, size_t work_bytes,
+                   exc_stream_t stream);
 /* The last two columns and the first (client.cpp:830-833, 897-905): d_out = f_i.u_{i-1};
  * tail_update: f_i -= beta u_{i-1} (beta = reduced value, stored in d_beta[i-1]), d_out =
  * ||f_i||^2 (i = -1: ||f_0||^2 only); tail_scale: alpha_i = sqrt(reduced), u_i = f_i / alpha_i. */
-int exc_ralha_dot(int i, int c, const double* d_f, const double* d_u, double* d_out, exc_stream_t stream);
-int exc_ralha_tail_update(int i, int c, double* d_f, const double* d_u, const double* d_breduced, double* d_beta,
-                          double* d_out, exc_stream_t stream);
-int exc_ralha_tail_scale(int i, int c, const double* d_f, double* d_u, const double* d_n2, double* d_alpha,
+int exc_ralha_dot(int i, int m, int c, const double* d_fc, const double* d_uc, double* d_out, exc_stream_t stream);
+int exc_ralha_tail_update(int i, int m, int c, double* d_fc, const double* d_uc, const double* d_breduced,
+                          double* d_beta, double* d_out, exc_stream_t stream);
+int exc_ralha_tail_scale(int i, int m, int c, const double* d_fc, double* d_uc, const double* d_n2, double* d_alpha,
                          exc_stream_t stream);
 /* Lower to upper bidiagonal by drotg (client.cpp:947-954); d_cs = (c_i, s_i) pairs, 2(m-1). */
 int exc_lower_to_upper(int m, double* d_alpha, double* d_beta, double* d_cs, exc_stream_t stream);

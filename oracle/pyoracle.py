@@ -203,6 +203,25 @@ def dpqds(l, d, tau):
     return up[: n - 1], rp[:n], p[:n]
 
 
+def inverse_iteration(n, k, l, u, v, ldv):
+    """bbtsvd.cpp:65-71 on a copy of v (at least (n-1)*ldv+1 entries); returns the copy."""
+    v = f64(v).copy()
+    l, u = f64(l), f64(u)
+    lib().orc_inverse_iteration(n, k, _d(l if l.size else np.zeros(1)), _d(u if u.size else np.zeros(1)), _d(v),
+                                ldv)
+    return v
+
+
+def one_pair(n, ldu, omega, l, d, a, b, v, u):
+    """bbtsvd.cpp:93-135 on copies of v (n) and u (stride ldu); returns (v, u, twist_v, twist_u)."""
+    v, u = f64(v).copy(), f64(u).copy()
+    l, d, a, b = f64(l), f64(d), f64(a), f64(b)
+    tv, tu = ctypes.c_int(0), ctypes.c_int(0)
+    lib().orc_one_pair(n, ldu, omega, _d(l if l.size else np.zeros(1)), _d(d), _d(a), _d(b if b.size else np.zeros(1)),
+                       _d(v), _d(u), ctypes.byref(tv), ctypes.byref(tu))
+    return v, u, tv.value, tu.value
+
+
 def two_side(a, b, sigmas):
     a, b, sigmas = f64(a), f64(b), f64(sigmas)
     n, k = a.size, sigmas.size

@@ -35,6 +35,7 @@ def test_library_exports_declared_symbols():
 
 def test_binding_signatures_cover_header():
     from excalibur_amd import hip
+    import excalibur_amd.bbtsvd  # noqa: F401  (registers the host-pointer entry points)
     names = declared_symbols()
     bound = set(hip.SIGNATURES) | set(hip.EXTRA_SIGNATURES)
     missing = [n for n in names if n not in bound]
@@ -51,3 +52,52 @@ def test_givens_plan_matches_oracle(oracle, K, mask):
         op, oc, os_ = oracle.givens_plan(K, passes, mask, seed)
         assert np.array_equal(pairs.astype(np.int64), op)
         assert np.array_equal(cs[..., 0], oc) and np.array_equal(cs[..., 1], os_)
+
+
+@pytest.mark.parametrize("k", [4, 10, 9973, 10001, 12345, 99_999_999, 20_000_000, 17 * 9001])
+def test_find_best_convert_matches_oracle(oracle, k):
+    """Host-only integer search of the K > 10000 mask (bbtsvd.cpp:166-188)."""
+    from excalibur_amd import bbtsvd
+    assert bbtsvd.find_best_convert(k) == oracle.find_best_convert(k)
+    m, n, r = bbtsvd.find_best_convert(k)
+    assert m * n + r == k and (k < 10000 or (1 < n < m < 10000 and m / n < 2))
+
+
+def test_cpp_dropin_header_compiles_and_links(tmp_path):
+    """A C++ caller written against the reference's bbtsvd.hpp names compiles with
+    include/bbtsvd.hpp and links against the library (not run: no GPU here)."""
+    import shutil
+    import subprocess
+    from excalibur_amd import LIB_PATH
+    if shutil.which("g++") is None:
+        pytest.skip("no g++")
+    src = tmp_path / "caller.cpp"
+    src.write_text(r"""
+#include "bbtsvd.hpp"
+#include <vector>
+int main() {
+  std::vector<double> a = {3, 2, 1}, b = {0.5, 0.25}, l(2), d(3), s(3), u(9), v(9), x(12);
+  int cnt = 0;
+  int nc = negative_count(3, a.data(), b.data(), 1.5);
+  bisection_singular_values(3, a.data(), b.data(), 0.0, 20.0, 1e-20, 3, s.data(), &cnt);
+  ldl_decomposition(3, a.data(), b.data(), l.data(), d.data());
+  std::vector<double> lp(2), dp(3), ss(3), up(2), rp(3), p(3);
+  dstqds(3, l.data(), d.data(), 1.0, lp.data(), dp.data(), ss.data());
+  dpqds(3, l.data(), d.data(), 1.0, up.data(), rp.data(), p.data());
+  inverse_iteration(3, 1, lp.data(), up.data(), v.data(), 1);
+  one_pair_singular_vector_twisted_factorization(3, 3, s[0], l.data(), d.data(), a.data(), b.data(), v.data(), u.data());
+  two_side_singular_vectors_twisted_factorization(3, 3, s.data(), a.data(), b.data(), u.data(), v.data());
+  bidiagonal_bisection_twisted_svd(3, a.data(), b.data(), u.data(), v.data(), s.data(), 3);
+  givens_mask_cache_optimized_parallel_with_simd(3, 4, 32, x.data(), true, true, 7u);
+  block_shift_mask(3, 4, 32, x.data(), false, true, 7u);
+  SquareMaskConvert smc = {12345, 0, 0, 0};
+  find_best_convert(&smc);
+  return nc + cnt + (int)smc.m > 0 ? 0 : 1;
+}
+""")
+    inc = os.path.join(ROOT, "include")
+    libdir = os.path.dirname(LIB_PATH)
+    out = tmp_path / "caller"
+    r = subprocess.run(["g++", "-std=c++17", "-I", inc, str(src), "-L", libdir, "-lexcalibur_amd",
+                        f"-Wl,-rpath,{libdir}", "-o", str(out)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-2000:]
