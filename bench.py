@@ -87,6 +87,51 @@ def accuracy(peers, xs, sigma_true, A):
     return out
 
 
+def pmc_traffic(name="k_gemm_final_v_c2_pmc.md"):
+    """HBM bytes per launch of the final V product from the committed rocprofv3 PMC summary
+    (FETCH_SIZE + WRITE_SIZE, scaled by a plain copy of known size); None when absent."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
+def isolated_gemms(n, rows, reps=5):
+    """The two largest products of the run, each launched alone on the GPU from this process and
+    timed with HIP events through the library's profiler: the final V product (n x rows x n) and
+    the local-QR Gram matrix (n x n x rows, lower half, split-K)."""
+    import ctypes
+    from excalibur_amd import hip
+    rng = np.random.default_rng(5)
+    A = hip.DeviceArray.from_host(rng.standard_normal((n, n)))
+    B = hip.DeviceArray.from_host(rng.standard_normal((n, rows)))
+    C = hip.DeviceArray((n, rows)).zero()
+    G = hip.DeviceArray((n, n)).zero()
+    ws = hip.workspace(hip.lib.exc_gemm_workspace(n, n, rows))
+    buf = ctypes.create_string_buffer(1 << 16)
+    out = {}
+    for name, launch, flops in (
+            ("final_v", lambda: hip.call("exc_dgemm", 0, 0, n, rows, n, 1.0, A.ptr, n, B.ptr, rows, 0.0, C.ptr, rows,
+                                         None, 0, None), 2.0 * n * rows * n),
+            ("gram", lambda: hip.call("exc_dgemm", 0, 1, n, n, rows, 1.0, B.ptr, rows, B.ptr, rows, 0.0, G.ptr, n,
+                                      ws.ptr, ws.nbytes, None), 2.0 * n * n * rows)):
+        launch()
+        hip.sync()
+        hip.call("exc_prof_enable", 1)
+        for _ in range(reps):
+            launch()
+        hip.sync()
+        hip.call("exc_prof_enable", 0)
+        hip.call("exc_prof_report", buf, len(buf))
+        rec = json.loads(buf.value.decode())["gemm_f64"]
+        ms = rec["ms"] / rec["launches"]
+        out[name] = {"avg_launch_us": round(1000.0 * ms, 2), "achieved": round(flops / (ms * 1e-3) / 1e12, 3)}
+    return out
+
+
 def cpu_baseline(sample="cpu_sample"):
     """The oracle (CPU FP64 restatement, one core) on a bounded sample of the workload, scaled to the
     full configuration by the ratio of the pipeline's dominant work (m^2 r terms)."""
@@ -122,12 +167,29 @@ def main():
     ap.add_argument("--workdir", default=None)
     ap.add_argument("--prepare", default=None,
                     help="only write the peers' Client{i}.mat files into this directory and exit")
+    ap.add_argument("--isolated-only", action="store_true",
+                    help="only time the largest products of the configuration alone on the GPU (for PMC runs)")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only copy MIB mebibytes device to device once (HBM traffic calibration for PMC runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: every GPU box of this build has one MI355X and the peers share it; "
               f"--gpus {args.gpus} is not supported", file=sys.stderr)
         sys.exit(2)
     ensure_built()
+    if args.calibrate_copy:
+        from excalibur_amd import hip
+        nbytes = args.calibrate_copy << 20
+        src = hip.DeviceArray((nbytes // 8,)).zero()
+        dst = hip.DeviceArray((nbytes // 8,))
+        hip.call("exc_memcpy_d2d", dst.ptr, src.ptr, nbytes, None)
+        hip.sync()
+        print(json.dumps({"copy_bytes": nbytes}))
+        return
+    if args.isolated_only:
+        P, rows, n, _ = CONFIGS[args.config]
+        print(json.dumps(isolated_gemms(n, rows)))
+        return
     from excalibur_amd.launch import run_peers, write_peer_files
 
     if args.prepare:
@@ -170,16 +232,23 @@ def main():
                 a = agg.setdefault(name, {"launches": 0, "ms": 0.0, "flops": 0.0, "bytes": 0.0})
                 for f in ("launches", "ms", "flops", "bytes"):
                     a[f] += v[f]
-    gname = "gemm_f32" if args.form == "mixed" and "gemm_f32" in agg else "gemm_f64"
-    g = agg.get(gname, {"launches": 0, "ms": 0.0, "flops": 0.0, "bytes": 0.0})
-    peak = PEAK_FP32_TFLOPS if gname == "gemm_f32" else PEAK_FP64_TFLOPS
-    achieved = g["flops"] / (g["ms"] * 1e-3) / 1e12 if g["ms"] > 0 else 0.0
-    roofline = {"bound": "mfma", "kernel": "k_gemm<" + ("float" if gname == "gemm_f32" else "double") + ">",
-                "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
-                "frac": round(achieved / peak, 4), "traffic": None,
+    def rate(rec):
+        return rec["flops"] / (rec["ms"] * 1e-3) / 1e12 if rec["ms"] > 0 else 0.0
+    g = agg.get("gemm_f64", {"launches": 0, "ms": 0.0, "flops": 0.0, "bytes": 0.0})
+    fv = agg.get("gemm_f64_final_v", {"launches": 0, "ms": 0.0, "flops": 0.0, "bytes": 0.0})
+    achieved = rate(g)
+    roofline = {"bound": "mfma", "kernel": "k_gemm<double> (all FP64 MFMA GEMM launches of both peers)",
+                "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": None,
+                "traffic_final_v": pmc_traffic(),
                 "launches_per_step": g["launches"] / max(1, len(timed)),
-                "avg_launch_us": 1000.0 * g["ms"] / max(1, g["launches"]),
-                "flops_per_step": g["flops"] / max(1, len(timed))}
+                "avg_launch_us": round(1000.0 * g["ms"] / max(1, g["launches"]), 2),
+                "flops_per_step": g["flops"] / max(1, len(timed)),
+                "final_v": {"launches_per_step": fv["launches"] / max(1, len(timed)),
+                            "avg_launch_us": round(1000.0 * fv["ms"] / max(1, fv["launches"]), 2),
+                            "achieved": round(rate(fv), 3), "frac": round(rate(fv) / PEAK_FP64_TFLOPS, 4)},
+                "note": "in-pipeline: both peer processes share the GPU, so a launch's duration includes "
+                        "the other peer's concurrent work; 'isolated' is the same kernel alone"}
     kernels = {k: {"launches": v["launches"] / len(timed), "ms": v["ms"] / len(timed),
                    "achieved": (v["flops"] / (v["ms"] * 1e-3) / 1e12) if v["ms"] > 0 else 0.0,
                    "gbs": (v["bytes"] / (v["ms"] * 1e-3) / 1e9) if v["ms"] > 0 else 0.0}
@@ -189,12 +258,16 @@ def main():
         t0 = time.time()
         acc = accuracy(res["peers"], xs, sigma_true, A)
         log(f"accuracy checked in {time.time() - t0:.1f} s: {acc}")
+    P, rows, n, _ = CONFIGS[args.config]
+    try:
+        roofline["isolated"] = isolated_gemms(n, rows)
+    except Exception as e:  # the measurement is extra information; the run itself already finished
+        log(f"isolated GEMM timing failed: {e}")
     cpu = None
     if not args.no_cpu:
         t0 = time.time()
         cpu = cpu_baseline()
         log(f"cpu baseline in {time.time() - t0:.1f} s")
-    P, rows, n, _ = CONFIGS[args.config]
     line = {
         "metric": "SVD wall-time (s) + % roofline, mixed vs plain FP64, at <=1e-12 rel sigma err",
         "value": round(ms_per_step / 1000.0, 6),

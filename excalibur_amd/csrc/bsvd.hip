@@ -27,6 +27,8 @@ __device__ __forceinline__ int sturm_count(int n, const double* __restrict__ a, 
   double t = -mu2;
   double d = a[0] * a[0] + t;
   int count = d < 0 ? 1 : 0;
+  // unrolled so the wave-uniform loads of a and b are issued ahead of the dependent chain
+#pragma unroll 8
   for (int i = 1; i < n; i++) {
     const double bb = b[i - 1] * b[i - 1];
     const double aa = a[i] * a[i];
@@ -252,6 +254,7 @@ __global__ __launch_bounds__(256) void k_twisted(int n, int k, const double* __r
   // dstqds (bbtsvd.cpp:73-81)
   double sc = -w2;
   S[at(0)] = sc;
+  #pragma unroll 4
   for (int i = 0; i < n - 1; i++) {
     const double dp = d[i] + sc;
     const double lp = d[i] * l[i] / dp;
@@ -263,6 +266,7 @@ __global__ __launch_bounds__(256) void k_twisted(int n, int k, const double* __r
   // dpqds (bbtsvd.cpp:83-91)
   double pc = d[n - 1] - w2;
   Pp[at(n - 1)] = pc;
+  #pragma unroll 4
   for (int i = n - 2; i >= 0; i--) {
     const double rp = d[i] * (l[i] * l[i]) + pc;
     RP[at(i + 1)] = rp;
@@ -273,6 +277,7 @@ __global__ __launch_bounds__(256) void k_twisted(int n, int k, const double* __r
   // twist: first index of min |s + p + omega^2|
   int k0 = 0;
   double best = 0.0;
+  #pragma unroll 4
   for (int i = 0; i < n; i++) {
     const double g = S[at(i)] + Pp[at(i)] + w2;
     const double ag = fabs(g);
@@ -281,23 +286,27 @@ __global__ __launch_bounds__(256) void k_twisted(int n, int k, const double* __r
   // inverse iteration for v (bbtsvd.cpp:65-71), l+ = d l / d+, u+ = l d / r+
   vs[at(k0)] = 1.0;
   double z = 1.0;
+  #pragma unroll 4
   for (int i = k0 - 1; i >= 0; i--) {
     const double lp = d[i] * l[i] / DP[at(i)];
     z = -lp * z;
     vs[at(i)] = z;
   }
   z = 1.0;
+  #pragma unroll 4
   for (int i = k0 + 1; i < n; i++) {
     const double up = l[i - 1] * d[i - 1] / RP[at(i)];
     z = -up * z;
     vs[at(i)] = z;
   }
   double ss = 0.0;
+  #pragma unroll 4
   for (int i = 0; i < n; i++) {
     const double v = vs[at(i)];
     ss = ss + v * v;
   }
   double inv = 1.0 / sqrt(ss);
+  #pragma unroll 4
   for (int i = 0; i < n; i++) vs[at(i)] = inv * vs[at(i)];
 
   // coupling (bbtsvd.cpp:113-124): s[n] = -omega^2, p[n] = 1, r+[n] = 1
@@ -306,6 +315,7 @@ __global__ __launch_bounds__(256) void k_twisted(int n, int k, const double* __r
   auto Rv = [&](int i) -> double { return i == n ? 1.0 : RP[at(i)]; };
   int k1 = 0;
   best = 0.0;
+  #pragma unroll 4
   for (int i = 0; i < n; i++) {
     const double si = Sv(i), pi = Pv(i);
     const double g = si + pi + w2;
@@ -317,6 +327,7 @@ __global__ __launch_bounds__(256) void k_twisted(int n, int k, const double* __r
   // d+'_i = s_{i+1} / s_i d+_i, r+'_i = p_i / p_{i+1} r+_{i+1}
   u[at(k1)] = 1.0;
   z = 1.0;
+  #pragma unroll 4
   for (int i = k1 - 1; i >= 0; i--) {
     const double dpn = Sv(i + 1) / Sv(i) * DP[at(i)];
     const double lpn = a[i + 1] / dpn * b[i];
@@ -324,6 +335,7 @@ __global__ __launch_bounds__(256) void k_twisted(int n, int k, const double* __r
     u[at(i)] = z;
   }
   z = 1.0;
+  #pragma unroll 4
   for (int i = k1 + 1; i < n; i++) {
     const int q = i - 1;
     const double rpn = Pv(q + 1) / Pv(q + 2) * Rv(q + 2);
@@ -332,11 +344,13 @@ __global__ __launch_bounds__(256) void k_twisted(int n, int k, const double* __r
     u[at(i)] = z;
   }
   ss = 0.0;
+  #pragma unroll 4
   for (int i = 0; i < n; i++) {
     const double v = u[at(i)];
     ss = ss + v * v;
   }
   inv = 1.0 / sqrt(ss);
+  #pragma unroll 4
   for (int i = 0; i < n; i++) u[at(i)] = inv * u[at(i)];
   if (twists) {
     twists[2 * j] = k0;
@@ -354,6 +368,7 @@ __global__ __launch_bounds__(256) void k_vsign(int n, int k, const double* __res
   const size_t K = (size_t)k;
   int p = 0;
   double best = -1.0;
+#pragma unroll 8
   for (int i = 0; i < n; i++) {
     const double av = fabs(vs[(size_t)i * K + j]);
     if (av > best) { best = av; p = i; }

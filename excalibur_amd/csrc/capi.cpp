@@ -36,10 +36,11 @@ namespace exc_prof {
 
 namespace {
 struct Rec {
-  std::string name;
+  std::string name, label;
   hipEvent_t a, b;
   double flops, bytes;
 };
+thread_local std::string t_label;
 std::mutex g_mu;
 bool g_on = false;
 std::vector<Rec> g_recs;
@@ -64,10 +65,14 @@ hipEvent_t mark(hipStream_t s) {
 }
 
 void book(const char* name, hipEvent_t a, hipEvent_t b, double flops, double bytes) {
+  std::string label;
+  label.swap(t_label);
   if (!a || !b) return;
   std::lock_guard<std::mutex> g(g_mu);
-  g_recs.push_back(Rec{name, a, b, flops, bytes});
+  g_recs.push_back(Rec{name, label, a, b, flops, bytes});
 }
+
+void label_next(const char* label) { t_label = label ? label : ""; }
 
 }  // namespace exc_prof
 
@@ -175,15 +180,18 @@ int exc_prof_report(char* buf, size_t cap) {
     hipEventSynchronize(r.b);
     float ms = 0.f;
     hipEventElapsedTime(&ms, r.a, r.b);
-    auto it = std::find_if(agg.begin(), agg.end(), [&](auto& p) { return p.first == r.name; });
-    if (it == agg.end()) {
-      agg.push_back({r.name, Agg()});
-      it = agg.end() - 1;
+    for (const std::string* nm : {&r.name, &r.label}) {
+      if (nm->empty()) continue;
+      auto it = std::find_if(agg.begin(), agg.end(), [&](auto& p) { return p.first == *nm; });
+      if (it == agg.end()) {
+        agg.push_back({*nm, Agg()});
+        it = agg.end() - 1;
+      }
+      it->second.n++;
+      it->second.ms += ms;
+      it->second.flops += r.flops;
+      it->second.bytes += r.bytes;
     }
-    it->second.n++;
-    it->second.ms += ms;
-    it->second.flops += r.flops;
-    it->second.bytes += r.bytes;
     exc_prof::g_pool.push_back(r.a);
     exc_prof::g_pool.push_back(r.b);
   }

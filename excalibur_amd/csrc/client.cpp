@@ -17,6 +17,7 @@
 #include <string>
 #include <vector>
 
+#include "exc_common.hpp"
 #include "excalibur_hip.h"
 #include "net.hpp"
 
@@ -546,6 +547,7 @@ void DecentralizedClient::run() {
     givens_mask(D.tmp.p, k, (long)rl, 0, 0, private_seed, D.plan, s);
     if (lqr) {
       D.FinalVT.alloc((size_t)k * local_n, s);
+      exc_prof::label_next("gemm_f64_final_v");  // the largest single product of the run
       ck(exc_dgemm(0, 0, k, (long)local_n, (long)rl, 1.0, D.tmp.p, (long)rl, D.X.p, (long)local_n, 0.0, D.FinalVT.p,
                    (long)local_n, nullptr, 0, s),
          "final V");

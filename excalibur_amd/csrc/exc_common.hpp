@@ -55,6 +55,9 @@ namespace exc_prof {
 bool enabled();
 // Returns an event recorded on `s` now (nullptr when profiling is off).
 hipEvent_t mark(hipStream_t s);
-// Books the interval between two marks under `name` with its algorithmic flops and bytes.
+// Books the interval between two marks under `name` with its algorithmic flops and bytes; the
+// same interval is also booked under the label set by label_next() on this thread, if any.
 void book(const char* name, hipEvent_t start, hipEvent_t end, double flops, double bytes);
+// Names the next booked launch of this thread a second time (e.g. one particular product).
+void label_next(const char* label);
 }  // namespace exc_prof

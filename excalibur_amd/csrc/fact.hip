@@ -1,8 +1,10 @@
-// Blocked dense factorizations on gfx950 (FP64): Cholesky, triangular solves and the
-// sign-choosing LU used to recover LAPACK's Householder convention from an orthonormal factor.
-// Diagonal blocks (up to 128 x 128) are factored or solved inside one workgroup from LDS; the
-// off-diagonal work is MFMA GEMM (gemm.hip). Every triangular solve is a substitution (no
-// explicit inverse), so each step is backward stable.
+// Dense factorizations on gfx950 (FP64): Cholesky, triangular solves and the sign-choosing LU
+// used to recover LAPACK's Householder convention from an orthonormal factor. All four recurse on
+// halves (split at a multiple of the 64-wide diagonal block), so the off-diagonal work is a few
+// large MFMA GEMMs (gemm.hip) rather than many 64-deep ones; the 64 x 64 diagonal blocks are
+// factored in one workgroup from LDS or solved one lane per right-hand side from registers.
+// Every triangular solve is a substitution (no explicit inverse), so each step is backward
+// stable.
 #include "exc_common.hpp"
 #include "fact.hpp"
 #include "gemm.hpp"
@@ -11,141 +13,201 @@ namespace {
 
 using fact::NB;
 
-// Cholesky of one diagonal block (kb <= NB) at A[k0][k0], lower triangle in, L out.
-__global__ __launch_bounds__(256) void k_potf2(int kb, double* __restrict__ A, long lda, int k0,
-                                              int* __restrict__ info) {
-  __shared__ double S[NB][NB + 1];
-  const int tid = threadIdx.x;
-  double* Ab = A + (long)k0 * lda + k0;
-  for (int e = tid; e < kb * kb; e += 256) {
-    const int i = e / kb, j = e % kb;
-    S[i][j] = j <= i ? Ab[(long)i * lda + j] : 0.0;
-  }
-  __syncthreads();
-  for (int j = 0; j < kb; j++) {
-    if (tid == 0) {
-      double d = S[j][j];
-      if (!(d > 0.0)) {
-        if (info && *info == 0) *info = k0 + j + 1;
-        d = 1.0;
-      }
-      S[j][j] = sqrt(d);
-    }
-    __syncthreads();
-    const double piv = S[j][j];
-    for (int i = j + 1 + tid; i < kb; i += 256) S[i][j] = S[i][j] / piv;
-    __syncthreads();
-    const int m = kb - j - 1;
-    for (int e = tid; e < m * m; e += 256) {
-      const int ii = e / m, kk = e % m;
-      if (kk <= ii) S[j + 1 + ii][j + 1 + kk] -= S[j + 1 + ii][j] * S[j + 1 + kk][j];
-    }
-    __syncthreads();
-  }
-  for (int e = tid; e < kb * kb; e += 256) {
-    const int i = e / kb, j = e % kb;
-    Ab[(long)i * lda + j] = j <= i ? S[i][j] : 0.0;
-  }
-}
+// The 64 x 64 diagonal-block kernels are branch-free: a block smaller than 64 is padded with the
+// identity (unit diagonal, zero off-diagonal), which leaves the padded unknowns untouched, so
+// every loop has a constant trip count and the compiler can keep many LDS reads in flight. The
+// updates are right-looking: once unknown j is final, the later ones take one fused
+// multiply-add each with the contiguous row j of the (transposed as needed) triangle in LDS.
+// Multiplies by a reciprocal of the pivot and uses fused multiply-adds (these kernels are not
+// part of the bit-exact set).
 
-// LU without pivoting of one diagonal block of A - S, S_jj = -sign(current diagonal).
-__global__ __launch_bounds__(256) void k_getf2_sign(int kb, double* __restrict__ A, long lda, int k0,
-                                                   double* __restrict__ sgn, int last_plus) {
-  __shared__ double S[NB][NB + 1];
-  const int tid = threadIdx.x;
+// Cholesky of one diagonal block (kb <= NB) at A[k0][k0], lower triangle in, L out. One
+// wavefront; lane i holds row i; the pivot column is exchanged through LDS.
+__global__ __launch_bounds__(64) void k_potf2(int kb, double* __restrict__ A, long lda, int k0,
+                                             int* __restrict__ info) {
+  __shared__ double col[2][NB];
+  const int lane = threadIdx.x;
   double* Ab = A + (long)k0 * lda + k0;
-  for (int e = tid; e < kb * kb; e += 256) {
-    const int i = e / kb, j = e % kb;
-    S[i][j] = Ab[(long)i * lda + j];
-  }
-  __syncthreads();
-  for (int j = 0; j < kb; j++) {
-    if (tid == 0) {
-      const double w = S[j][j];
-      double sj = signbit(w) ? 1.0 : -1.0;
-      if (last_plus && j == kb - 1) sj = -sj;
-      sgn[k0 + j] = sj;
-      S[j][j] = w - sj;
-    }
-    __syncthreads();
-    const double piv = S[j][j];
-    for (int i = j + 1 + tid; i < kb; i += 256) S[i][j] = S[i][j] / piv;
-    __syncthreads();
-    const int m = kb - j - 1;
-    for (int e = tid; e < m * m; e += 256) {
-      const int ii = e / m, kk = e % m;
-      S[j + 1 + ii][j + 1 + kk] -= S[j + 1 + ii][j] * S[j][j + 1 + kk];
-    }
-    __syncthreads();
-  }
-  for (int e = tid; e < kb * kb; e += 256) {
-    const int i = e / kb, j = e % kb;
-    Ab[(long)i * lda + j] = S[i][j];
-  }
-}
-
-// X_b <- op(L_bb)^-1 X_b for one diagonal block (kb <= NB), one lane per right-hand side: the
-// column lives in registers and the triangle is read with wave-uniform addresses (scalar loads),
-// so the substitution runs at VALU rate with no LDS traffic.
-template <bool UPPER, bool UNIT>
-__global__ __launch_bounds__(64) void k_trsm_left_blk(int kb, const double* __restrict__ L, long ldl,
-                                                      double* __restrict__ X, long ldx, long nrhs) {
-  const long col = (long)blockIdx.x * 64 + threadIdx.x;
-  if (col >= nrhs) return;
+  const int rl = lane < kb ? lane : kb - 1;
   double x[NB];
 #pragma unroll
-  for (int i = 0; i < NB; i++) x[i] = i < kb ? X[(long)i * ldx + col] : 0.0;
-  if (!UPPER) {
+  for (int k = 0; k < NB; k++) {
+    const double v = Ab[(long)rl * lda + (k < kb ? k : kb - 1)];
+    x[k] = (lane < kb && k < kb) ? (k <= lane ? v : 0.0) : (k == lane ? 1.0 : 0.0);
+  }
+  int bad = 0;
 #pragma unroll
-    for (int i = 0; i < NB; i++) {
-      if (i < kb) {
-        double acc = x[i];
+  for (int j = 0; j < NB; j++) {
+    const int buf = j & 1;
+    // the pivot: x[j] of lane j, published with the column
+    col[buf][lane] = x[j];
+    __syncthreads();
+    double d = col[buf][j];
+    if (!(d > 0.0)) {
+      bad = bad ? bad : j + 1;
+      d = 1.0;
+    }
+    const double piv = sqrt(d);
+    const double rp = 1.0 / piv;
+    const double l = lane > j ? x[j] * rp : 0.0;  // l_i below the pivot, zero elsewhere
+    x[j] = lane == j ? piv : (lane > j ? l : x[j]);
+    __syncthreads();
+    col[buf][lane] = l;
+    __syncthreads();
 #pragma unroll
-        for (int k = 0; k < i; k++) acc -= L[(long)i * ldl + k] * x[k];
-        x[i] = UNIT ? acc : acc / L[(long)i * ldl + i];
+    for (int k = j + 1; k < NB; k++) x[k] = fma(-l, col[buf][k], x[k]);
+  }
+  if (bad && lane == 0 && info && *info == 0 && bad <= kb) *info = k0 + bad;
+#pragma unroll
+  for (int k = 0; k < NB; k++)
+    if (lane < kb && k < kb) Ab[(long)lane * lda + k] = k <= lane ? x[k] : 0.0;
+}
+
+// LU without pivoting of one diagonal block of A - S, S_jj = -sign(current diagonal). One
+// wavefront; lane i holds row i; the pivot row goes through LDS.
+__global__ __launch_bounds__(64) void k_getf2_sign(int kb, double* __restrict__ A, long lda, int k0,
+                                                  double* __restrict__ sgn, int last_plus) {
+  __shared__ double row[NB];
+  __shared__ double pv[1];
+  const int lane = threadIdx.x;
+  double* Ab = A + (long)k0 * lda + k0;
+  const int rl = lane < kb ? lane : kb - 1;
+  double x[NB];
+#pragma unroll
+  for (int k = 0; k < NB; k++) {
+    const double v = Ab[(long)rl * lda + (k < kb ? k : kb - 1)];
+    x[k] = (lane < kb && k < kb) ? v : (k == lane ? 1.0 : 0.0);
+  }
+#pragma unroll
+  for (int j = 0; j < NB; j++) {
+    double w = 0.0;
+    if (lane == j) {
+      w = x[j];
+      double sj = signbit(w) ? 1.0 : -1.0;
+      if (last_plus && j == kb - 1) sj = -sj;
+      if (j < kb) sgn[k0 + j] = sj;
+      x[j] = w - sj;
+      pv[0] = x[j];
+#pragma unroll
+      for (int k = 0; k < NB; k++) row[k] = x[k];
+    }
+    __syncthreads();
+    const double rp = 1.0 / pv[0];
+    const double l = lane > j ? x[j] * rp : 0.0;
+    if (lane > j) x[j] = l;
+#pragma unroll
+    for (int k = j + 1; k < NB; k++) x[k] = fma(-l, row[k], x[k]);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int k = 0; k < NB; k++)
+    if (lane < kb && k < kb) Ab[(long)lane * lda + k] = x[k];
+}
+
+// Substitution on one lane's column x (FULL: kb == NB, so every bound is a constant).
+template <bool UPPER, bool UNIT, bool FULL>
+__device__ __forceinline__ void left_solve(int kb, const double* __restrict__ L, long ldl, double (&x)[NB]) {
+#pragma unroll
+  for (int q = 0; q < NB; q++) {
+    const int i = UPPER ? NB - 1 - q : q;
+    if (FULL || i < kb) {
+      const double* Li = L + (long)i * ldl;
+      double a0 = x[i], a1 = 0.0, a2 = 0.0, a3 = 0.0;
+#pragma unroll
+      for (int k = UPPER ? i + 1 : 0; k < (UPPER ? NB : i); k++) {
+        if (FULL || k < kb) {
+          if ((k & 3) == 0) a0 = fma(-Li[k], x[k], a0);
+          else if ((k & 3) == 1) a1 = fma(-Li[k], x[k], a1);
+          else if ((k & 3) == 2) a2 = fma(-Li[k], x[k], a2);
+          else a3 = fma(-Li[k], x[k], a3);
+        }
+      }
+      const double acc = (a0 + a1) + (a2 + a3);
+      x[i] = UNIT ? acc : acc / Li[i];
+    }
+  }
+}
+
+// Substitution from the right on one lane's row x: x op(U) = b.
+template <bool LOWER_T, bool UNIT, bool FULL>
+__device__ __forceinline__ void right_solve(int kb, const double* __restrict__ U, long ldu, double (&x)[NB]) {
+  if (LOWER_T) {
+#pragma unroll
+    for (int j = 0; j < NB; j++) {
+      if (FULL || j < kb) {
+        const double* Lj = U + (long)j * ldu;  // op(U)[k][j] = L[j][k]: row j of L
+        double a0 = x[j], a1 = 0.0, a2 = 0.0, a3 = 0.0;
+#pragma unroll
+        for (int k = 0; k < j; k++) {
+          if ((k & 3) == 0) a0 = fma(-x[k], Lj[k], a0);
+          else if ((k & 3) == 1) a1 = fma(-x[k], Lj[k], a1);
+          else if ((k & 3) == 2) a2 = fma(-x[k], Lj[k], a2);
+          else a3 = fma(-x[k], Lj[k], a3);
+        }
+        const double acc = (a0 + a1) + (a2 + a3);
+        x[j] = UNIT ? acc : acc / Lj[j];
       }
     }
   } else {
 #pragma unroll
-    for (int i = NB - 1; i >= 0; i--) {
-      if (i < kb) {
-        double acc = x[i];
+    for (int j = 0; j < NB; j++) {
+      if (FULL || j < kb) {
+        const double* Uj = U + (long)j * ldu;  // row j of U, right-looking
+        if (!UNIT) x[j] = x[j] / Uj[j];
 #pragma unroll
-        for (int k = i + 1; k < NB; k++)
-          if (k < kb) acc -= L[(long)i * ldl + k] * x[k];
-        x[i] = UNIT ? acc : acc / L[(long)i * ldl + i];
+        for (int k = j + 1; k < NB; k++)
+          if (FULL || k < kb) x[k] = fma(-x[j], Uj[k], x[k]);
       }
     }
   }
-#pragma unroll
-  for (int i = 0; i < NB; i++)
-    if (i < kb) X[(long)i * ldx + col] = x[i];
 }
 
-// X_b <- X_b U_bb^-1 (or X_b L_bb^-T) for one diagonal block, one lane per row (in registers).
+// X_b <- op(L_bb)^-1 X_b for one diagonal block (kb <= NB), one lane per right-hand side, the
+// column in registers. Left-looking: unknown i takes the dot product of row i of the triangle
+// (contiguous, wave-uniform, so it comes through the scalar cache) with the finished unknowns,
+// in four partial sums.
+template <bool UPPER, bool UNIT>
+__global__ __launch_bounds__(64) void k_trsm_left_blk(int kb, const double* __restrict__ L, long ldl,
+                                                      double* __restrict__ X, long ldx, long nrhs) {
+  const long col = (long)blockIdx.x * 64 + threadIdx.x;
+  const unsigned cl = (unsigned)(col < nrhs ? col : nrhs - 1);  // per-lane offset from a uniform row base
+  double x[NB];
+#pragma unroll
+  for (int i = 0; i < NB; i++) {
+    x[i] = 0.0;
+    if (i < kb) x[i] = (X + (long)i * ldx)[cl];
+  }
+  if (kb == NB) left_solve<UPPER, UNIT, true>(kb, L, ldl, x);
+  else left_solve<UPPER, UNIT, false>(kb, L, ldl, x);
+  if (col < nrhs) {
+#pragma unroll
+    for (int i = 0; i < NB; i++)
+      if (i < kb) (X + (long)i * ldx)[cl] = x[i];
+  }
+}
+
+// X_b <- X_b U_bb^-1 (or X_b L_bb^-T) for one diagonal block, one lane per row in registers.
+// Unknown j takes the dot product of the finished unknowns with column j of op(U), which is the
+// contiguous row j of L for L^T; for U the update runs right-looking along the contiguous row j.
 template <bool LOWER_T, bool UNIT>
 __global__ __launch_bounds__(64) void k_trsm_right_blk(int kb, const double* __restrict__ U, long ldu,
                                                        double* __restrict__ X, long ldx, long rows) {
   const long row = (long)blockIdx.x * 64 + threadIdx.x;
-  if (row >= rows) return;
+  const long rr = row < rows ? row : rows - 1;
+  double* xr = X + rr * ldx;  // this lane's row; element j at a constant offset
   double x[NB];
 #pragma unroll
-  for (int j = 0; j < NB; j++) x[j] = j < kb ? X[row * ldx + j] : 0.0;
-#pragma unroll
   for (int j = 0; j < NB; j++) {
-    if (j < kb) {
-      double acc = x[j];
-#pragma unroll
-      for (int k = 0; k < j; k++) {
-        const double ukj = LOWER_T ? U[(long)j * ldu + k] : U[(long)k * ldu + j];
-        acc -= x[k] * ukj;
-      }
-      x[j] = UNIT ? acc : acc / U[(long)j * ldu + j];
-    }
+    x[j] = 0.0;
+    if (j < kb) x[j] = xr[j];
   }
+  if (kb == NB) right_solve<LOWER_T, UNIT, true>(kb, U, ldu, x);
+  else right_solve<LOWER_T, UNIT, false>(kb, U, ldu, x);
+  if (row < rows) {
 #pragma unroll
-  for (int j = 0; j < NB; j++)
-    if (j < kb) X[row * ldx + j] = x[j];
+    for (int j = 0; j < NB; j++)
+      if (j < kb) xr[j] = x[j];
+  }
 }
 
 __global__ __launch_bounds__(256) void k_zero_strict_upper(int n, double* A, long lda) {
@@ -157,22 +219,22 @@ __global__ __launch_bounds__(256) void k_zero_strict_upper(int n, double* A, lon
 
 int launch_left_blk(int upper, int unit, int kb, const double* L, long ldl, double* X, long ldx, long nrhs,
                     hipStream_t s) {
-  const dim3 grid(exc_detail::ceil_div(nrhs, 64));
-  if (!upper && !unit) hipLaunchKernelGGL((k_trsm_left_blk<false, false>), grid, dim3(64), 0, s, kb, L, ldl, X, ldx, nrhs);
-  if (!upper && unit) hipLaunchKernelGGL((k_trsm_left_blk<false, true>), grid, dim3(64), 0, s, kb, L, ldl, X, ldx, nrhs);
-  if (upper && !unit) hipLaunchKernelGGL((k_trsm_left_blk<true, false>), grid, dim3(64), 0, s, kb, L, ldl, X, ldx, nrhs);
-  if (upper && unit) hipLaunchKernelGGL((k_trsm_left_blk<true, true>), grid, dim3(64), 0, s, kb, L, ldl, X, ldx, nrhs);
+  const dim3 grid(exc_detail::ceil_div(nrhs, 64)), block(64);
+  if (!upper && !unit) hipLaunchKernelGGL((k_trsm_left_blk<false, false>), grid, block, 0, s, kb, L, ldl, X, ldx, nrhs);
+  if (!upper && unit) hipLaunchKernelGGL((k_trsm_left_blk<false, true>), grid, block, 0, s, kb, L, ldl, X, ldx, nrhs);
+  if (upper && !unit) hipLaunchKernelGGL((k_trsm_left_blk<true, false>), grid, block, 0, s, kb, L, ldl, X, ldx, nrhs);
+  if (upper && unit) hipLaunchKernelGGL((k_trsm_left_blk<true, true>), grid, block, 0, s, kb, L, ldl, X, ldx, nrhs);
   EXC_CHECK_HIP(hipGetLastError());
   return EXC_OK;
 }
 
 int launch_right_blk(int lower_t, int unit, int kb, const double* U, long ldu, double* X, long ldx, long rows,
                      hipStream_t s) {
-  const dim3 grid(exc_detail::ceil_div(rows, 64));
-  if (!lower_t && !unit) hipLaunchKernelGGL((k_trsm_right_blk<false, false>), grid, dim3(64), 0, s, kb, U, ldu, X, ldx, rows);
-  if (!lower_t && unit) hipLaunchKernelGGL((k_trsm_right_blk<false, true>), grid, dim3(64), 0, s, kb, U, ldu, X, ldx, rows);
-  if (lower_t && !unit) hipLaunchKernelGGL((k_trsm_right_blk<true, false>), grid, dim3(64), 0, s, kb, U, ldu, X, ldx, rows);
-  if (lower_t && unit) hipLaunchKernelGGL((k_trsm_right_blk<true, true>), grid, dim3(64), 0, s, kb, U, ldu, X, ldx, rows);
+  const dim3 grid(exc_detail::ceil_div(rows, 64)), block(64);
+  if (!lower_t && !unit) hipLaunchKernelGGL((k_trsm_right_blk<false, false>), grid, block, 0, s, kb, U, ldu, X, ldx, rows);
+  if (!lower_t && unit) hipLaunchKernelGGL((k_trsm_right_blk<false, true>), grid, block, 0, s, kb, U, ldu, X, ldx, rows);
+  if (lower_t && !unit) hipLaunchKernelGGL((k_trsm_right_blk<true, false>), grid, block, 0, s, kb, U, ldu, X, ldx, rows);
+  if (lower_t && unit) hipLaunchKernelGGL((k_trsm_right_blk<true, true>), grid, block, 0, s, kb, U, ldu, X, ldx, rows);
   EXC_CHECK_HIP(hipGetLastError());
   return EXC_OK;
 }
@@ -186,105 +248,114 @@ size_t work_bytes(int n, long wide) {
   return (size_t)8 * n * n * sizeof(double);
 }
 
+// Split point of a recursive step: a multiple of NB near n / 2, 0 < n1 < n (for n > NB).
+static int split(int n) {
+  int n1 = ((n / NB + 1) / 2) * NB;
+  if (n1 <= 0) n1 = NB;
+  if (n1 >= n) n1 = n - NB;
+  return n1;
+}
+
 int trsm_left(int upper, int unit, int n, long nrhs, const double* L, long ldl, double* X, long ldx, void* work,
               size_t wb, hipStream_t s) {
   EXC_REQUIRE(n >= 0 && nrhs >= 0 && L && X, EXC_ERR_ARG, "trsm_left: bad arguments");
   if (n == 0 || nrhs == 0) return EXC_OK;
-  const int nblk = (n + NB - 1) / NB;
+  if (n <= NB) return launch_left_blk(upper, unit, n, L, ldl, X, ldx, nrhs, s);
+  const int n1 = split(n), n2 = n - n1;
   ExcGemmOpts o;
   o.work = work;
   o.work_bytes = wb;
-  for (int q = 0; q < nblk; q++) {
-    const int b = upper ? nblk - 1 - q : q;
-    const int r0 = b * NB, kb = std::min(NB, n - r0), r1 = r0 + kb;
-    if (!upper && r0 > 0) {
-      int rc = exc_gemm_opts(0, 0, kb, nrhs, r0, -1.0, L + (long)r0 * ldl, ldl, X, ldx, 1.0, X + (long)r0 * ldx,
-                             ldx, o, s);
-      if (rc) return rc;
-    }
-    if (upper && r1 < n) {
-      int rc = exc_gemm_opts(0, 0, kb, nrhs, n - r1, -1.0, L + (long)r0 * ldl + r1, ldl, X + (long)r1 * ldx, ldx,
-                             1.0, X + (long)r0 * ldx, ldx, o, s);
-      if (rc) return rc;
-    }
-    int rc = launch_left_blk(upper, unit, kb, L + (long)r0 * ldl + r0, ldl, X + (long)r0 * ldx, ldx, nrhs, s);
-    if (rc) return rc;
+  double* X2 = X + (long)n1 * ldx;
+  const double* L22 = L + (long)n1 * ldl + n1;
+  int rc;
+  if (!upper) {
+    // [L11 0; L21 L22]: X1 <- L11^-1 X1, X2 <- L22^-1 (X2 - L21 X1)
+    if ((rc = trsm_left(0, unit, n1, nrhs, L, ldl, X, ldx, work, wb, s))) return rc;
+    if ((rc = exc_gemm_opts(0, 0, n2, nrhs, n1, -1.0, L + (long)n1 * ldl, ldl, X, ldx, 1.0, X2, ldx, o, s)))
+      return rc;
+    return trsm_left(0, unit, n2, nrhs, L22, ldl, X2, ldx, work, wb, s);
   }
-  return EXC_OK;
+  // [U11 U12; 0 U22]: X2 <- U22^-1 X2, X1 <- U11^-1 (X1 - U12 X2)
+  if ((rc = trsm_left(1, unit, n2, nrhs, L22, ldl, X2, ldx, work, wb, s))) return rc;
+  if ((rc = exc_gemm_opts(0, 0, n1, nrhs, n2, -1.0, L + n1, ldl, X2, ldx, 1.0, X, ldx, o, s))) return rc;
+  return trsm_left(1, unit, n1, nrhs, L, ldl, X, ldx, work, wb, s);
 }
 
 int trsm_right(int lower_t, int unit, int n, long rows, const double* U, long ldu, double* X, long ldx, void* work,
                size_t wb, hipStream_t s) {
   EXC_REQUIRE(n >= 0 && rows >= 0 && U && X, EXC_ERR_ARG, "trsm_right: bad arguments");
   if (n == 0 || rows == 0) return EXC_OK;
-  const int nblk = (n + NB - 1) / NB;
+  if (n <= NB) return launch_right_blk(lower_t, unit, n, U, ldu, X, ldx, rows, s);
+  const int n1 = split(n), n2 = n - n1;
   ExcGemmOpts o;
   o.work = work;
   o.work_bytes = wb;
-  for (int b = 0; b < nblk; b++) {
-    const int c0 = b * NB, kb = std::min(NB, n - c0);
-    if (c0 > 0) {
-      // X[:, c0:c0+kb] -= X[:, 0:c0] op(U)[0:c0, c0:c0+kb], op(U) = L^T for lower_t
-      int rc = lower_t ? exc_gemm_opts(0, 1, rows, kb, c0, -1.0, X, ldx, U + (long)c0 * ldu, ldu, 1.0, X + c0, ldx, o, s)
-                       : exc_gemm_opts(0, 0, rows, kb, c0, -1.0, X, ldx, U + c0, ldu, 1.0, X + c0, ldx, o, s);
-      if (rc) return rc;
-    }
-    int rc = launch_right_blk(lower_t, unit, kb, U + (long)c0 * ldu + c0, ldu, X + c0, ldx, rows, s);
-    if (rc) return rc;
+  // op(U) = [V11 V12; 0 V22] (V = U, or L^T): X1 <- X1 V11^-1, X2 <- (X2 - X1 V12) V22^-1
+  int rc;
+  if ((rc = trsm_right(lower_t, unit, n1, rows, U, ldu, X, ldx, work, wb, s))) return rc;
+  rc = lower_t ? exc_gemm_opts(0, 1, rows, n2, n1, -1.0, X, ldx, U + (long)n1 * ldu, ldu, 1.0, X + n1, ldx, o, s)
+               : exc_gemm_opts(0, 0, rows, n2, n1, -1.0, X, ldx, U + n1, ldu, 1.0, X + n1, ldx, o, s);
+  if (rc) return rc;
+  return trsm_right(lower_t, unit, n2, rows, U + (long)n1 * ldu + n1, ldu, X + n1, ldx, work, wb, s);
+}
+
+static int potrf_rec(int n, double* A, long lda, int k0, int* info, void* work, size_t wb, hipStream_t s) {
+  if (n <= NB) {
+    hipLaunchKernelGGL(k_potf2, dim3(1), dim3(64), 0, s, n, A, lda, k0, info);
+    EXC_CHECK_HIP(hipGetLastError());
+    return EXC_OK;
   }
-  return EXC_OK;
+  const int n1 = split(n), n2 = n - n1;
+  double* A11 = A + (long)k0 * lda + k0;
+  double* A21 = A11 + (long)n1 * lda;
+  int rc;
+  if ((rc = potrf_rec(n1, A, lda, k0, info, work, wb, s))) return rc;
+  if ((rc = trsm_right(1, 0, n1, n2, A11, lda, A21, lda, work, wb, s))) return rc;  // A21 <- A21 L11^-T
+  ExcGemmOpts o;
+  o.work = work;
+  o.work_bytes = wb;
+  o.lower_only = 1;
+  if ((rc = exc_gemm_opts(0, 1, n2, n2, n1, -1.0, A21, lda, A21, lda, 1.0, A21 + n1, lda, o, s))) return rc;
+  return potrf_rec(n2, A, lda, k0 + n1, info, work, wb, s);
 }
 
 int potrf_lower(int n, double* A, long lda, int* info, void* work, size_t wb, hipStream_t s) {
   EXC_REQUIRE(n >= 0 && A, EXC_ERR_ARG, "potrf_lower: bad arguments");
   if (n == 0) return EXC_OK;
-  ExcGemmOpts o;
-  o.work = work;
-  o.work_bytes = wb;
-  o.lower_only = 1;
-  for (int k0 = 0; k0 < n; k0 += NB) {
-    const int kb = std::min(NB, n - k0);
-    hipLaunchKernelGGL(k_potf2, dim3(1), dim3(256), 0, s, kb, A, lda, k0, info);
-    EXC_CHECK_HIP(hipGetLastError());
-    const int r0 = k0 + kb;
-    if (r0 < n) {
-      const long rows = n - r0;
-      int rc = launch_right_blk(1, 0, kb, A + (long)k0 * lda + k0, lda, A + (long)r0 * lda + k0, lda, rows, s);
-      if (rc) return rc;
-      rc = exc_gemm_opts(0, 1, rows, rows, kb, -1.0, A + (long)r0 * lda + k0, lda, A + (long)r0 * lda + k0, lda, 1.0,
-                         A + (long)r0 * lda + r0, lda, o, s);
-      if (rc) return rc;
-    }
-  }
+  int rc = potrf_rec(n, A, lda, 0, info, work, wb, s);
+  if (rc) return rc;
   hipLaunchKernelGGL(k_zero_strict_upper, dim3(exc_detail::ceil_div((long)n * n, 256)), dim3(256), 0, s, n, A, lda);
   EXC_CHECK_HIP(hipGetLastError());
   return EXC_OK;
 }
 
-int lu_sign(int n, double* A, long lda, double* sgn, int square_last, void* work, size_t wb, hipStream_t s) {
-  EXC_REQUIRE(n >= 0 && A && sgn, EXC_ERR_ARG, "lu_sign: bad arguments");
+static int lu_rec(int n, double* A, long lda, int k0, int total, double* sgn, int square_last, void* work, size_t wb,
+                  hipStream_t s) {
+  double* A11 = A + (long)k0 * lda + k0;
+  if (n <= NB) {
+    const int last_plus = (square_last && k0 + n == total) ? 1 : 0;
+    hipLaunchKernelGGL(k_getf2_sign, dim3(1), dim3(64), 0, s, n, A, lda, k0, sgn, last_plus);
+    EXC_CHECK_HIP(hipGetLastError());
+    return EXC_OK;
+  }
+  const int n1 = split(n), n2 = n - n1;
+  double* A12 = A11 + n1;
+  double* A21 = A11 + (long)n1 * lda;
+  int rc;
+  if ((rc = lu_rec(n1, A, lda, k0, total, sgn, square_last, work, wb, s))) return rc;
+  if ((rc = trsm_left(0, 1, n1, n2, A11, lda, A12, lda, work, wb, s))) return rc;   // A12 <- L11^-1 A12
+  if ((rc = trsm_right(0, 0, n1, n2, A11, lda, A21, lda, work, wb, s))) return rc;  // A21 <- A21 U11^-1
   ExcGemmOpts o;
   o.work = work;
   o.work_bytes = wb;
-  for (int k0 = 0; k0 < n; k0 += NB) {
-    const int kb = std::min(NB, n - k0);
-    const int last_plus = (square_last && k0 + kb == n) ? 1 : 0;
-    hipLaunchKernelGGL(k_getf2_sign, dim3(1), dim3(256), 0, s, kb, A, lda, k0, sgn, last_plus);
-    EXC_CHECK_HIP(hipGetLastError());
-    const int r0 = k0 + kb;
-    if (r0 < n) {
-      const long rest = n - r0;
-      double* A11 = A + (long)k0 * lda + k0;
-      int rc = launch_left_blk(0, 1, kb, A11, lda, A + (long)k0 * lda + r0, lda, rest, s);
-      if (rc) return rc;
-      rc = launch_right_blk(0, 0, kb, A11, lda, A + (long)r0 * lda + k0, lda, rest, s);
-      if (rc) return rc;
-      rc = exc_gemm_opts(0, 0, rest, rest, kb, -1.0, A + (long)r0 * lda + k0, lda, A + (long)k0 * lda + r0, lda, 1.0,
-                         A + (long)r0 * lda + r0, lda, o, s);
-      if (rc) return rc;
-    }
-  }
-  return EXC_OK;
+  if ((rc = exc_gemm_opts(0, 0, n2, n2, n1, -1.0, A21, lda, A12, lda, 1.0, A21 + n1, lda, o, s))) return rc;
+  return lu_rec(n2, A, lda, k0 + n1, total, sgn, square_last, work, wb, s);
+}
+
+int lu_sign(int n, double* A, long lda, double* sgn, int square_last, void* work, size_t wb, hipStream_t s) {
+  EXC_REQUIRE(n >= 0 && A && sgn, EXC_ERR_ARG, "lu_sign: bad arguments");
+  if (n == 0) return EXC_OK;
+  return lu_rec(n, A, lda, 0, n, sgn, square_last, work, wb, s);
 }
 
 }  // namespace fact

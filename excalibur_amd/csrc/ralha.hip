@@ -41,6 +41,182 @@ __device__ inline double block_sum(double v, double* red) {
   return out;
 }
 
+// The reflector of step i from the reduced message g (LAPACK dlarfg on n = m-1-i entries, the
+// same fixed-order reduction in every workgroup): hbuf[0..n) = h (h_0 = 1), returns tau; also
+// beta_{i-1} and alpha_i from the four reduced dots (client.cpp:858-884). Workgroup 0 stores
+// column i+1 of house, tau[i+1], beta[i-1] and alpha[i]. Ends with a barrier.
+__device__ double ralha_reflector(int i, int m, const double* __restrict__ g, double* hbuf, double* red,
+                                  double* __restrict__ house, double* __restrict__ tau, double* __restrict__ alpha,
+                                  double* __restrict__ beta, double* out_b, double* out_a) {
+  const int tid = threadIdx.x;
+  const int n = m - 1 - i;
+  double ss = 0.0;
+  for (int r = 1 + tid; r < n; r += blockDim.x) ss += g[r] * g[r];
+  double xnorm = sqrt(block_sum(ss, red));
+  double alph = g[0], t = 0.0, scale = 0.0;
+  if (n > 1 && xnorm != 0.0) {
+    double bet = -copysign(dlapy2_dev(alph, xnorm), alph);
+    double xs = 1.0;
+    const double safmin = DBL_MIN / (DBL_EPSILON * 0.5);
+    if (fabs(bet) < safmin) {
+      const double rs = 1.0 / safmin;
+      int knt = 0;
+      do {
+        knt++;
+        xs *= rs;
+        bet *= rs;
+        alph *= rs;
+      } while (fabs(bet) < safmin && knt < 20);
+      double s2 = 0.0;
+      for (int r = 1 + tid; r < n; r += blockDim.x) s2 += (g[r] * xs) * (g[r] * xs);
+      xnorm = sqrt(block_sum(s2, red));
+      bet = -copysign(dlapy2_dev(alph, xnorm), alph);
+    }
+    t = (bet - alph) / bet;
+    scale = xs * (1.0 / (alph - bet));
+  }
+  for (int r = tid; r < n; r += blockDim.x) hbuf[r] = r == 0 ? 1.0 : g[r] * scale;
+  const double b = i > 0 ? g[m - i - 1] : 0.0;
+  const double a = i > 0 ? sqrt(g[m - i] * (b * b) + g[m - i + 1] - 2 * b * g[m - i + 2]) : 0.0;
+  if (blockIdx.x == 0) {
+    for (int r = tid; r < n; r += blockDim.x) house[(long)(i + 1 + r) * m + (i + 1)] = r == 0 ? 1.0 : g[r] * scale;
+    if (tid == 0) {
+      tau[i + 1] = t;
+      if (i > 0) {
+        beta[i - 1] = b;
+        alpha[i] = a;
+      }
+    }
+  }
+  __syncthreads();
+  *out_b = b;
+  *out_a = a;
+  return t;
+}
+
+// Register-resident step for m <= 64 * KM: each wavefront holds CPW whole columns of F (row
+// lane + 64q in slot q), so a column is read once, updated, stored once, and its share of the next
+// message is formed from the registers; the four wavefronts' shares are added through LDS.
+// Same operations as k_ralha_step below, with the dot products split by row slot.
+template <int KM, int CPW>
+__global__ __launch_bounds__(256) void k_ralha_step_reg(int i, int m, int c, double* __restrict__ Fc,
+                                                        double* __restrict__ Uc, const double* __restrict__ g,
+                                                        double* __restrict__ house, double* __restrict__ tau,
+                                                        double* __restrict__ alpha, double* __restrict__ beta,
+                                                        double* __restrict__ partial) {
+  extern __shared__ double lds[];  // hbuf: m, then mp: 4 x (m + 4)
+  double* hbuf = lds;
+  double* mp = lds + m;
+  __shared__ double red[8];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int ldm = m + 4;
+  double fr[CPW][KM];
+  double um1[CPW];  // u_{i-1} of each column (i > 0)
+  int cols[CPW];
+#pragma unroll
+  for (int cc = 0; cc < CPW; cc++) {
+    const int col = (blockIdx.x * 4 + wave) * CPW + cc;
+    cols[cc] = col;
+    um1[cc] = 0.0;
+#pragma unroll
+    for (int q = 0; q < KM; q++) {
+      const int row = lane + 64 * q;
+      fr[cc][q] = (col < c && row < m && row >= (i < 0 ? 0 : i)) ? Fc[(long)col * m + row] : 0.0;
+    }
+    if (col < c && i > 0) um1[cc] = Uc[(long)col * m + i - 1];
+  }
+  double ui[CPW];  // u_i (i >= 0), the u_{j-1} of the next message
+#pragma unroll
+  for (int cc = 0; cc < CPW; cc++) ui[cc] = 0.0;
+  if (i >= 0) {
+    double b, a;
+    const double t = ralha_reflector(i, m, g, hbuf, red, house, tau, alpha, beta, &b, &a);
+#pragma unroll
+    for (int cc = 0; cc < CPW; cc++) {
+      if (cols[cc] >= c) continue;  // uniform within the wavefront
+      if (t != 0.0) {
+        double s = 0.0;
+#pragma unroll
+        for (int q = 0; q < KM; q++) {
+          const int row = lane + 64 * q;
+          if (row > i && row < m) s += fr[cc][q] * hbuf[row - i - 1];
+        }
+        s = wave_sum(s);
+#pragma unroll
+        for (int q = 0; q < KM; q++) {
+          const int row = lane + 64 * q;
+          if (row > i && row < m) fr[cc][q] += s * (-t * hbuf[row - i - 1]);
+        }
+      }
+      if (i > 0) {
+        // f_i -= beta u_{i-1}; u_i = f_i / alpha_i (row i sits in lane i % 64, slot i / 64)
+#pragma unroll
+        for (int q = 0; q < KM; q++) {
+          if (lane + 64 * q == i) {
+            const double fi = fr[cc][q] + (-b) * um1[cc];
+            fr[cc][q] = fi;
+            ui[cc] = (1.0 / a) * fi;
+            Uc[(long)cols[cc] * m + i] = ui[cc];
+          }
+        }
+      } else {
+        // u_0 was set by the tail kernels before the loop
+#pragma unroll
+        for (int q = 0; q < KM; q++)
+          if (lane + 64 * q == 0) ui[cc] = Uc[(long)cols[cc] * m];
+      }
+      ui[cc] = __shfl(ui[cc], i & 63, kWave);
+#pragma unroll
+      for (int q = 0; q < KM; q++) {
+        const int row = lane + 64 * q;
+        if (row >= i && row < m) Fc[(long)cols[cc] * m + row] = fr[cc][q];
+      }
+    }
+  }
+  const int j = i + 1;
+  if (j > m - 3) return;  // uniform: no message after the last full step
+  const int nrows = m - 1 - j;
+  // this wavefront's share: sum over its columns of f_{j+1+r} f_j, and the four dots
+  double pr[KM];
+#pragma unroll
+  for (int q = 0; q < KM; q++) pr[q] = 0.0;
+  double d0 = 0.0, d1 = 0.0, d2 = 0.0, d3 = 0.0;
+#pragma unroll
+  for (int cc = 0; cc < CPW; cc++) {
+    if (cols[cc] >= c) continue;
+    double fj = 0.0;
+#pragma unroll
+    for (int q = 0; q < KM; q++)
+      if (lane + 64 * q == j) fj = fr[cc][q];
+    fj = __shfl(fj, j & 63, kWave);
+#pragma unroll
+    for (int q = 0; q < KM; q++) pr[q] += fr[cc][q] * fj;
+    if (j > 0) {
+      const double u = ui[cc];
+      d0 += fj * u;
+      d1 += u * u;
+      d2 += fj * fj;
+      d3 += u * fj;
+    }
+  }
+  double* mw = mp + wave * ldm;
+#pragma unroll
+  for (int q = 0; q < KM; q++) {
+    const int row = lane + 64 * q;
+    if (row > j && row < m) mw[row - j - 1] = pr[q];
+  }
+  if (lane == 0) {
+    mw[nrows] = d0;
+    mw[nrows + 1] = d1;
+    mw[nrows + 2] = d2;
+    mw[nrows + 3] = d3;
+  }
+  __syncthreads();
+  const int cnt = nrows + (j > 0 ? 4 : 0);
+  double* out = partial + (long)blockIdx.x * (m + 3);
+  for (int r = tid; r < cnt; r += blockDim.x) out[r] = ((mp[r] + mp[ldm + r]) + mp[2 * ldm + r]) + mp[3 * ldm + r];
+}
+
 // One column step of the overlapped Ralha loop on column-major F and U (Fc[col*m + row]):
 //  apply (i >= 0): the reduced message g of step i -> LAPACK dlarfg (n = m-1-i) into column i+1
 //    of house and tau[i+1]; F[i+1:, col] -= tau h (h . F[i+1:, col]) for this workgroup's columns
@@ -59,45 +235,8 @@ __global__ __launch_bounds__(256) void k_ralha_step(int i, int m, int c, double*
   const int c0 = blockIdx.x * kColsPerWg;
   if (i >= 0) {
     const int n = m - 1 - i;
-    double ss = 0.0;
-    for (int r = 1 + tid; r < n; r += blockDim.x) ss += g[r] * g[r];
-    double xnorm = sqrt(block_sum(ss, red));
-    double alph = g[0], t = 0.0, scale = 0.0;
-    if (n > 1 && xnorm != 0.0) {
-      double bet = -copysign(dlapy2_dev(alph, xnorm), alph);
-      double xs = 1.0;
-      const double safmin = DBL_MIN / (DBL_EPSILON * 0.5);
-      if (fabs(bet) < safmin) {
-        const double rs = 1.0 / safmin;
-        int knt = 0;
-        do {
-          knt++;
-          xs *= rs;
-          bet *= rs;
-          alph *= rs;
-        } while (fabs(bet) < safmin && knt < 20);
-        double s2 = 0.0;
-        for (int r = 1 + tid; r < n; r += blockDim.x) s2 += (g[r] * xs) * (g[r] * xs);
-        xnorm = sqrt(block_sum(s2, red));
-        bet = -copysign(dlapy2_dev(alph, xnorm), alph);
-      }
-      t = (bet - alph) / bet;
-      scale = xs * (1.0 / (alph - bet));
-    }
-    for (int r = tid; r < n; r += blockDim.x) hbuf[r] = r == 0 ? 1.0 : g[r] * scale;
-    __syncthreads();
-    const double b = i > 0 ? g[m - i - 1] : 0.0;
-    const double a = i > 0 ? sqrt(g[m - i] * (b * b) + g[m - i + 1] - 2 * b * g[m - i + 2]) : 0.0;
-    if (blockIdx.x == 0) {
-      for (int r = tid; r < n; r += blockDim.x) house[(long)(i + 1 + r) * m + (i + 1)] = hbuf[r];
-      if (tid == 0) {
-        tau[i + 1] = t;
-        if (i > 0) {
-          beta[i - 1] = b;
-          alpha[i] = a;
-        }
-      }
-    }
+    double b, a;
+    const double t = ralha_reflector(i, m, g, hbuf, red, house, tau, alpha, beta, &b, &a);
     for (int q = wave; q < kColsPerWg; q += (int)(blockDim.x >> 6)) {
       const int col = c0 + q;
       if (col >= c) break;
@@ -136,17 +275,32 @@ __global__ __launch_bounds__(256) void k_ralha_step(int i, int m, int c, double*
   }
 }
 
-// msg[r] = sum over workgroups (in order) of partial[wg][r]; zeros after the valid entries.
+// msg[r] = sum over the step workgroups of partial[wg][r]; zeros after the valid entries. A
+// workgroup owns 64 message entries; its four wavefronts each add every fourth partial (eight
+// loads in flight per lane), then the four sums are added in wavefront order.
 __global__ __launch_bounds__(256) void k_ralha_reduce(int j, int m, int nwg, const double* __restrict__ partial,
                                                       double* __restrict__ msg) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ double acc[4][64];
+  const int lane = threadIdx.x & 63, grp = threadIdx.x >> 6;
+  const int r = blockIdx.x * 64 + lane;
   const int len = m + 3 - j;
-  if (r >= len) return;
   const int valid = (m - 1 - j) + (j > 0 ? 4 : 0);
+  const long ld = m + 3;
   double s = 0.0;
-  if (r < valid)
-    for (int w = 0; w < nwg; w++) s += partial[(long)w * (m + 3) + r];
-  msg[r] = s;
+  if (r < valid) {
+    int w = grp;
+    for (; w + 28 < nwg; w += 32) {
+      double v[8];
+#pragma unroll
+      for (int q = 0; q < 8; q++) v[q] = partial[(long)(w + 4 * q) * ld + r];
+#pragma unroll
+      for (int q = 0; q < 8; q++) s += v[q];
+    }
+    for (; w < nwg; w += 4) s += partial[(long)w * ld + r];
+  }
+  acc[grp][lane] = s;
+  __syncthreads();
+  if (grp == 0 && r < len) msg[r] = ((acc[0][lane] + acc[1][lane]) + acc[2][lane]) + acc[3][lane];
 }
 
 // dot = f_i . u_{i-1} over the columns (column-major F and U)
@@ -194,12 +348,16 @@ __global__ __launch_bounds__(256) void k_ralha_tail_scale(int i, int m, int c, c
     Uc[(long)k * m + i] = inv * Fc[(long)k * m + i];
 }
 
-// Lower to upper bidiagonal with the reference BLAS drotg (client.cpp:947-954), one thread.
+// Lower to upper bidiagonal with the reference BLAS drotg (client.cpp:947-954), one thread. The
+// updated alpha_{i+1} is carried in a register, so the loads of beta and alpha ahead can be
+// issued before the dependent chain needs them.
 __global__ void k_lower_to_upper(int m, double* __restrict__ alpha, double* __restrict__ beta,
                                  double* __restrict__ cs) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double a = alpha[0];
+#pragma unroll 8
   for (int i = 0; i < m - 1; i++) {
-    double a = alpha[i], b = beta[i];
+    const double b = beta[i], an = alpha[i + 1];
     double roe = fabs(a) > fabs(b) ? a : b;
     double scl = fabs(a) + fabs(b);
     double c, s, r;
@@ -215,24 +373,42 @@ __global__ void k_lower_to_upper(int m, double* __restrict__ alpha, double* __re
     alpha[i] = r;
     cs[2 * i] = c;
     cs[2 * i + 1] = s;
-    beta[i] = s * alpha[i + 1];
-    alpha[i + 1] = c * alpha[i + 1];
+    beta[i] = s * an;
+    a = c * an;
   }
+  alpha[m - 1] = a;
 }
 
 // Sign rule of client.cpp:987-995: flip[j] = -1 when the first largest-magnitude entry of
-// column j of bu (m x k) is negative.
+// column j of bu (m x k) is negative. A workgroup takes 32 columns; its 8 row groups scan every
+// eighth row (coalesced across the columns), then the candidates are merged keeping the larger
+// magnitude and, on ties, the smaller row (cblas_idamax's first maximum).
 __global__ __launch_bounds__(256) void k_sign_flags(int m, int k, const double* __restrict__ bu,
                                                     double* __restrict__ flip) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= k) return;
-  int p = 0;
+  __shared__ double bv[8][32];
+  __shared__ int bi[8][32];
+  const int cl = threadIdx.x & 31, g = threadIdx.x >> 5;
+  const int j = blockIdx.x * 32 + cl;
+  int p = -1;
   double best = -1.0;
-  for (int i = 0; i < m; i++) {
-    const double a = fabs(bu[(long)i * k + j]);
-    if (a > best) { best = a; p = i; }
+  if (j < k) {
+#pragma unroll 8
+    for (int i = g; i < m; i += 8) {
+      const double v = fabs(bu[(long)i * k + j]);
+      if (v > best) { best = v; p = i; }
+    }
   }
-  flip[j] = bu[(long)p * k + j] < 0 ? -1.0 : 1.0;
+  bv[g][cl] = best;
+  bi[g][cl] = p;
+  __syncthreads();
+  if (g == 0 && j < k) {
+    for (int q = 1; q < 8; q++) {
+      const double v = bv[q][cl];
+      const int iq = bi[q][cl];
+      if (iq >= 0 && (v > best || (v == best && iq < p))) { best = v; p = iq; }
+    }
+    flip[j] = (p >= 0 && bu[(long)p * k + j] < 0) ? -1.0 : 1.0;
+  }
 }
 
 __global__ __launch_bounds__(256) void k_apply_flags(int m, int k, double* __restrict__ bu, double* __restrict__ bv,
@@ -251,6 +427,7 @@ __global__ __launch_bounds__(256) void k_rot_back(int m, int k, double* __restri
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= k || m < 2) return;
   double y = bu[(long)(m - 1) * k + j];
+#pragma unroll 8
   for (int i = m - 2; i >= 0; i--) {
     const double c = cs[2 * i], s = -cs[2 * i + 1];
     const double x = bu[(long)i * k + j];
@@ -270,30 +447,78 @@ size_t exc_ralha_workspace(int m, int c) {
   return (size_t)exc_detail::ceil_div(c, kColsPerWg) * (m + 3) * sizeof(double) + 256;
 }
 
-int exc_ralha_step(int i, int m, int c, double* d_fc, double* d_uc, const double* d_g, double* d_house,
-                   double* d_tau, double* d_alpha, double* d_beta, double* d_msg, void* d_work, size_t work_bytes,
-                   exc_stream_t stream) {
+}  // extern "C"
+
+namespace {
+
+template <int KM, int CPW>
+int launch_step_reg(int i, int m, int c, double* fc, double* uc, const double* g, double* house, double* tau,
+                    double* alpha, double* beta, double* partial, hipStream_t s, unsigned* nwg_out) {
+  const unsigned nwg = exc_detail::ceil_div(c, 4 * CPW);
+  const size_t lds = (size_t)(5 * m + 16) * sizeof(double);
+  if (lds > 64 * 1024)
+    EXC_CHECK_HIP(hipFuncSetAttribute((const void*)k_ralha_step_reg<KM, CPW>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  hipLaunchKernelGGL((k_ralha_step_reg<KM, CPW>), dim3(nwg), dim3(256), lds, s, i, m, c, fc, uc, g, house, tau, alpha,
+                     beta, partial);
+  EXC_CHECK_HIP(hipGetLastError());
+  *nwg_out = nwg;
+  return EXC_OK;
+}
+
+int ralha_step(bool generic, int i, int m, int c, double* d_fc, double* d_uc, const double* d_g, double* d_house,
+               double* d_tau, double* d_alpha, double* d_beta, double* d_msg, void* d_work, size_t work_bytes,
+               exc_stream_t stream) {
   EXC_REQUIRE(m >= 3 && i >= -1 && i <= m - 3 && c >= 1 && d_fc && d_uc && (i < 0 || d_g) && d_house && d_tau &&
                   d_alpha && d_beta && d_msg && d_work,
               EXC_ERR_ARG, "exc_ralha_step: bad arguments");
-  const unsigned nwg = exc_detail::ceil_div(c, kColsPerWg);
-  EXC_REQUIRE(work_bytes >= (size_t)nwg * (m + 3) * sizeof(double), EXC_ERR_SPACE,
+  EXC_REQUIRE(work_bytes >= (size_t)exc_detail::ceil_div(c, kColsPerWg) * (m + 3) * sizeof(double), EXC_ERR_SPACE,
               "exc_ralha_step: workspace too small");
   hipStream_t s = exc_detail::stream(stream);
   double* partial = (double*)d_work;
-  const size_t lds = (size_t)m * sizeof(double);
-  EXC_REQUIRE(lds <= 160 * 1024, EXC_ERR_ARG, "exc_ralha_step: m above 20480 does not fit the reflector in LDS");
-  if (lds > 64 * 1024)
-    EXC_CHECK_HIP(hipFuncSetAttribute((const void*)k_ralha_step, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  hipLaunchKernelGGL(k_ralha_step, dim3(nwg), dim3(256), lds, s, i, m, c, d_fc, d_uc, d_g, d_house, d_tau, d_alpha,
-                     d_beta, partial);
-  EXC_CHECK_HIP(hipGetLastError());
+  unsigned nwg = 0;
+  int rc = EXC_OK;
+  if (!generic && m <= 256) rc = launch_step_reg<4, 2>(i, m, c, d_fc, d_uc, d_g, d_house, d_tau, d_alpha, d_beta, partial, s, &nwg);
+  else if (!generic && m <= 512) rc = launch_step_reg<8, 2>(i, m, c, d_fc, d_uc, d_g, d_house, d_tau, d_alpha, d_beta, partial, s, &nwg);
+  else if (!generic && m <= 1024) rc = launch_step_reg<16, 2>(i, m, c, d_fc, d_uc, d_g, d_house, d_tau, d_alpha, d_beta, partial, s, &nwg);
+  else if (!generic && m <= 2048) rc = launch_step_reg<32, 1>(i, m, c, d_fc, d_uc, d_g, d_house, d_tau, d_alpha, d_beta, partial, s, &nwg);
+  else if (!generic && m <= 4000) rc = launch_step_reg<64, 1>(i, m, c, d_fc, d_uc, d_g, d_house, d_tau, d_alpha, d_beta, partial, s, &nwg);
+  else {
+    nwg = exc_detail::ceil_div(c, kColsPerWg);
+    const size_t lds = (size_t)m * sizeof(double);
+    EXC_REQUIRE(lds <= 160 * 1024, EXC_ERR_ARG, "exc_ralha_step: m above 20480 does not fit the reflector in LDS");
+    if (lds > 64 * 1024)
+      EXC_CHECK_HIP(
+          hipFuncSetAttribute((const void*)k_ralha_step, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(k_ralha_step, dim3(nwg), dim3(256), lds, s, i, m, c, d_fc, d_uc, d_g, d_house, d_tau, d_alpha,
+                       d_beta, partial);
+    EXC_CHECK_HIP(hipGetLastError());
+  }
+  if (rc) return rc;
   if (i + 1 <= m - 3) {
-    hipLaunchKernelGGL(k_ralha_reduce, dim3(exc_detail::ceil_div(m + 3, 256)), dim3(256), 0, s, i + 1, m, (int)nwg,
+    hipLaunchKernelGGL(k_ralha_reduce, dim3(exc_detail::ceil_div(m + 3, 64)), dim3(256), 0, s, i + 1, m, (int)nwg,
                        partial, d_msg);
     EXC_CHECK_HIP(hipGetLastError());
   }
   return EXC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int exc_ralha_step(int i, int m, int c, double* d_fc, double* d_uc, const double* d_g, double* d_house,
+                   double* d_tau, double* d_alpha, double* d_beta, double* d_msg, void* d_work, size_t work_bytes,
+                   exc_stream_t stream) {
+  return ralha_step(false, i, m, c, d_fc, d_uc, d_g, d_house, d_tau, d_alpha, d_beta, d_msg, d_work, work_bytes,
+                    stream);
+}
+
+int exc_ralha_step_generic(int i, int m, int c, double* d_fc, double* d_uc, const double* d_g, double* d_house,
+                           double* d_tau, double* d_alpha, double* d_beta, double* d_msg, void* d_work,
+                           size_t work_bytes, exc_stream_t stream) {
+  return ralha_step(true, i, m, c, d_fc, d_uc, d_g, d_house, d_tau, d_alpha, d_beta, d_msg, d_work, work_bytes,
+                    stream);
 }
 
 int exc_ralha_dot(int i, int m, int c, const double* d_fc, const double* d_uc, double* d_out, exc_stream_t stream) {
@@ -332,7 +557,7 @@ int exc_sign_normalize(int m, int k, double* d_bu, double* d_bv, double* d_flip,
   EXC_REQUIRE(m >= 1 && k >= 0 && d_bu && d_bv && d_flip, EXC_ERR_ARG, "exc_sign_normalize: bad arguments");
   if (k == 0) return EXC_OK;
   hipStream_t s = exc_detail::stream(stream);
-  hipLaunchKernelGGL(k_sign_flags, dim3(exc_detail::ceil_div(k, 256)), dim3(256), 0, s, m, k, d_bu, d_flip);
+  hipLaunchKernelGGL(k_sign_flags, dim3(exc_detail::ceil_div(k, 32)), dim3(256), 0, s, m, k, d_bu, d_flip);
   hipLaunchKernelGGL(k_apply_flags, dim3(exc_detail::ceil_div((long)m * k, 256)), dim3(256), 0, s, m, k, d_bu, d_bv,
                      d_flip);
   EXC_CHECK_HIP(hipGetLastError());

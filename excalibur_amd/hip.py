@@ -69,6 +69,7 @@ SIGNATURES = {
     "exc_ormqr_house_ln": (_I, [_I, _L, _dp, _dp, _dp, _L, _dp, _S, _vp]),
     "exc_ralha_workspace": (_S, [_I, _I]),
     "exc_ralha_step": (_I, [_I, _I, _I, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _vp, _S, _vp]),
+    "exc_ralha_step_generic": (_I, [_I, _I, _I, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _vp, _S, _vp]),
     "exc_ralha_dot": (_I, [_I, _I, _I, _dp, _dp, _dp, _vp]),
     "exc_ralha_tail_update": (_I, [_I, _I, _I, _dp, _dp, _dp, _dp, _dp, _vp]),
     "exc_ralha_tail_scale": (_I, [_I, _I, _I, _dp, _dp, _dp, _dp, _vp]),
@@ -362,10 +363,11 @@ def ormqr_house_ln(house, tau, C):
     return dC.to_host()
 
 
-def ralha_peers(F_list):
+def ralha_peers(F_list, generic=False):
     """The overlapped Ralha loop over P simulated peers in this process: GPU kernels per peer,
     the all-reduce done here in each peer's own order (own buffer, then the others by id).
-    F_list[p] is peer p's m x c_p share of the rows f_j; U is returned in the same orientation."""
+    F_list[p] is peer p's m x c_p share of the rows f_j; U is returned in the same orientation.
+    generic selects the streamed (HBM) form of the step kernel instead of the register form."""
     P = len(F_list)
     m = F_list[0].shape[0]
     cs = [f.shape[1] for f in F_list]
@@ -397,7 +399,7 @@ def ralha_peers(F_list):
         call("exc_memcpy_h2d", dst.ptr, buf.ctypes.data, buf.nbytes, None)
 
     def step(i, p):
-        call("exc_ralha_step", i, m, cs[p], dF[p].ptr, dU[p].ptr, dg[p].ptr, dh[p].ptr, dt[p].ptr, da[p].ptr,
+        call("exc_ralha_step_generic" if generic else "exc_ralha_step", i, m, cs[p], dF[p].ptr, dU[p].ptr, dg[p].ptr, dh[p].ptr, dt[p].ptr, da[p].ptr,
              db[p].ptr, dmsg[p].ptr, ws[p].ptr, ws[p].nbytes, None)
 
     for p in range(P):

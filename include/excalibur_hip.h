@@ -191,6 +191,12 @@ size_t exc_ralha_workspace(int m, int c);
 int exc_ralha_step(int i, int m, int c, double* d_fc, double* d_uc, const double* d_g, double* d_house,
                    double* d_tau, double* d_alpha, double* d_beta, double* d_msg, void* d_work, size_t work_bytes,
                    exc_stream_t stream);
+/* The same step with the columns streamed from HBM instead of held in registers: exc_ralha_step
+ * uses it for m > 4000 and keeps every column in registers below that. Exported so that both
+ * forms can be checked against each other at any m. */
+int exc_ralha_step_generic(int i, int m, int c, double* d_fc, double* d_uc, const double* d_g, double* d_house,
+                           double* d_tau, double* d_alpha, double* d_beta, double* d_msg, void* d_work,
+                           size_t work_bytes, exc_stream_t stream);
 /* The last two columns and the first (client.cpp:830-833, 897-905): d_out = f_i.u_{i-1};
  * tail_update: f_i -= beta u_{i-1} (beta = reduced value, stored in d_beta[i-1]), d_out =
  * ||f_i||^2 (i = -1: ||f_0||^2 only); tail_scale: alpha_i = sqrt(reduced), u_i = f_i / alpha_i. */

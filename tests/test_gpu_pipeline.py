@@ -50,14 +50,17 @@ def test_global_qr_vs_oracle(hip, oracle, P, m, widths):
     assert np.max(np.abs(qgt.T - qg_o)) <= 1e-12
 
 
-@pytest.mark.parametrize("P,m,cols", [(1, 12, [12]), (2, 60, [30, 30]), (3, 101, [34, 34, 33])])
-def test_ralha_vs_oracle(hip, oracle, P, m, cols):
-    banner(f"ralha P={P} m={m}")
+@pytest.mark.parametrize("generic", [False, True])
+@pytest.mark.parametrize("P,m,cols", [(1, 12, [12]), (2, 60, [30, 30]), (3, 101, [34, 34, 33]),
+                                      (2, 300, [150, 150]), (2, 600, [301, 299]), (1, 1100, [1100])])
+def test_ralha_vs_oracle(hip, oracle, P, m, cols, generic):
+    """Both forms of the step kernel: registers (one size class per m range) and streamed."""
+    banner(f"ralha P={P} m={m} generic={generic}")
     rng = np.random.default_rng(m + P)
     # an R factor of Gaussian data, split by columns: the matrix the reference feeds to Ralha
     Rf = np.linalg.qr(rng.standard_normal((2 * m, m)))[1]
     F = [Rf[:, sum(cols[:p]):sum(cols[:p + 1])].copy() for p in range(P)]
-    got = hip.ralha_peers(F)
+    got = hip.ralha_peers(F, generic=generic)
     want = oracle.ralha(F)
     for p in range(P):
         for key in ("alpha", "beta", "tau", "U"):
