@@ -8,10 +8,13 @@
 #include "exc_common.hpp"
 #include "fact.hpp"
 #include "gemm.hpp"
+#include "solve.hpp"
 
 namespace {
 
 using fact::NB;
+using fact::left_solve;
+using fact::right_solve;
 
 // The 64 x 64 diagonal-block kernels are branch-free: a block smaller than 64 is padded with the
 // identity (unit diagonal, zero off-diagonal), which leaves the padded unknowns untouched, so
@@ -102,64 +105,6 @@ __global__ __launch_bounds__(64) void k_getf2_sign(int kb, double* __restrict__ 
 #pragma unroll
   for (int k = 0; k < NB; k++)
     if (lane < kb && k < kb) Ab[(long)lane * lda + k] = x[k];
-}
-
-// Substitution on one lane's column x (FULL: kb == NB, so every bound is a constant).
-template <bool UPPER, bool UNIT, bool FULL>
-__device__ __forceinline__ void left_solve(int kb, const double* __restrict__ L, long ldl, double (&x)[NB]) {
-#pragma unroll
-  for (int q = 0; q < NB; q++) {
-    const int i = UPPER ? NB - 1 - q : q;
-    if (FULL || i < kb) {
-      const double* Li = L + (long)i * ldl;
-      double a0 = x[i], a1 = 0.0, a2 = 0.0, a3 = 0.0;
-#pragma unroll
-      for (int k = UPPER ? i + 1 : 0; k < (UPPER ? NB : i); k++) {
-        if (FULL || k < kb) {
-          if ((k & 3) == 0) a0 = fma(-Li[k], x[k], a0);
-          else if ((k & 3) == 1) a1 = fma(-Li[k], x[k], a1);
-          else if ((k & 3) == 2) a2 = fma(-Li[k], x[k], a2);
-          else a3 = fma(-Li[k], x[k], a3);
-        }
-      }
-      const double acc = (a0 + a1) + (a2 + a3);
-      x[i] = UNIT ? acc : acc / Li[i];
-    }
-  }
-}
-
-// Substitution from the right on one lane's row x: x op(U) = b.
-template <bool LOWER_T, bool UNIT, bool FULL>
-__device__ __forceinline__ void right_solve(int kb, const double* __restrict__ U, long ldu, double (&x)[NB]) {
-  if (LOWER_T) {
-#pragma unroll
-    for (int j = 0; j < NB; j++) {
-      if (FULL || j < kb) {
-        const double* Lj = U + (long)j * ldu;  // op(U)[k][j] = L[j][k]: row j of L
-        double a0 = x[j], a1 = 0.0, a2 = 0.0, a3 = 0.0;
-#pragma unroll
-        for (int k = 0; k < j; k++) {
-          if ((k & 3) == 0) a0 = fma(-x[k], Lj[k], a0);
-          else if ((k & 3) == 1) a1 = fma(-x[k], Lj[k], a1);
-          else if ((k & 3) == 2) a2 = fma(-x[k], Lj[k], a2);
-          else a3 = fma(-x[k], Lj[k], a3);
-        }
-        const double acc = (a0 + a1) + (a2 + a3);
-        x[j] = UNIT ? acc : acc / Lj[j];
-      }
-    }
-  } else {
-#pragma unroll
-    for (int j = 0; j < NB; j++) {
-      if (FULL || j < kb) {
-        const double* Uj = U + (long)j * ldu;  // row j of U, right-looking
-        if (!UNIT) x[j] = x[j] / Uj[j];
-#pragma unroll
-        for (int k = j + 1; k < NB; k++)
-          if (FULL || k < kb) x[k] = fma(-x[j], Uj[k], x[k]);
-      }
-    }
-  }
 }
 
 // X_b <- op(L_bb)^-1 X_b for one diagonal block (kb <= NB), one lane per right-hand side, the
@@ -322,6 +267,7 @@ static int potrf_rec(int n, double* A, long lda, int k0, int* info, void* work, 
 int potrf_lower(int n, double* A, long lda, int* info, void* work, size_t wb, hipStream_t s) {
   EXC_REQUIRE(n >= 0 && A, EXC_ERR_ARG, "potrf_lower: bad arguments");
   if (n == 0) return EXC_OK;
+  if (n > NB && n <= kTileMaxN) return potrf_tiles(n, A, lda, info, work, wb, s);
   int rc = potrf_rec(n, A, lda, 0, info, work, wb, s);
   if (rc) return rc;
   hipLaunchKernelGGL(k_zero_strict_upper, dim3(exc_detail::ceil_div((long)n * n, 256)), dim3(256), 0, s, n, A, lda);
@@ -355,6 +301,7 @@ static int lu_rec(int n, double* A, long lda, int k0, int total, double* sgn, in
 int lu_sign(int n, double* A, long lda, double* sgn, int square_last, void* work, size_t wb, hipStream_t s) {
   EXC_REQUIRE(n >= 0 && A && sgn, EXC_ERR_ARG, "lu_sign: bad arguments");
   if (n == 0) return EXC_OK;
+  if (n > NB && n <= kTileMaxN) return lu_tiles(n, A, lda, sgn, square_last, work, wb, s);
   return lu_rec(n, A, lda, 0, n, sgn, square_last, work, wb, s);
 }
 

@@ -34,4 +34,11 @@ int lu_sign(int n, double* A, long lda, double* sgn, int square_last, void* work
 
 size_t work_bytes(int n, long wide);
 
+// One-launch tile-parallel forms (tiles.hip), used by potrf_lower and lu_sign for
+// 64 < n <= kTileMaxN. Both synchronize the stream to check their bounded waits.
+constexpr int kTileMaxN = 4096;
+size_t tiles_work_bytes(int n);
+int potrf_tiles(int n, double* A, long lda, int* info, void* work, size_t work_bytes, hipStream_t s);
+int lu_tiles(int n, double* A, long lda, double* sgn, int square_last, void* work, size_t work_bytes, hipStream_t s);
+
 }  // namespace fact

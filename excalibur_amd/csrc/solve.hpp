@@ -1,0 +1,69 @@
+// Substitution on one lane's register vector against a triangular block of at most 64 x 64
+// read with wave-uniform addresses (shared by the triangular-solve leaves and the tile
+// factorizations). FULL: the block is 64 wide, so every bound is a constant.
+#pragma once
+
+#include "exc_common.hpp"
+#include "fact.hpp"
+
+namespace fact {
+
+// Substitution on one lane's column x (FULL: kb == NB, so every bound is a constant).
+template <bool UPPER, bool UNIT, bool FULL>
+__device__ __forceinline__ void left_solve(int kb, const double* __restrict__ L, long ldl, double (&x)[NB]) {
+#pragma unroll
+  for (int q = 0; q < NB; q++) {
+    const int i = UPPER ? NB - 1 - q : q;
+    if (FULL || i < kb) {
+      const double* Li = L + (long)i * ldl;
+      double a0 = x[i], a1 = 0.0, a2 = 0.0, a3 = 0.0;
+#pragma unroll
+      for (int k = UPPER ? i + 1 : 0; k < (UPPER ? NB : i); k++) {
+        if (FULL || k < kb) {
+          if ((k & 3) == 0) a0 = fma(-Li[k], x[k], a0);
+          else if ((k & 3) == 1) a1 = fma(-Li[k], x[k], a1);
+          else if ((k & 3) == 2) a2 = fma(-Li[k], x[k], a2);
+          else a3 = fma(-Li[k], x[k], a3);
+        }
+      }
+      const double acc = (a0 + a1) + (a2 + a3);
+      x[i] = UNIT ? acc : acc / Li[i];
+    }
+  }
+}
+
+// Substitution from the right on one lane's row x: x op(U) = b.
+template <bool LOWER_T, bool UNIT, bool FULL>
+__device__ __forceinline__ void right_solve(int kb, const double* __restrict__ U, long ldu, double (&x)[NB]) {
+  if (LOWER_T) {
+#pragma unroll
+    for (int j = 0; j < NB; j++) {
+      if (FULL || j < kb) {
+        const double* Lj = U + (long)j * ldu;  // op(U)[k][j] = L[j][k]: row j of L
+        double a0 = x[j], a1 = 0.0, a2 = 0.0, a3 = 0.0;
+#pragma unroll
+        for (int k = 0; k < j; k++) {
+          if ((k & 3) == 0) a0 = fma(-x[k], Lj[k], a0);
+          else if ((k & 3) == 1) a1 = fma(-x[k], Lj[k], a1);
+          else if ((k & 3) == 2) a2 = fma(-x[k], Lj[k], a2);
+          else a3 = fma(-x[k], Lj[k], a3);
+        }
+        const double acc = (a0 + a1) + (a2 + a3);
+        x[j] = UNIT ? acc : acc / Lj[j];
+      }
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < NB; j++) {
+      if (FULL || j < kb) {
+        const double* Uj = U + (long)j * ldu;  // row j of U, right-looking
+        if (!UNIT) x[j] = x[j] / Uj[j];
+#pragma unroll
+        for (int k = j + 1; k < NB; k++)
+          if (FULL || k < kb) x[k] = fma(-x[j], Uj[k], x[k]);
+      }
+    }
+  }
+}
+
+}  // namespace fact

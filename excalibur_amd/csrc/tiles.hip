@@ -1,0 +1,445 @@
+// Tile-parallel Cholesky and sign-choosing LU (the factorizations of the Cholesky QR and of the
+// Householder reconstruction) in one launch each: one workgroup per 64 x 64 tile.
+//
+// A tile accumulates its updates with MFMA (v_mfma_f64_16x16x4f64) as soon as the tiles it needs
+// are final, then factors itself (diagonal tile) or solves against the diagonal tile of its
+// column or row, and publishes a per-tile flag. The recursive launch chain of the blocked
+// algorithm (about sixty dependent launches per factorization of order 1000) becomes one
+// launch whose critical path is the chain of diagonal tiles.
+//
+// Forward progress without co-residency: workgroups are numbered so that every tile depends only
+// on tiles with smaller numbers, and a compute unit's workgroups are dispatched in increasing
+// order, so the lowest unfinished workgroup is always resident and never waits. Every wait is
+// also bounded: after a long spin the kernel records a timeout and all workgroups exit.
+#include "exc_common.hpp"
+#include "fact.hpp"
+#include "solve.hpp"
+
+namespace {
+
+constexpr int TB = 64;    // tile edge
+constexpr int TNT = 64;   // threads per workgroup: one wavefront per tile
+constexpr int LD = TB + 1;
+constexpr long kSpinLimit = 1L << 22;
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+// Waits until *f != 0 (thread 0 spins with acquire loads; the barrier extends the acquire to the
+// workgroup). Returns false when the wait timed out or another workgroup reported an error.
+__device__ __forceinline__ bool wait_flag(int* f, int* error, int* sh) {
+  if (threadIdx.x == 0) {
+    int ok = 1;
+    long spins = 0;
+    while (__hip_atomic_load(f, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+      if (__hip_atomic_load(error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) { ok = 0; break; }
+      if (++spins > kSpinLimit) {
+        __hip_atomic_store(error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        ok = 0;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(2);
+    }
+    sh[0] = ok;
+  }
+  __syncthreads();
+  const bool ok = sh[0] != 0;
+  __syncthreads();  // every thread has read sh before the next wait can overwrite it
+  return ok;
+}
+
+// Publishes a tile: every thread's stores are made visible device-wide before the flag.
+__device__ __forceinline__ void signal_flag(int* f) {
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) __hip_atomic_store(f, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Accumulator layout: the wavefront holds the tile as 4 x 4 MFMA blocks of 16 x 16; element v of
+// lane l of block (a, b) is row 16a + (l / 16) + 4v, column 16b + l % 16.
+struct Quad {
+  int li, lk;
+  __device__ Quad() {
+    li = threadIdx.x & 15;
+    lk = threadIdx.x >> 4;
+  }
+  __device__ int row(int a, int v) const { return 16 * a + lk + 4 * v; }
+  __device__ int col(int b) const { return 16 * b + li; }
+};
+
+// acc <- the nr x nc tile of A at (r0, c0), zero outside.
+__device__ __forceinline__ void acc_load(d4 (&acc)[4][4], const Quad& q, const double* A, long lda, long r0, long c0, int nr,
+                         int nc) {
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int b = 0; b < 4; b++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int r = q.row(a, v), c = q.col(b);
+        acc[a][b][v] = (r < nr && c < nc) ? A[(r0 + r) * lda + c0 + c] : 0.0;
+      }
+}
+
+// S (LDS, 64 x 65) <- acc
+__device__ __forceinline__ void acc_to_lds(const d4 (&acc)[4][4], const Quad& q, double* S) {
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int b = 0; b < 4; b++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) S[q.row(a, v) * LD + q.col(b)] = acc[a][b][v];
+}
+
+// T (LDS, 64 x 65) <- the nr x nc tile of A at (r0, c0), zero outside.
+__device__ __forceinline__ void tile_to_lds(double* T, const double* A, long lda, long r0, long c0, int nr, int nc) {
+  for (int e = threadIdx.x; e < TB * TB; e += TNT) {
+    const int r = e >> 6, c = e & 63;
+    T[r * LD + c] = (r < nr && c < nc) ? A[(r0 + r) * lda + c0 + c] : 0.0;
+  }
+}
+
+// T (LDS) <- the transpose of the nr x nc tile of A at (r0, c0), zero outside.
+__device__ __forceinline__ void tile_to_lds_t(double* T, const double* A, long lda, long r0, long c0, int nr, int nc) {
+  for (int e = threadIdx.x; e < TB * TB; e += TNT) {
+    const int r = e >> 6, c = e & 63;
+    T[c * LD + r] = (r < nr && c < nc) ? A[(r0 + r) * lda + c0 + c] : 0.0;
+  }
+}
+
+// The nr x nc tile of A at (r0, c0) <- S (LDS); lower_zero writes zeros above the diagonal.
+__device__ __forceinline__ void lds_to_tile(const double* S, double* A, long lda, long r0, long c0, int nr, int nc, bool lower_zero) {
+  for (int e = threadIdx.x; e < TB * TB; e += TNT) {
+    const int r = e >> 6, c = e & 63;
+    if (r < nr && c < nc) A[(r0 + r) * lda + c0 + c] = (lower_zero && c > r) ? 0.0 : S[r * LD + c];
+  }
+}
+
+// acc -= X Y^T (TRANS_Y) or X Y, X and Y 64 x 64 in LDS.
+template <bool TRANS_Y>
+__device__ __forceinline__ void acc_update(d4 (&acc)[4][4], const Quad& q, const double* X, const double* Y) {
+#pragma unroll 2
+  for (int kk = 0; kk < TB; kk += 4) {
+    double xa[4], yb[4];
+#pragma unroll
+    for (int a = 0; a < 4; a++) xa[a] = -X[(16 * a + q.li) * LD + kk + q.lk];
+#pragma unroll
+    for (int b = 0; b < 4; b++)
+      yb[b] = TRANS_Y ? Y[(16 * b + q.li) * LD + kk + q.lk] : Y[(kk + q.lk) * LD + 16 * b + q.li];
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+      for (int b = 0; b < 4; b++) acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(xa[a], yb[b], acc[a][b], 0, 0, 0);
+  }
+}
+
+// 1 / sqrt(d) and sqrt(d) for d > 0: hardware reciprocal square root refined by two Newton steps.
+__device__ inline void rsqrt_pair(double d, double* r, double* s) {
+  double y = __builtin_amdgcn_rsq(d);
+  y = y * (1.5 - 0.5 * d * y * y);
+  y = y * (1.5 - 0.5 * d * y * y);
+  *r = y;
+  *s = d * y;
+}
+
+// In-tile factorizations and solves run with one row (or column) of the tile per lane in
+// registers; the pivot column, pivot row or reciprocals go through LDS between steps.
+// The loops are unrolled so every register index is a constant; wave_sync orders the LDS traffic
+// of the one wavefront (and keeps the compiler from moving LDS reads across steps).
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Cholesky of the diagonal tile in S (padded rows/cols identity), lane i = row i. Returns the
+// first non-positive pivot (1-based) or 0; such a pivot is replaced by 1 like the blocked path.
+__device__ __forceinline__ int potf2_reg(double* S, double* xc, int kb) {
+  const int lane = threadIdx.x;
+  __asm__ volatile("" ::: "memory");  // keep this routine's loads inside its branch
+  double x[TB];
+#pragma unroll
+  for (int k = 0; k < TB; k++) x[k] = k <= lane ? S[lane * LD + k] : 0.0;
+  int bad = 0;
+#pragma unroll
+  for (int j = 0; j < TB; j++) {
+    double* c = xc + (j & 1) * TB;
+    c[lane] = x[j];
+    wave_sync();
+    double d = c[j];
+    if (!(d > 0.0)) {
+      if (!bad && j < kb) bad = j + 1;
+      d = 1.0;
+    }
+    double rp, piv;
+    rsqrt_pair(d, &rp, &piv);
+    const double l = lane > j ? x[j] * rp : 0.0;
+    x[j] = lane == j ? piv : (lane > j ? l : x[j]);
+    wave_sync();
+    c[lane] = l;
+    wave_sync();
+#pragma unroll
+    for (int k = j + 1; k < TB; k++) x[k] = fma(-l, c[k], x[k]);
+  }
+#pragma unroll
+  for (int k = 0; k < TB; k++) S[lane * LD + k] = k <= lane ? x[k] : 0.0;
+  return bad;
+}
+
+// LU without pivoting of S - diag(s) (sign rule of the Householder reconstruction), lane i = row
+// i; row j is published through LDS at step j. sgn receives s for the kb valid pivots.
+__device__ __forceinline__ void getf2_reg(double* S, double* xr, int kb, double* sgn, bool last_plus) {
+  const int lane = threadIdx.x;
+  __asm__ volatile("" ::: "memory");  // keep this routine's loads inside its branch
+  double x[TB];
+#pragma unroll
+  for (int k = 0; k < TB; k++) x[k] = S[lane * LD + k];
+#pragma unroll
+  for (int j = 0; j < TB; j++) {
+    double* r = xr + (j & 1) * (TB + 1);  // row j (r[k], k > j) and the pivot (r[TB])
+    if (lane == j) {
+      const double w = x[j];
+      double sj = signbit(w) ? 1.0 : -1.0;
+      if (last_plus && j == kb - 1) sj = -sj;
+      x[j] = w - sj;
+      r[TB] = x[j];
+      if (j < kb) sgn[j] = sj;
+#pragma unroll
+      for (int k = j + 1; k < TB; k++) r[k] = x[k];
+    }
+    wave_sync();
+    const double rp = 1.0 / r[TB];
+    const double l = lane > j ? x[j] * rp : 0.0;
+    if (lane > j) x[j] = l;
+#pragma unroll
+    for (int k = j + 1; k < TB; k++) x[k] = fma(-l, r[k], x[k]);
+  }
+#pragma unroll
+  for (int k = 0; k < TB; k++) S[lane * LD + k] = x[k];
+}
+
+// Off-diagonal tile solves against the final diagonal tile, read from HBM with wave-uniform
+// addresses (the flag's acquire made it visible): ROWS: lane = row r of S, S[r] <- S[r] op(D)^-1
+// (LOWER_T: op(D) = L^T, Cholesky; else U, LU); else lane = column c, S[:, c] <- L^-1 S[:, c]
+// with L the unit lower part of D (LU).
+template <int MODE>  // 0: rows, L^T; 1: rows, U; 2: columns, unit L
+__device__ __forceinline__ void tile_solve(double* S, const double* Dg, long lda, int kd) {
+  __asm__ volatile("" ::: "memory");
+  const int lane = threadIdx.x;
+  double x[TB];
+#pragma unroll
+  for (int k = 0; k < TB; k++) x[k] = MODE == 2 ? S[k * LD + lane] : S[lane * LD + k];
+  if (MODE == 0) {
+    if (kd == TB) fact::right_solve<true, false, true>(kd, Dg, lda, x);
+    else fact::right_solve<true, false, false>(kd, Dg, lda, x);
+  } else if (MODE == 1) {
+    if (kd == TB) fact::right_solve<false, false, true>(kd, Dg, lda, x);
+    else fact::right_solve<false, false, false>(kd, Dg, lda, x);
+  } else {
+    if (kd == TB) fact::left_solve<false, true, true>(kd, Dg, lda, x);
+    else fact::left_solve<false, true, false>(kd, Dg, lda, x);
+  }
+#pragma unroll
+  for (int k = 0; k < TB; k++) {
+    if (MODE == 2) S[k * LD + lane] = x[k];
+    else S[lane * LD + k] = x[k];
+  }
+}
+
+// Identity padding of a diagonal tile with kb < 64 valid rows/cols.
+__device__ __forceinline__ void pad_identity(double* S, int kb) {
+  for (int e = threadIdx.x; e < TB * TB; e += TNT) {
+    const int r = e >> 6, c = e & 63;
+    if (r >= kb || c >= kb) S[r * LD + c] = (r == c) ? 1.0 : 0.0;
+  }
+}
+
+// Tile numbering of the lower triangle, column by column: t -> (i, j), i >= j.
+__device__ void chol_tile(int t, int nb, int* i, int* j) {
+  int col = 0, rem = t;
+  while (rem >= nb - col) {
+    rem -= nb - col;
+    col++;
+  }
+  *j = col;
+  *i = col + rem;
+}
+__host__ __device__ inline int chol_index(int i, int j, int nb) { return j * nb - j * (j - 1) / 2 + (i - j); }
+
+__global__ __launch_bounds__(TNT) void k_potrf_tiles(int n, double* __restrict__ A, long lda, int nb,
+                                                     int* __restrict__ done, int* __restrict__ error) {
+  extern __shared__ double smem[];  // S, X, Y: three 64 x 65 tiles
+  double* S = smem;
+  double* X = smem + TB * LD;
+  double* Y = smem + 2 * TB * LD;
+  __shared__ int sh[1];
+  int ti, tj;
+  chol_tile(blockIdx.x, nb, &ti, &tj);
+  const long r0 = (long)ti * TB, c0 = (long)tj * TB;
+  const int nr = min(TB, n - (int)r0), nc = min(TB, n - (int)c0);
+  const Quad q;
+  d4 acc[4][4];
+  acc_load(acc, q, A, lda, r0, c0, nr, nc);
+  for (int k = 0; k < tj; k++) {
+    if (!wait_flag(&done[chol_index(ti, k, nb)], error, sh)) return;
+    if (ti != tj && !wait_flag(&done[chol_index(tj, k, nb)], error, sh)) return;
+    const int kc = min(TB, n - k * TB);
+    tile_to_lds(X, A, lda, r0, (long)k * TB, nr, kc);
+    if (ti != tj) tile_to_lds(Y, A, lda, c0, (long)k * TB, nc, kc);
+    __syncthreads();
+    acc_update<true>(acc, q, X, ti != tj ? Y : X);
+    __syncthreads();
+  }
+  acc_to_lds(acc, q, S);
+  __syncthreads();
+  if (ti == tj) {
+    pad_identity(S, nr);
+    __syncthreads();
+    const int bad = potf2_reg(S, X, nr);
+    if (threadIdx.x == 0 && bad) atomicCAS(&error[1], 0, (int)r0 + bad);
+    __syncthreads();
+    lds_to_tile(S, A, lda, r0, c0, nr, nc, true);
+  } else {
+    if (!wait_flag(&done[chol_index(tj, tj, nb)], error, sh)) return;
+    tile_solve<0>(S, A + c0 * lda + c0, lda, nc);  // L(i, j) = A(i, j) L(j, j)^-T
+    __syncthreads();
+    lds_to_tile(S, A, lda, r0, c0, nr, nc, false);
+  }
+  signal_flag(&done[blockIdx.x]);
+}
+
+// LU tiles in order of d = min(i, j): (d, d), then (d, d+1), (d+1, d), (d, d+2), (d+2, d), ...
+__device__ void lu_tile(int t, int nb, int* i, int* j) {
+  int d = 0, rem = t;
+  while (rem >= 2 * (nb - d) - 1) {
+    rem -= 2 * (nb - d) - 1;
+    d++;
+  }
+  if (rem == 0) { *i = d; *j = d; return; }
+  const int off = (rem + 1) / 2;
+  if (rem & 1) { *i = d; *j = d + off; }  // U tile in row d
+  else { *i = d + off; *j = d; }          // L tile in column d
+}
+__host__ __device__ inline int lu_index(int i, int j, int nb) {
+  const int d = i < j ? i : j;
+  const int before = d * (2 * nb - d);  // sum over e < d of 2 (nb - e) - 1
+  if (i == j) return before;
+  const int off = (i < j) ? (j - d) : (i - d);
+  return before + 2 * off - ((i < j) ? 1 : 0);
+}
+
+__global__ __launch_bounds__(TNT) void k_lu_tiles(int n, double* __restrict__ A, long lda, int nb,
+                                                  double* __restrict__ sgn, int square_last,
+                                                  int* __restrict__ done, int* __restrict__ error) {
+  extern __shared__ double smem[];  // S, X, Y: three 64 x 65 tiles
+  double* S = smem;
+  double* X = smem + TB * LD;
+  double* Y = smem + 2 * TB * LD;
+  __shared__ int sh[1];
+  int ti, tj;
+  lu_tile(blockIdx.x, nb, &ti, &tj);
+  const long r0 = (long)ti * TB, c0 = (long)tj * TB;
+  const int nr = min(TB, n - (int)r0), nc = min(TB, n - (int)c0);
+  const int d = min(ti, tj);
+  const Quad q;
+  d4 acc[4][4];
+  acc_load(acc, q, A, lda, r0, c0, nr, nc);
+  for (int k = 0; k < d; k++) {
+    if (!wait_flag(&done[lu_index(ti, k, nb)], error, sh)) return;
+    if (!wait_flag(&done[lu_index(k, tj, nb)], error, sh)) return;
+    const int kc = min(TB, n - k * TB);
+    tile_to_lds(X, A, lda, r0, (long)k * TB, nr, kc);  // L(i, k)
+    tile_to_lds(Y, A, lda, (long)k * TB, c0, kc, nc);  // U(k, j)
+    __syncthreads();
+    acc_update<false>(acc, q, X, Y);
+    __syncthreads();
+  }
+  acc_to_lds(acc, q, S);
+  __syncthreads();
+  if (ti == tj) {
+    pad_identity(S, nr);
+    __syncthreads();
+    getf2_reg(S, X, nr, sgn + r0, square_last && ti == nb - 1);
+    __syncthreads();
+    lds_to_tile(S, A, lda, r0, c0, nr, nc, false);
+  } else {
+    if (!wait_flag(&done[lu_index(d, d, nb)], error, sh)) return;
+    const long dd = (long)d * TB;
+    const int kd = min(TB, n - (int)dd);
+    if (ti > tj) tile_solve<1>(S, A + dd * lda + dd, lda, kd);  // L(i, j) = A(i, j) U(j, j)^-1
+    else tile_solve<2>(S, A + dd * lda + dd, lda, kd);          // U(i, j) = L(i, i)^-1 A(i, j)
+    __syncthreads();
+    lds_to_tile(S, A, lda, r0, c0, nr, nc, false);
+  }
+  signal_flag(&done[blockIdx.x]);
+}
+
+__global__ void k_zero_ints(int* p, int count) {
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < count; e += gridDim.x * blockDim.x) p[e] = 0;
+}
+
+__global__ __launch_bounds__(256) void k_zero_upper_blocks(int n, double* A, long lda) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long)n * n) return;
+  const long i = e / n, j = e - i * n;
+  if (j > i) A[i * lda + j] = 0.0;
+}
+
+constexpr size_t kTileLds = 3 * TB * LD * sizeof(double);
+
+}  // namespace
+
+namespace fact {
+
+size_t tiles_work_bytes(int n) {
+  const int nb = (n + TB - 1) / TB;
+  return (size_t)(nb * nb + 8) * sizeof(int);
+}
+
+int potrf_tiles(int n, double* A, long lda, int* info, void* work, size_t wb, hipStream_t s) {
+  const int nb = (n + TB - 1) / TB;
+  const int T = nb * (nb + 1) / 2;
+  EXC_REQUIRE(wb >= tiles_work_bytes(n), EXC_ERR_SPACE, "potrf: workspace too small");
+  int* done = (int*)work;
+  int* err = done + nb * nb;
+  static bool attr = false;
+  if (!attr) {
+    EXC_CHECK_HIP(hipFuncSetAttribute((const void*)k_potrf_tiles, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)kTileLds));
+    attr = true;
+  }
+  hipLaunchKernelGGL(k_zero_ints, dim3(exc_detail::ceil_div(nb * nb + 8, 256)), dim3(256), 0, s, done, nb * nb + 8);
+  hipLaunchKernelGGL(k_potrf_tiles, dim3(T), dim3(TNT), kTileLds, s, n, A, lda, nb, done, err);
+  hipLaunchKernelGGL(k_zero_upper_blocks, dim3(exc_detail::ceil_div((long)n * n, 256)), dim3(256), 0, s, n, A, lda);
+  EXC_CHECK_HIP(hipGetLastError());
+  int herr[2] = {0, 0};
+  EXC_CHECK_HIP(hipMemcpyAsync(herr, err, sizeof herr, hipMemcpyDeviceToHost, s));
+  EXC_CHECK_HIP(hipStreamSynchronize(s));
+  EXC_REQUIRE(herr[0] == 0, EXC_ERR_NUMERIC, "potrf: a tile wait timed out");
+  if (info && herr[1]) EXC_CHECK_HIP(hipMemcpyAsync(info, &herr[1], sizeof(int), hipMemcpyHostToDevice, s));
+  EXC_CHECK_HIP(hipStreamSynchronize(s));
+  return EXC_OK;
+}
+
+int lu_tiles(int n, double* A, long lda, double* sgn, int square_last, void* work, size_t wb, hipStream_t s) {
+  const int nb = (n + TB - 1) / TB;
+  const int T = nb * nb;
+  EXC_REQUIRE(wb >= tiles_work_bytes(n), EXC_ERR_SPACE, "lu_sign: workspace too small");
+  int* done = (int*)work;
+  int* err = done + nb * nb;
+  static bool attr = false;
+  if (!attr) {
+    EXC_CHECK_HIP(hipFuncSetAttribute((const void*)k_lu_tiles, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)kTileLds));
+    attr = true;
+  }
+  hipLaunchKernelGGL(k_zero_ints, dim3(exc_detail::ceil_div(nb * nb + 8, 256)), dim3(256), 0, s, done, nb * nb + 8);
+  hipLaunchKernelGGL(k_lu_tiles, dim3(T), dim3(TNT), kTileLds, s, n, A, lda, nb, sgn, square_last, done, err);
+  EXC_CHECK_HIP(hipGetLastError());
+  int herr = 0;
+  EXC_CHECK_HIP(hipMemcpyAsync(&herr, err, sizeof herr, hipMemcpyDeviceToHost, s));
+  EXC_CHECK_HIP(hipStreamSynchronize(s));
+  EXC_REQUIRE(herr == 0, EXC_ERR_NUMERIC, "lu_sign: a tile wait timed out");
+  return EXC_OK;
+}
+
+}  // namespace fact

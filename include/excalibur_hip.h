@@ -128,8 +128,10 @@ int exc_sgemm(int trans_a, int trans_b, long M, long N, long K, double alpha, co
               const float* d_b, long ldb, double beta, float* d_c, long ldc, void* d_work, size_t work_bytes,
               exc_stream_t stream);
 
-/* Blocked factorizations (FP64, row-major, in place). The workspace (exc_fact_workspace) holds
- * split-K partials of the GEMM updates. */
+/* Dense factorizations (FP64, row-major, in place). The workspace (exc_fact_workspace) holds
+ * split-K partials of the GEMM updates and the per-tile flags of the one-launch tile forms.
+ * exc_potrf_lower and exc_lu_sign synchronize the stream for 64 < n <= 4096 (the tile form
+ * checks that its bounded waits completed). */
 size_t exc_fact_workspace(int n, long wide);
 /* Lower Cholesky of a symmetric n x n matrix (lower triangle read, strict upper zeroed);
  * *d_info = 0 or the 1-based index of the first non-positive pivot. */
