@@ -6,7 +6,9 @@
 #include <arpa/inet.h>
 #include <netinet/in.h>
 #include <netinet/tcp.h>
+#include <poll.h>
 #include <sys/socket.h>
+#include <sys/uio.h>
 #include <unistd.h>
 
 #include <chrono>
@@ -22,6 +24,7 @@ double now_seconds() {
 namespace {
 
 constexpr size_t kMaxChunk = size_t(1) << 30;  // client.cpp:13
+constexpr double kHotSeconds = 2e-3;            // spin this long after a frame before blocking
 
 void write_all(int fd, const void* p, size_t n) {
   const char* c = (const char*)p;
@@ -47,6 +50,54 @@ bool read_all(int fd, void* p, size_t n) {
     }
     c += k;
     n -= (size_t)k;
+  }
+  return true;
+}
+
+// One frame (8-byte length + payload) in as few sendmsg calls as the socket takes, so the
+// receiver wakes once per frame instead of once for the length and again for the payload.
+void write_frame(int fd, const void* p, size_t n) {
+  uint64_t size = n;
+  iovec iov[2] = {{&size, sizeof size}, {const_cast<void*>(p), n}};
+  int first = 0, cnt = n ? 2 : 1;
+  while (first < cnt) {
+    msghdr mh{};
+    mh.msg_iov = iov + first;
+    mh.msg_iovlen = cnt - first;
+    ssize_t k = ::sendmsg(fd, &mh, MSG_NOSIGNAL);
+    if (k < 0) {
+      if (errno == EINTR) continue;
+      throw std::runtime_error(std::string("send failed: ") + strerror(errno));
+    }
+    size_t left = (size_t)k;
+    while (first < cnt && left >= iov[first].iov_len) left -= iov[first++].iov_len;
+    if (first < cnt) {
+      iov[first].iov_base = (char*)iov[first].iov_base + left;
+      iov[first].iov_len -= left;
+    }
+  }
+}
+
+// Reads exactly n bytes. While `hot_until` lies ahead the socket is polled without sleeping
+// (frames of the per-column all-reduce follow each other within tens of microseconds, and a
+// thread woken from a blocking recv adds several microseconds to each); after it, the thread
+// blocks in poll() with a 100 ms period so that `stop` is noticed.
+bool read_spin(int fd, void* p, size_t n, double hot_until, const std::atomic<bool>& stop) {
+  char* c = (char*)p;
+  while (n > 0) {
+    ssize_t k = ::recv(fd, c, std::min(n, kMaxChunk), MSG_DONTWAIT);
+    if (k > 0) {
+      c += k;
+      n -= (size_t)k;
+      continue;
+    }
+    if (k == 0) return false;
+    if (errno == EINTR) continue;
+    if (errno != EAGAIN && errno != EWOULDBLOCK) return false;
+    if (stop.load(std::memory_order_relaxed)) return false;
+    if (now_seconds() < hot_until) continue;
+    pollfd pf{fd, POLLIN, 0};
+    ::poll(&pf, 1, 100);
   }
   return true;
 }
@@ -120,11 +171,13 @@ void PeerNet::accept_loop() {
 
 void PeerNet::reader_loop(int fd, int from) {
   Inbox& box = *inbox_[from];
+  double hot_until = 0.0;
   while (!stop_) {
     uint64_t size = 0;
-    if (!read_all(fd, &size, sizeof size)) break;
+    if (!read_spin(fd, &size, sizeof size, hot_until, stop_)) break;
     std::vector<char> msg(size);
-    if (size && !read_all(fd, msg.data(), size)) break;
+    if (size && !read_spin(fd, msg.data(), size, now_seconds() + 1.0, stop_)) break;
+    hot_until = now_seconds() + kHotSeconds;
     {
       std::lock_guard<std::mutex> g(box.mu);
       box.q.push_back(std::move(msg));
@@ -177,9 +230,7 @@ void PeerNet::connect_all(double timeout_s) {
 }
 
 void PeerNet::send_to(int peer, const void* buf, size_t bytes) {
-  uint64_t size = bytes;
-  write_all(out_fd_[peer], &size, sizeof size);
-  if (bytes) write_all(out_fd_[peer], buf, bytes);
+  write_frame(out_fd_[peer], buf, bytes);
 }
 
 std::vector<char> PeerNet::receive(int peer) {
