@@ -91,7 +91,10 @@ struct ClientDevice {
   DBuf masked, P, Vt, tau_g, Qg, F, U, alpha, beta, house, tau_h, cs, bu, bv, sigma, flip, part, FinalU, tmp,
       FinalVT, msg, g, scal, work, plan, rt, msgd, b2, cnt2;
   HBuf h_masked, h_repart, h_msg, h_U;
+  void* link = nullptr;  // message link of the one-launch Ralha loop (coherent pinned memory)
+  int link_m = 0;
   ~ClientDevice() {
+    if (link) exc_ralha_link_free(link);
     if (s) hipStreamDestroy(s);
   }
 };
@@ -438,21 +441,51 @@ void DecentralizedClient::run() {
     ck(exc_ralha_tail_update(-1, mi, ci, D.F.p, D.U.p, nullptr, D.beta.p, D.scal.p, s), "ralha init");
     allreduce_dev(D.scal.p, D.scal.p + 1, 1);
     ck(exc_ralha_tail_scale(0, mi, ci, D.F.p, D.U.p, D.scal.p + 1, D.alpha.p, s), "ralha init");
-    const size_t rwb = exc_ralha_workspace(mi, ci);
-    D.work.alloc((rwb + 7) / 8, s);
-    // the step kernels write the local message straight into pinned host memory
-    ck(exc_ralha_step(-1, mi, ci, D.F.p, D.U.p, nullptr, D.house.p, D.tau_h.p, D.alpha.p, D.beta.p, hb.p, D.work.p,
-                      D.work.n * 8, s),
-       "ralha step");
-    for (int i = 0; i + 2 < mi; i++) {
-      sync();
-      const double ta = now_seconds();
-      net.ring_all_reduce_sum(hb.p, (long)m + 3 - i);
-      ar += now_seconds() - ta;
-      ckh(hipMemcpyAsync(D.g.p, hb.p, ((long)m + 3 - i) * sizeof(double), hipMemcpyHostToDevice, s), "h2d");
-      ck(exc_ralha_step(i, mi, ci, D.F.p, D.U.p, D.g.p, D.house.p, D.tau_h.p, D.alpha.p, D.beta.p, hb.p, D.work.p,
-                        D.work.n * 8, s),
+    if (opt_.ralha_one_launch && exc_ralha_loop_fits(mi, ci, 0) == 1) {
+      // steps -1 .. m-3 in one launch; this thread only all-reduces the messages it publishes
+      const double tmo = std::max(30.0, opt_.recv_timeout_s);
+      D.work.alloc((exc_ralha_loop_workspace(mi, ci) + 7) / 8, s);
+      if (!D.link || D.link_m != mi) {
+        if (D.link) ck(exc_ralha_link_free(D.link), "ralha link");
+        D.link = nullptr;
+        ck(exc_ralha_link_alloc(mi, &D.link), "ralha link");
+        D.link_m = mi;
+      }
+      ck(exc_ralha_loop_start(mi, ci, D.F.p, D.U.p, D.house.p, D.tau_h.p, D.alpha.p, D.beta.p, D.link, D.work.p,
+                              D.work.n * 8, 0, tmo, s),
+         "ralha loop");
+      try {
+        for (int j = 0; j + 2 < mi; j++) {
+          double* msg = nullptr;
+          ck(exc_ralha_loop_wait(D.link, j, tmo, &msg), "ralha loop");
+          const double ta = now_seconds();
+          net.ring_all_reduce_sum(msg, (long)m + 3 - j);
+          ar += now_seconds() - ta;
+          ck(exc_ralha_loop_post(D.link, j), "ralha loop");
+        }
+      } catch (...) {
+        exc_ralha_loop_abort(D.link);
+        exc_ralha_loop_finish(D.link, s);
+        throw;
+      }
+      ck(exc_ralha_loop_finish(D.link, s), "ralha loop");
+    } else {
+      const size_t rwb = exc_ralha_workspace(mi, ci);
+      D.work.alloc((rwb + 7) / 8, s);
+      // the step kernels write the local message straight into pinned host memory
+      ck(exc_ralha_step(-1, mi, ci, D.F.p, D.U.p, nullptr, D.house.p, D.tau_h.p, D.alpha.p, D.beta.p, hb.p,
+                        D.work.p, D.work.n * 8, s),
          "ralha step");
+      for (int i = 0; i + 2 < mi; i++) {
+        sync();
+        const double ta = now_seconds();
+        net.ring_all_reduce_sum(hb.p, (long)m + 3 - i);
+        ar += now_seconds() - ta;
+        ckh(hipMemcpyAsync(D.g.p, hb.p, ((long)m + 3 - i) * sizeof(double), hipMemcpyHostToDevice, s), "h2d");
+        ck(exc_ralha_step(i, mi, ci, D.F.p, D.U.p, D.g.p, D.house.p, D.tau_h.p, D.alpha.p, D.beta.p, hb.p,
+                          D.work.p, D.work.n * 8, s),
+           "ralha step");
+      }
     }
     for (int i = mi - 2; i < mi; i++) {
       ck(exc_ralha_dot(i, mi, ci, D.F.p, D.U.p, D.scal.p, s), "ralha dot");

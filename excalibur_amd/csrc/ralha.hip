@@ -7,7 +7,10 @@
 // reflector from the reduced message in a fixed order, so all workgroups see the same h and tau)
 // and then writes its partial sums of step i+1's message; a second kernel adds the partials in
 // workgroup order. The host all-reduces that message over the peers between steps.
+#include <atomic>
 #include <cfloat>
+#include <chrono>
+#include <cstring>
 
 #include "exc_common.hpp"
 
@@ -94,105 +97,126 @@ __device__ double ralha_reflector(int i, int m, const double* __restrict__ g, do
   return t;
 }
 
-// Register-resident step for m <= 64 * KM: each wavefront holds CPW whole columns of F (row
-// lane + 64q in slot q), so a column is read once, updated, stored once, and its share of the next
-// message is formed from the registers; the four wavefronts' shares are added through LDS.
-// Same operations as k_ralha_step below, with the dot products split by row slot.
+// Register-resident columns for m <= 64 * KM: each wavefront holds CPW whole columns of F (row
+// lane + 64q in slot q), with u_{i-1} and u_i of each column.
 template <int KM, int CPW>
-__global__ __launch_bounds__(256) void k_ralha_step_reg(int i, int m, int c, double* __restrict__ Fc,
-                                                        double* __restrict__ Uc, const double* __restrict__ g,
-                                                        double* __restrict__ house, double* __restrict__ tau,
-                                                        double* __restrict__ alpha, double* __restrict__ beta,
-                                                        double* __restrict__ partial) {
-  extern __shared__ double lds[];  // hbuf: m, then mp: 4 x (m + 4)
-  double* hbuf = lds;
-  double* mp = lds + m;
-  __shared__ double red[8];
-  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  const int ldm = m + 4;
+struct RegCols {
   double fr[CPW][KM];
-  double um1[CPW];  // u_{i-1} of each column (i > 0)
+  double um1[CPW];  // u_{i-1} (i > 0)
+  double ui[CPW];   // u_i (i >= 0), the u_{j-1} of the next message; the same on every lane
   int cols[CPW];
+};
+
+// Loads rows >= max(i, 0) of this wavefront's columns (zeros elsewhere) and u_{i-1}.
+template <int KM, int CPW>
+__device__ __forceinline__ void reg_load(RegCols<KM, CPW>& R, int i, int m, int c, const double* __restrict__ Fc,
+                                         const double* __restrict__ Uc) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
 #pragma unroll
   for (int cc = 0; cc < CPW; cc++) {
     const int col = (blockIdx.x * 4 + wave) * CPW + cc;
-    cols[cc] = col;
-    um1[cc] = 0.0;
+    R.cols[cc] = col;
+    R.um1[cc] = 0.0;
+    R.ui[cc] = 0.0;
 #pragma unroll
     for (int q = 0; q < KM; q++) {
       const int row = lane + 64 * q;
-      fr[cc][q] = (col < c && row < m && row >= (i < 0 ? 0 : i)) ? Fc[(long)col * m + row] : 0.0;
+      R.fr[cc][q] = (col < c && row < m && row >= (i < 0 ? 0 : i)) ? Fc[(long)col * m + row] : 0.0;
     }
-    if (col < c && i > 0) um1[cc] = Uc[(long)col * m + i - 1];
+    if (col < c && i > 0) R.um1[cc] = Uc[(long)col * m + i - 1];
   }
-  double ui[CPW];  // u_i (i >= 0), the u_{j-1} of the next message
+}
+
+// Step i >= 0 on the registers (client.cpp:858-891): the reflector from the reduced message g,
+// F[i+1:, col] -= tau h (h . F[i+1:, col]), then f_i -= beta u_{i-1} and u_i = f_i / alpha_i
+// (stored to U); for i = 0, u_0 comes from the tail kernels. Leaves u_i in R.ui on every lane.
+template <int KM, int CPW>
+__device__ __forceinline__ void reg_apply(RegCols<KM, CPW>& R, int i, int m, int c, const double* __restrict__ g,
+                                          double* hbuf, double* red, double* __restrict__ Uc,
+                                          double* __restrict__ house, double* __restrict__ tau,
+                                          double* __restrict__ alpha, double* __restrict__ beta) {
+  const int lane = threadIdx.x & 63;
+  double b, a;
+  const double t = ralha_reflector(i, m, g, hbuf, red, house, tau, alpha, beta, &b, &a);
 #pragma unroll
-  for (int cc = 0; cc < CPW; cc++) ui[cc] = 0.0;
-  if (i >= 0) {
-    double b, a;
-    const double t = ralha_reflector(i, m, g, hbuf, red, house, tau, alpha, beta, &b, &a);
-#pragma unroll
-    for (int cc = 0; cc < CPW; cc++) {
-      if (cols[cc] >= c) continue;  // uniform within the wavefront
-      if (t != 0.0) {
-        double s = 0.0;
-#pragma unroll
-        for (int q = 0; q < KM; q++) {
-          const int row = lane + 64 * q;
-          if (row > i && row < m) s += fr[cc][q] * hbuf[row - i - 1];
-        }
-        s = wave_sum(s);
-#pragma unroll
-        for (int q = 0; q < KM; q++) {
-          const int row = lane + 64 * q;
-          if (row > i && row < m) fr[cc][q] += s * (-t * hbuf[row - i - 1]);
-        }
-      }
-      if (i > 0) {
-        // f_i -= beta u_{i-1}; u_i = f_i / alpha_i (row i sits in lane i % 64, slot i / 64)
-#pragma unroll
-        for (int q = 0; q < KM; q++) {
-          if (lane + 64 * q == i) {
-            const double fi = fr[cc][q] + (-b) * um1[cc];
-            fr[cc][q] = fi;
-            ui[cc] = (1.0 / a) * fi;
-            Uc[(long)cols[cc] * m + i] = ui[cc];
-          }
-        }
-      } else {
-        // u_0 was set by the tail kernels before the loop
-#pragma unroll
-        for (int q = 0; q < KM; q++)
-          if (lane + 64 * q == 0) ui[cc] = Uc[(long)cols[cc] * m];
-      }
-      ui[cc] = __shfl(ui[cc], i & 63, kWave);
+  for (int cc = 0; cc < CPW; cc++) {
+    if (R.cols[cc] >= c) continue;  // uniform within the wavefront
+    if (t != 0.0) {
+      double s = 0.0;
 #pragma unroll
       for (int q = 0; q < KM; q++) {
         const int row = lane + 64 * q;
-        if (row >= i && row < m) Fc[(long)cols[cc] * m + row] = fr[cc][q];
+        if (row > i && row < m) s += R.fr[cc][q] * hbuf[row - i - 1];
+      }
+      s = wave_sum(s);
+#pragma unroll
+      for (int q = 0; q < KM; q++) {
+        const int row = lane + 64 * q;
+        if (row > i && row < m) R.fr[cc][q] += s * (-t * hbuf[row - i - 1]);
       }
     }
+    if (i > 0) {
+      // row i sits in lane i % 64, slot i / 64
+#pragma unroll
+      for (int q = 0; q < KM; q++) {
+        if (lane + 64 * q == i) {
+          const double fi = R.fr[cc][q] + (-b) * R.um1[cc];
+          R.fr[cc][q] = fi;
+          R.ui[cc] = (1.0 / a) * fi;
+          Uc[(long)R.cols[cc] * m + i] = R.ui[cc];
+        }
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < KM; q++)
+        if (lane + 64 * q == 0) R.ui[cc] = Uc[(long)R.cols[cc] * m];
+    }
+    R.ui[cc] = __shfl(R.ui[cc], i & 63, kWave);
   }
-  const int j = i + 1;
-  if (j > m - 3) return;  // uniform: no message after the last full step
+}
+
+// Stores rows >= rowmin of this wavefront's columns.
+template <int KM, int CPW>
+__device__ __forceinline__ void reg_store(const RegCols<KM, CPW>& R, int rowmin, int m, int c,
+                                          double* __restrict__ Fc) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int cc = 0; cc < CPW; cc++) {
+    if (R.cols[cc] >= c) continue;
+#pragma unroll
+    for (int q = 0; q < KM; q++) {
+      const int row = lane + 64 * q;
+      if (row >= rowmin && row < m) Fc[(long)R.cols[cc] * m + row] = R.fr[cc][q];
+    }
+  }
+}
+
+// This workgroup's share of message j (client.cpp:842-855): f_{j+1+r} . f_j summed over its
+// columns, then the four dots with u_{j-1} (j > 0); the four wavefronts' shares are added in
+// wavefront order through mp (4 x (m + 4) doubles of LDS) into out[0, m - 1 - j + 4); with
+// COHERENT the row is stored with device-scope atomics for other workgroups to read.
+template <int KM, int CPW, bool COHERENT = false>
+__device__ __forceinline__ void reg_share(const RegCols<KM, CPW>& R, int j, int m, int c, double* mp,
+                                          double* __restrict__ out) {
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int ldm = m + 4;
   const int nrows = m - 1 - j;
-  // this wavefront's share: sum over its columns of f_{j+1+r} f_j, and the four dots
   double pr[KM];
 #pragma unroll
   for (int q = 0; q < KM; q++) pr[q] = 0.0;
   double d0 = 0.0, d1 = 0.0, d2 = 0.0, d3 = 0.0;
 #pragma unroll
   for (int cc = 0; cc < CPW; cc++) {
-    if (cols[cc] >= c) continue;
+    if (R.cols[cc] >= c) continue;
     double fj = 0.0;
 #pragma unroll
     for (int q = 0; q < KM; q++)
-      if (lane + 64 * q == j) fj = fr[cc][q];
+      if (lane + 64 * q == j) fj = R.fr[cc][q];
     fj = __shfl(fj, j & 63, kWave);
 #pragma unroll
-    for (int q = 0; q < KM; q++) pr[q] += fr[cc][q] * fj;
+    for (int q = 0; q < KM; q++) pr[q] += R.fr[cc][q] * fj;
     if (j > 0) {
-      const double u = ui[cc];
+      const double u = R.ui[cc];
       d0 += fj * u;
       d1 += u * u;
       d2 += fj * fj;
@@ -213,8 +237,259 @@ __global__ __launch_bounds__(256) void k_ralha_step_reg(int i, int m, int c, dou
   }
   __syncthreads();
   const int cnt = nrows + (j > 0 ? 4 : 0);
-  double* out = partial + (long)blockIdx.x * (m + 3);
-  for (int r = tid; r < cnt; r += blockDim.x) out[r] = ((mp[r] + mp[ldm + r]) + mp[2 * ldm + r]) + mp[3 * ldm + r];
+  for (int r = tid; r < cnt; r += blockDim.x) {
+    const double v = ((mp[r] + mp[ldm + r]) + mp[2 * ldm + r]) + mp[3 * ldm + r];
+    if (COHERENT)
+      __hip_atomic_store(&out[r], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);  // read by other workgroups
+    else
+      out[r] = v;
+  }
+}
+
+// One launch per step: the register form of k_ralha_step below. A column is read once,
+// updated, stored once, and its share of the next message is formed from the registers.
+template <int KM, int CPW>
+__global__ __launch_bounds__(256) void k_ralha_step_reg(int i, int m, int c, double* __restrict__ Fc,
+                                                        double* __restrict__ Uc, const double* __restrict__ g,
+                                                        double* __restrict__ house, double* __restrict__ tau,
+                                                        double* __restrict__ alpha, double* __restrict__ beta,
+                                                        double* __restrict__ partial) {
+  extern __shared__ double lds[];  // hbuf: m, then mp: 4 x (m + 4)
+  double* hbuf = lds;
+  double* mp = lds + m;
+  __shared__ double red[8];
+  RegCols<KM, CPW> R;
+  reg_load(R, i, m, c, Fc, Uc);
+  if (i >= 0) {
+    reg_apply(R, i, m, c, g, hbuf, red, Uc, house, tau, alpha, beta);
+    reg_store(R, i, m, c, Fc);
+  }
+  const int j = i + 1;
+  if (j > m - 3) return;  // uniform: no message after the last full step
+  reg_share(R, j, m, c, mp, partial + (long)blockIdx.x * (m + 3));
+}
+
+// ---- The whole loop in one launch -------------------------------------------------------------
+//
+// k_ralha_loop runs steps -1 .. m-3 in one launch. Every workgroup keeps its columns in registers
+// for the whole loop, and the host only moves the messages over the network:
+//
+//   GPU: message j published (each workgroup reduces a slice of the partials into link.msg, then
+//        sets link.done[wg] = j + 1)
+//   host: waits for every done[wg] > j, all-reduces msg in place, sets link.in_seq = j + 1
+//   GPU: workgroup 0 copies msg to device memory and sets dsync[kGin]; every workgroup applies
+//        step j and forms its share of message j + 1.
+//
+// Memory ordering without cache maintenance: on gfx950 a device-scope fence writes back and
+// invalidates the whole L2 of the XCD, and with dozens of workgroups fencing several times per
+// step those operations stall every memory access on the chip. So every value that crosses
+// workgroups is stored and loaded with relaxed device-scope atomics (sc1: coherent across the
+// XCDs' L2s) or system-scope ones for the host link (sc0 sc1), and a wavefront waits for its own
+// stores to complete (s_waitcnt vmcnt(0)) before the workgroup barrier that precedes a flag
+// store. A flag is read with the same relaxed loads, and the data after it with them too, so no
+// cache can hold a stale copy.
+//
+// Every wait is bounded by a wall-clock limit and also ends on dsync[kStop] (set by any
+// workgroup that gave up) or link.abort (set by the host), so the grid always drains. The grid is
+// small (one workgroup per 4 x CPW columns) and the host launches it only when twice that many
+// workgroups fit on the device at once, so all workgroups are resident together.
+
+constexpr int kLoopMaxWg = 256;
+constexpr int kLoopMaxM = 3000;       // LDS: hbuf, the message copy and the four wavefronts' shares
+constexpr int kLinkMsgOffset = 1280;  // bytes: 64-byte header, done[256], then the message
+
+struct LoopLink {  // pinned, coherent host memory
+  int in_seq;      // host: reduced messages delivered so far
+  int abort;       // host: leave at the next wait
+  int error;       // GPU: why the first workgroup gave up: 1 = a wait ran out of time, 2 = the abort flag
+  int nwg, m;      // set by the host at launch
+  int pad[11];
+  int done[kLoopMaxWg];  // GPU: messages published by workgroup w's slice
+};
+static_assert(sizeof(LoopLink) <= kLinkMsgOffset, "link header");
+
+enum { kGin = 0, kBarrier = 1, kStop = 2 };  // device words
+
+__device__ __forceinline__ double ld_dev(const double* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_dev(double* p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double ld_sys(const double* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+__device__ __forceinline__ void st_sys(double* p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+// This wavefront's stores have completed (vmcnt 0; expcnt and lgkmcnt left alone).
+__device__ __forceinline__ void stores_done() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
+__device__ __forceinline__ void loop_fail(int* dsync, LoopLink* link, int why) {
+  __hip_atomic_store(&dsync[kStop], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(&link->error, why, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// Thread 0 waits until *word >= target (device memory, relaxed device-scope loads); the verdict
+// reaches every thread through LDS. Returns false when the wait gave up.
+__device__ __forceinline__ bool loop_wait_dev(int* word, int target, int* dsync, LoopLink* link, long long limit,
+                                              int* sh) {
+  if (threadIdx.x == 0) {
+    int ok = 1;
+    const long long t0 = wall_clock64();
+    while (__hip_atomic_load(word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      if (__hip_atomic_load(&dsync[kStop], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+        ok = 0;
+        break;
+      }
+      if (wall_clock64() - t0 > limit) {
+        loop_fail(dsync, link, 1);
+        ok = 0;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+    sh[0] = ok;
+  }
+  __syncthreads();
+  const bool ok = sh[0] != 0;
+  __syncthreads();
+  return ok;
+}
+
+// Workgroup 0's wait for the host (link.in_seq >= target, relaxed system-scope loads).
+__device__ __forceinline__ bool loop_wait_host(LoopLink* link, int target, int* dsync, long long limit, int* sh) {
+  if (threadIdx.x == 0) {
+    int ok = 1;
+    const long long t0 = wall_clock64();
+    for (unsigned k = 0;; k++) {
+      if (__hip_atomic_load(&link->in_seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) >= target) break;
+      if ((k & 15) == 15) {
+        if (__hip_atomic_load(&dsync[kStop], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+          ok = 0;  // another workgroup gave up and recorded why
+          break;
+        }
+        if (__hip_atomic_load(&link->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) != 0) {
+          loop_fail(dsync, link, 2);
+          ok = 0;
+          break;
+        }
+        if (wall_clock64() - t0 > limit) {
+          loop_fail(dsync, link, 1);
+          ok = 0;
+          break;
+        }
+      }
+    }
+    sh[0] = ok;
+  }
+  __syncthreads();
+  const bool ok = sh[0] != 0;
+  __syncthreads();
+  return ok;
+}
+
+// Publishes message j. This workgroup's partial row is in `partial` (stored with st_dev). After a
+// grid barrier each workgroup adds the partials of its slice of the message (fixed order: four
+// wavefronts take every fourth partial, then their sums are added in wavefront order) into
+// link.msg, zeros after the valid entries, and sets its done word.
+__device__ __forceinline__ bool loop_publish(int j, int m, const double* __restrict__ partial, LoopLink* link,
+                                             double* __restrict__ hmsg, int* dsync, long long limit, int* sh,
+                                             double (*acc)[64]) {
+  const int G = gridDim.x, w = blockIdx.x, lane = threadIdx.x & 63, grp = threadIdx.x >> 6;
+  stores_done();
+  __syncthreads();
+  if (threadIdx.x == 0) __hip_atomic_fetch_add(&dsync[kBarrier], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (!loop_wait_dev(&dsync[kBarrier], G * (j + 1), dsync, link, limit, sh)) return false;
+  const int len = m + 3 - j;
+  const int valid = (m - 1 - j) + (j > 0 ? 4 : 0);
+  const int per = (len + G - 1) / G;
+  const int r0 = w * per, r1 = min(len, r0 + per);
+  const long ld = m + 3;
+  for (int base = r0; base < r1; base += 64) {
+    const int r = base + lane;
+    double s = 0.0;
+    if (r < r1 && r < valid) {
+      int v = grp;
+      for (; v + 28 < G; v += 32) {
+        double x[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) x[q] = ld_dev(&partial[(long)(v + 4 * q) * ld + r]);
+#pragma unroll
+        for (int q = 0; q < 8; q++) s += x[q];
+      }
+      for (; v < G; v += 4) s += ld_dev(&partial[(long)v * ld + r]);
+    }
+    acc[grp][lane] = s;
+    __syncthreads();
+    if (grp == 0 && r < r1) st_sys(&hmsg[r], ((acc[0][lane] + acc[1][lane]) + acc[2][lane]) + acc[3][lane]);
+    __syncthreads();
+  }
+  stores_done();
+  __syncthreads();
+  if (threadIdx.x == 0) __hip_atomic_store(&link->done[w], j + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  return true;
+}
+
+// Brings the reduced message i into gl (LDS, m + 3 doubles): workgroup 0 copies it from the host
+// into gdev and sets dsync[kGin]; every workgroup then loads gdev.
+__device__ __forceinline__ bool loop_receive(int i, int m, LoopLink* link, const double* hmsg,
+                                             double* __restrict__ gdev, double* gl, int* dsync, long long limit,
+                                             int* sh) {
+  const int len = m + 3 - i;
+  if (blockIdx.x == 0) {
+    if (!loop_wait_host(link, i + 1, dsync, limit, sh)) return false;
+    for (int r = threadIdx.x; r < len; r += blockDim.x) {
+      const double v = ld_sys(&hmsg[r]);
+      st_dev(&gdev[r], v);
+      gl[r] = v;
+    }
+    stores_done();
+    __syncthreads();
+    if (threadIdx.x == 0) __hip_atomic_store(&dsync[kGin], i + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return true;
+  }
+  if (!loop_wait_dev(&dsync[kGin], i + 1, dsync, link, limit, sh)) return false;
+  for (int r = threadIdx.x; r < len; r += blockDim.x) gl[r] = ld_dev(&gdev[r]);
+  __syncthreads();
+  return true;
+}
+
+template <int KM, int CPW>
+__global__ __launch_bounds__(256) void k_ralha_loop(int m, int c, double* __restrict__ Fc, double* __restrict__ Uc,
+                                                    double* __restrict__ house, double* __restrict__ tau,
+                                                    double* __restrict__ alpha, double* __restrict__ beta,
+                                                    char* link_bytes, int* dsync, double* __restrict__ gdev,
+                                                    double* __restrict__ partial, long long limit) {
+  extern __shared__ double lds[];  // hbuf: m, mp: 4 x (m + 4), gl: m + 3
+  double* hbuf = lds;
+  double* mp = lds + m;
+  double* gl = mp + 4 * (m + 4);
+  __shared__ double red[8];
+  __shared__ double acc[4][64];
+  __shared__ int sh[2];
+  LoopLink* link = (LoopLink*)link_bytes;
+  double* hmsg = (double*)(link_bytes + kLinkMsgOffset);
+  double* mine = partial + (long)blockIdx.x * (m + 3);
+  RegCols<KM, CPW> R;
+  reg_load(R, -1, m, c, Fc, Uc);
+  reg_share<KM, CPW, true>(R, 0, m, c, mp, mine);
+  if (!loop_publish(0, m, partial, link, hmsg, dsync, limit, sh, acc)) return;
+  for (int i = 0; i <= m - 3; i++) {
+    if (!loop_receive(i, m, link, hmsg, gdev, gl, dsync, limit, sh)) return;
+    reg_apply(R, i, m, c, gl, hbuf, red, Uc, house, tau, alpha, beta);
+#pragma unroll
+    for (int cc = 0; cc < CPW; cc++) R.um1[cc] = R.ui[cc];
+    if (i + 1 <= m - 3) {
+      reg_share<KM, CPW, true>(R, i + 1, m, c, mp, mine);
+      if (!loop_publish(i + 1, m, partial, link, hmsg, dsync, limit, sh, acc)) return;
+    }
+  }
+  reg_store(R, 0, m, c, Fc);
+}
+
+__global__ void k_loop_clear(int* dsync) {
+  if (threadIdx.x < 4) dsync[threadIdx.x] = 0;
 }
 
 // One column step of the overlapped Ralha loop on column-major F and U (Fc[col*m + row]):
@@ -519,6 +794,187 @@ int exc_ralha_step_generic(int i, int m, int c, double* d_fc, double* d_uc, cons
                            size_t work_bytes, exc_stream_t stream) {
   return ralha_step(true, i, m, c, d_fc, d_uc, d_g, d_house, d_tau, d_alpha, d_beta, d_msg, d_work, work_bytes,
                     stream);
+}
+
+size_t exc_ralha_loop_workspace(int m, int c) {
+  if (m < 3 || c < 1) return 0;
+  const size_t g = ((size_t)(m + 3) * sizeof(double) + 255) / 256 * 256;
+  return 256 + g + (size_t)exc_detail::ceil_div(c, 4) * (m + 3) * sizeof(double);
+}
+
+int exc_ralha_link_alloc(int m, void** h_link) {
+  EXC_REQUIRE(m >= 3 && h_link, EXC_ERR_ARG, "exc_ralha_link_alloc: bad arguments");
+  EXC_CHECK_HIP(hipHostMalloc(h_link, kLinkMsgOffset + (size_t)(m + 3) * sizeof(double),
+                              hipHostMallocCoherent | hipHostMallocMapped));
+  memset(*h_link, 0, kLinkMsgOffset + (size_t)(m + 3) * sizeof(double));
+  ((LoopLink*)*h_link)->m = m;
+  return EXC_OK;
+}
+
+int exc_ralha_link_free(void* h_link) {
+  if (h_link) EXC_CHECK_HIP(hipHostFree(h_link));
+  return EXC_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+struct LoopShape {
+  int km = 0, cpw = 0, nwg = 0;
+  size_t lds = 0;
+};
+
+int loop_km(int m) { return m <= 256 ? 4 : m <= 512 ? 8 : m <= 1024 ? 16 : m <= 2048 ? 32 : m <= 4000 ? 64 : 0; }
+
+// Columns per wavefront: the fewest that keep the grid at 64 workgroups or below, within
+// KM * CPW <= 64 registers of columns (a hint of 1, 2 or 4 overrides, within the same bound).
+LoopShape loop_shape(int m, int c, int hint) {
+  LoopShape sh;
+  sh.km = m <= kLoopMaxM ? loop_km(m) : 0;
+  if (!sh.km) return sh;
+  const int maxc = sh.km >= 64 ? 1 : sh.km >= 32 ? 2 : 4;
+  int cpw = 1;
+  if (hint == 1 || hint == 2 || hint == 4) cpw = hint;
+  else
+    while (cpw < maxc && (long)exc_detail::ceil_div(c, 4 * cpw) > 64) cpw *= 2;
+  if (cpw > maxc) return LoopShape{};
+  sh.cpw = cpw;
+  sh.nwg = (int)exc_detail::ceil_div(c, 4 * cpw);
+  sh.lds = (size_t)(6 * m + 19) * sizeof(double);
+  return sh;
+}
+
+template <int KM, int CPW>
+const void* loop_kernel() {
+  return (const void*)k_ralha_loop<KM, CPW>;
+}
+
+const void* loop_kernel_for(const LoopShape& sh) {
+#define EXC_LOOP_CASE(K, C) \
+  if (sh.km == K && sh.cpw == C) return loop_kernel<K, C>();
+  EXC_LOOP_CASE(4, 1) EXC_LOOP_CASE(4, 2) EXC_LOOP_CASE(4, 4)
+  EXC_LOOP_CASE(8, 1) EXC_LOOP_CASE(8, 2) EXC_LOOP_CASE(8, 4)
+  EXC_LOOP_CASE(16, 1) EXC_LOOP_CASE(16, 2) EXC_LOOP_CASE(16, 4)
+  EXC_LOOP_CASE(32, 1) EXC_LOOP_CASE(32, 2)
+  EXC_LOOP_CASE(64, 1)
+#undef EXC_LOOP_CASE
+  return nullptr;
+}
+
+// True when twice the grid fits on the device at once (this peer's loop and another's).
+int loop_resident(const LoopShape& sh, const void* fn, bool* fits) {
+  *fits = false;
+  if (!fn || sh.nwg < 1 || sh.nwg > kLoopMaxWg) return EXC_OK;
+  int dev = 0, cus = 0, per_cu = 0;
+  EXC_CHECK_HIP(hipGetDevice(&dev));
+  EXC_CHECK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  if (sh.lds > 64 * 1024)
+    EXC_CHECK_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh.lds));
+  EXC_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, sh.lds));
+  *fits = (long)per_cu * cus >= 2L * sh.nwg;
+  return EXC_OK;
+}
+
+double wall_seconds() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+extern "C" {
+
+int exc_ralha_loop_fits(int m, int c, int cols_per_wave) {
+  if (m < 3 || c < 1) return 0;
+  const LoopShape sh = loop_shape(m, c, cols_per_wave);
+  bool fits = false;
+  if (loop_resident(sh, loop_kernel_for(sh), &fits) != EXC_OK) return 0;
+  return fits ? 1 : 0;
+}
+
+int exc_ralha_loop_start(int m, int c, double* d_fc, double* d_uc, double* d_house, double* d_tau, double* d_alpha,
+                         double* d_beta, void* h_link, void* d_work, size_t work_bytes, int cols_per_wave,
+                         double timeout_s, exc_stream_t stream) {
+  EXC_REQUIRE(m >= 3 && c >= 1 && d_fc && d_uc && d_house && d_tau && d_alpha && d_beta && h_link && d_work &&
+                  timeout_s > 0,
+              EXC_ERR_ARG, "exc_ralha_loop_start: bad arguments");
+  EXC_REQUIRE(work_bytes >= exc_ralha_loop_workspace(m, c), EXC_ERR_SPACE, "exc_ralha_loop_start: workspace too small");
+  LoopLink* link = (LoopLink*)h_link;
+  EXC_REQUIRE(link->m == m, EXC_ERR_ARG, "exc_ralha_loop_start: link allocated for another m");
+  const LoopShape sh = loop_shape(m, c, cols_per_wave);
+  const void* fn = loop_kernel_for(sh);
+  bool fits = false;
+  const int rc = loop_resident(sh, fn, &fits);
+  if (rc) return rc;
+  EXC_REQUIRE(fits, EXC_ERR_ARG, "exc_ralha_loop_start: the grid would not be resident (see exc_ralha_loop_fits)");
+  hipStream_t s = exc_detail::stream(stream);
+  EXC_CHECK_HIP(hipStreamSynchronize(s));  // no earlier loop still uses the link
+  memset(link, 0, kLinkMsgOffset);
+  link->nwg = sh.nwg;
+  link->m = m;
+  std::atomic_thread_fence(std::memory_order_seq_cst);
+  void* d_link = nullptr;
+  EXC_CHECK_HIP(hipHostGetDevicePointer(&d_link, h_link, 0));
+  int dev = 0, khz = 0;
+  EXC_CHECK_HIP(hipGetDevice(&dev));
+  EXC_CHECK_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+  EXC_REQUIRE(khz > 0, EXC_ERR_HIP, "exc_ralha_loop_start: no wall clock rate");
+  const long long limit = (long long)(timeout_s * 1e3 * khz);
+  char* w = (char*)d_work;
+  int* dsync = (int*)w;
+  double* gdev = (double*)(w + 256);
+  double* partial = (double*)(w + 256 + ((size_t)(m + 3) * sizeof(double) + 255) / 256 * 256);
+  hipLaunchKernelGGL(k_loop_clear, dim3(1), dim3(64), 0, s, dsync);
+  void* args[] = {&m, &c, &d_fc, &d_uc, &d_house, &d_tau, &d_alpha, &d_beta, &d_link, &dsync, &gdev, &partial,
+                  (void*)&limit};
+  EXC_CHECK_HIP(hipLaunchKernel(fn, dim3(sh.nwg), dim3(256), args, sh.lds, s));
+  return EXC_OK;
+}
+
+int exc_ralha_loop_wait(void* h_link, int j, double timeout_s, double** h_msg) {
+  EXC_REQUIRE(h_link && h_msg && j >= 0, EXC_ERR_ARG, "exc_ralha_loop_wait: bad arguments");
+  LoopLink* link = (LoopLink*)h_link;
+  const int G = link->nwg;
+  const double t0 = wall_seconds();
+  for (int w = 0; w < G; w++) {
+    unsigned k = 0;
+    while (__atomic_load_n(&link->done[w], __ATOMIC_ACQUIRE) < j + 1) {
+      if ((++k & 255) == 0) {
+        const int e = __atomic_load_n(&link->error, __ATOMIC_RELAXED);
+        EXC_REQUIRE(e == 0, EXC_ERR_NUMERIC,
+                    e == 1 ? "exc_ralha_loop_wait: a wait of the loop kernel ran out of time"
+                           : "exc_ralha_loop_wait: the loop kernel stopped");
+        if (wall_seconds() - t0 > timeout_s) {
+          __atomic_store_n(&link->abort, 1, __ATOMIC_RELEASE);
+          EXC_REQUIRE(false, EXC_ERR_NUMERIC, "exc_ralha_loop_wait: timed out waiting for the loop kernel");
+        }
+      }
+    }
+  }
+  *h_msg = (double*)((char*)h_link + kLinkMsgOffset);
+  return EXC_OK;
+}
+
+int exc_ralha_loop_post(void* h_link, int j) {
+  EXC_REQUIRE(h_link && j >= 0, EXC_ERR_ARG, "exc_ralha_loop_post: bad arguments");
+  __atomic_store_n(&((LoopLink*)h_link)->in_seq, j + 1, __ATOMIC_RELEASE);
+  return EXC_OK;
+}
+
+int exc_ralha_loop_abort(void* h_link) {
+  EXC_REQUIRE(h_link, EXC_ERR_ARG, "exc_ralha_loop_abort: bad arguments");
+  __atomic_store_n(&((LoopLink*)h_link)->abort, 1, __ATOMIC_RELEASE);
+  return EXC_OK;
+}
+
+int exc_ralha_loop_finish(void* h_link, exc_stream_t stream) {
+  EXC_REQUIRE(h_link, EXC_ERR_ARG, "exc_ralha_loop_finish: bad arguments");
+  EXC_CHECK_HIP(hipStreamSynchronize(exc_detail::stream(stream)));
+  const int e = __atomic_load_n(&((LoopLink*)h_link)->error, __ATOMIC_ACQUIRE);
+  EXC_REQUIRE(e == 0, EXC_ERR_NUMERIC,
+              e == 1 ? "exc_ralha_loop_finish: a wait of the loop kernel ran out of time"
+                     : "exc_ralha_loop_finish: the loop kernel stopped");
+  return EXC_OK;
 }
 
 int exc_ralha_dot(int i, int m, int c, const double* d_fc, const double* d_uc, double* d_out, exc_stream_t stream) {

@@ -30,7 +30,9 @@ __device__ __forceinline__ bool wait_flag(int* f, int* error, int* sh) {
   if (threadIdx.x == 0) {
     int ok = 1;
     long spins = 0;
-    while (__hip_atomic_load(f, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+    // relaxed polls read the coherent value without invalidating the caches on every
+    // iteration; one acquire fence once the flag is seen
+    while (__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
       if (__hip_atomic_load(error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) { ok = 0; break; }
       if (++spins > kSpinLimit) {
         __hip_atomic_store(error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -39,6 +41,7 @@ __device__ __forceinline__ bool wait_flag(int* f, int* error, int* sh) {
       }
       __builtin_amdgcn_s_sleep(2);
     }
+    if (ok) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     sh[0] = ok;
   }
   __syncthreads();

@@ -70,6 +70,15 @@ SIGNATURES = {
     "exc_ralha_workspace": (_S, [_I, _I]),
     "exc_ralha_step": (_I, [_I, _I, _I, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _vp, _S, _vp]),
     "exc_ralha_step_generic": (_I, [_I, _I, _I, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _vp, _S, _vp]),
+    "exc_ralha_loop_workspace": (_S, [_I, _I]),
+    "exc_ralha_link_alloc": (_I, [_I, ctypes.POINTER(_vp)]),
+    "exc_ralha_link_free": (_I, [_vp]),
+    "exc_ralha_loop_fits": (_I, [_I, _I, _I]),
+    "exc_ralha_loop_start": (_I, [_I, _I, _dp, _dp, _dp, _dp, _dp, _dp, _vp, _vp, _S, _I, _D, _vp]),
+    "exc_ralha_loop_wait": (_I, [_vp, _I, _D, ctypes.POINTER(_vp)]),
+    "exc_ralha_loop_post": (_I, [_vp, _I]),
+    "exc_ralha_loop_abort": (_I, [_vp]),
+    "exc_ralha_loop_finish": (_I, [_vp, _vp]),
     "exc_ralha_dot": (_I, [_I, _I, _I, _dp, _dp, _dp, _vp]),
     "exc_ralha_tail_update": (_I, [_I, _I, _I, _dp, _dp, _dp, _dp, _dp, _vp]),
     "exc_ralha_tail_scale": (_I, [_I, _I, _I, _dp, _dp, _dp, _dp, _vp]),
@@ -363,11 +372,13 @@ def ormqr_house_ln(house, tau, C):
     return dC.to_host()
 
 
-def ralha_peers(F_list, generic=False):
+def ralha_peers(F_list, generic=False, loop=False, cols_per_wave=0, timeout_s=30.0):
     """The overlapped Ralha loop over P simulated peers in this process: GPU kernels per peer,
     the all-reduce done here in each peer's own order (own buffer, then the others by id).
     F_list[p] is peer p's m x c_p share of the rows f_j; U is returned in the same orientation.
-    generic selects the streamed (HBM) form of the step kernel instead of the register form."""
+    generic selects the streamed (HBM) form of the step kernel instead of the register form;
+    loop runs steps -1 .. m-3 as one launch per peer (one stream each, all running at once) with
+    this thread all-reducing the messages through the link buffers."""
     P = len(F_list)
     m = F_list[0].shape[0]
     cs = [f.shape[1] for f in F_list]
@@ -409,13 +420,17 @@ def ralha_peers(F_list, generic=False):
     for p in range(P):
         put(ds[p], red[p], 1)
         call("exc_ralha_tail_scale", 0, m, cs[p], dF[p].ptr, dU[p].ptr, ds[p].ptr + 8, da[p].ptr, None)
-        step(-1, p)
-    for i in range(m - 2):
-        sync()
-        red = allreduce(dmsg, m + 3 - i)
-        for p in range(P):
-            put(dg[p], red[p])
-            step(i, p)
+        if not loop:
+            step(-1, p)
+    if loop:
+        _ralha_loop_run(m, cs, dF, dU, dh, dt, da, db, cols_per_wave, timeout_s)
+    else:
+        for i in range(m - 2):
+            sync()
+            red = allreduce(dmsg, m + 3 - i)
+            for p in range(P):
+                put(dg[p], red[p])
+                step(i, p)
     sync()
     for i in range(m - 2, m):
         for p in range(P):
@@ -438,6 +453,59 @@ def ralha_peers(F_list, generic=False):
         sync()
     return [dict(alpha=da[p].to_host(), beta=db[p].to_host()[: m - 1], house=dh[p].to_host(), tau=dt[p].to_host(),
                  U=np.ascontiguousarray(dU[p].to_host().T)) for p in range(P)]
+
+
+def _ralha_loop_run(m, cs, dF, dU, dh, dt, da, db, cols_per_wave, timeout_s):
+    """Steps -1 .. m-3 of every peer as one launch each, on P streams; messages all-reduced here."""
+    P = len(cs)
+    sync()
+    for c in cs:
+        if not _lib.exc_ralha_loop_fits(m, c, cols_per_wave):
+            raise HipError(-1, "exc_ralha_loop_fits")
+    streams, links, ws = [], [], []
+    try:
+        for p in range(P):
+            st = _vp()
+            call("exc_stream_create", ctypes.byref(st))
+            streams.append(st.value)
+            ln = _vp()
+            call("exc_ralha_link_alloc", m, ctypes.byref(ln))
+            links.append(ln.value)
+            ws.append(workspace(_lib.exc_ralha_loop_workspace(m, cs[p])))
+        started = []
+        try:
+            for p in range(P):
+                call("exc_ralha_loop_start", m, cs[p], dF[p].ptr, dU[p].ptr, dh[p].ptr, dt[p].ptr, da[p].ptr,
+                     db[p].ptr, links[p], ws[p].ptr, ws[p].nbytes, cols_per_wave, timeout_s, streams[p])
+                started.append(p)
+            for j in range(m - 2):
+                n = m + 3 - j
+                views = []
+                for p in range(P):
+                    ptr = _vp()
+                    call("exc_ralha_loop_wait", links[p], j, timeout_s, ctypes.byref(ptr))
+                    views.append(np.ctypeslib.as_array((ctypes.c_double * n).from_address(ptr.value)))
+                vals = [v.copy() for v in views]
+                for p in range(P):
+                    acc = vals[p].copy()
+                    for q in range(P):
+                        if q != p:
+                            acc = acc + vals[q]
+                    views[p][:] = acc
+                    call("exc_ralha_loop_post", links[p], j)
+        except BaseException:
+            for p in started:
+                _lib.exc_ralha_loop_abort(links[p])
+            for p in started:
+                _lib.exc_ralha_loop_finish(links[p], streams[p])
+            raise
+        for p in started:
+            call("exc_ralha_loop_finish", links[p], streams[p])
+    finally:
+        for ln in links:
+            _lib.exc_ralha_link_free(ln)
+        for st in streams:
+            _lib.exc_stream_destroy(st)
 
 
 def lower_to_upper(alpha, beta):

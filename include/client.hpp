@@ -37,6 +37,7 @@ struct ClientOptions {
   const double* data = nullptr;   // optional in-memory input (m x local_n row-major) instead of dpath
   int repeat = 1;                 // run() this many times on the same input (benchmarking)
   bool profile = false;           // time the MFMA GEMM / Givens launches with HIP events
+  bool ralha_one_launch = true;   // the bidiagonalization loop as one kernel launch where it fits
 };
 
 class DecentralizedClient {

@@ -199,6 +199,38 @@ int exc_ralha_step(int i, int m, int c, double* d_fc, double* d_uc, const double
 int exc_ralha_step_generic(int i, int m, int c, double* d_fc, double* d_uc, const double* d_g, double* d_house,
                            double* d_tau, double* d_alpha, double* d_beta, double* d_msg, void* d_work,
                            size_t work_bytes, exc_stream_t stream);
+/* Steps -1 .. m-3 of the same loop (client.cpp:836-910) in ONE launch that runs while the host
+ * all-reduces the messages: every workgroup keeps its columns in registers for the whole loop.
+ * The messages pass through a link buffer in coherent pinned host memory
+ * (exc_ralha_link_alloc). The host drives it like this:
+ *
+ *   exc_ralha_loop_start(...);                      // synchronizes `stream`, then launches
+ *   for (j = 0; j <= m - 3; j++) {
+ *     exc_ralha_loop_wait(link, j, timeout, &msg);  // message j: m+3-j doubles, as exc_ralha_step
+ *     all-reduce msg in place over the peers;
+ *     exc_ralha_loop_post(link, j);
+ *   }
+ *   exc_ralha_loop_finish(link, stream);            // synchronizes; error if the kernel gave up
+ *
+ * On any error between start and finish, call exc_ralha_loop_abort(link) and then
+ * exc_ralha_loop_finish: every wait of the kernel also ends on the abort flag or after
+ * timeout_s, so the grid always drains. The results (house, tau, alpha, beta, U, F) are those of
+ * exc_ralha_step for i = -1 .. m-3, with the partial sums of each message added in another fixed
+ * order. Applies when exc_ralha_loop_fits(m, c, cols_per_wave) returns 1: m <= 3000, and twice
+ * the grid (one workgroup per 4 x cols_per_wave columns, at most 256) fits on the device at
+ * once, so two peers' loops can share it. cols_per_wave = 0 picks 1, 2 or 4. No counterpart in
+ * the reference, which runs each step as BLAS calls between blocking all-reduces. */
+size_t exc_ralha_loop_workspace(int m, int c);
+int exc_ralha_link_alloc(int m, void** h_link);
+int exc_ralha_link_free(void* h_link);
+int exc_ralha_loop_fits(int m, int c, int cols_per_wave);
+int exc_ralha_loop_start(int m, int c, double* d_fc, double* d_uc, double* d_house, double* d_tau, double* d_alpha,
+                         double* d_beta, void* h_link, void* d_work, size_t work_bytes, int cols_per_wave,
+                         double timeout_s, exc_stream_t stream);
+int exc_ralha_loop_wait(void* h_link, int j, double timeout_s, double** h_msg);
+int exc_ralha_loop_post(void* h_link, int j);
+int exc_ralha_loop_abort(void* h_link);
+int exc_ralha_loop_finish(void* h_link, exc_stream_t stream);
 /* The last two columns and the first (client.cpp:830-833, 897-905): d_out = f_i.u_{i-1};
  * tail_update: f_i -= beta u_{i-1} (beta = reduced value, stored in d_beta[i-1]), d_out =
  * ||f_i||^2 (i = -1: ||f_0||^2 only); tail_scale: alpha_i = sqrt(reduced), u_i = f_i / alpha_i. */

@@ -73,6 +73,110 @@ def test_ralha_vs_oracle(hip, oracle, P, m, cols, generic):
                 assert np.max(np.abs(hg[j + 1:, j] - ho[j + 1:, j])) <= 1e-10
 
 
+RALHA_SIZES = [(1, 12, [12]), (2, 60, [30, 30]), (3, 101, [34, 34, 33]), (2, 300, [150, 150]),
+               (2, 600, [301, 299]), (1, 1100, [1100])]
+
+
+def _ralha_input(P, m, cols):
+    rng = np.random.default_rng(m + P)
+    Rf = np.linalg.qr(rng.standard_normal((2 * m, m)))[1]
+    return [Rf[:, sum(cols[:p]):sum(cols[:p + 1])].copy() for p in range(P)]
+
+
+LOOP_SIZES = RALHA_SIZES[:-1] + [(2, 1100, [550, 550])]
+
+
+@pytest.mark.parametrize("P,m,cols", LOOP_SIZES)
+def test_ralha_loop_vs_oracle(hip, oracle, P, m, cols):
+    """The one-launch loop (one persistent kernel per peer, messages through the link buffers)."""
+    banner(f"ralha loop P={P} m={m}")
+    F = _ralha_input(P, m, cols)
+    got = hip.ralha_peers(F, loop=True)
+    want = oracle.ralha(F)
+    tol = max(1e-11, 1e-13 * m)  # m dependent steps; the last reflectors act on 2-4 entries
+    for p in range(P):
+        for key in ("alpha", "beta", "tau", "U"):
+            a, b = got[p][key], want[p][key]
+            assert np.max(np.abs(a - b)) <= tol * max(1.0, np.abs(b).max()), key
+        for j in range(1, m - 1):
+            if want[p]["tau"][j] != 0:
+                assert np.max(np.abs(got[p]["house"][j + 1:, j] - want[p]["house"][j + 1:, j])) <= 1e-10
+
+
+def test_ralha_loop_declines_a_grid_that_cannot_be_resident(hip):
+    """One peer with 1100 columns at m = 1100 needs more workgroups than can be resident twice
+    over; the runtime then keeps the per-step kernels."""
+    assert hip.lib.exc_ralha_loop_fits(1100, 1100, 0) == 0
+    assert hip.lib.exc_ralha_loop_fits(1000, 500, 0) == 1
+    assert hip.lib.exc_ralha_loop_fits(5000, 10, 0) == 0  # m above the register form
+
+
+@pytest.mark.parametrize("P,m,cols", [(2, 60, [30, 30]), (3, 101, [34, 34, 33]), (2, 600, [301, 299]),
+                                      (2, 1100, [550, 550])])
+def test_ralha_loop_equals_step_kernels(hip, P, m, cols):
+    """With the step kernel's grid (2 columns per wavefront up to m = 1024, 1 above), the loop adds
+    the same partial sums in the same order, so every output is bit for bit the per-step one."""
+    banner(f"ralha loop vs steps P={P} m={m}")
+    F = _ralha_input(P, m, cols)
+    cpw = 2 if m <= 1024 else 1
+    a = hip.ralha_peers(F, loop=True, cols_per_wave=cpw)
+    b = hip.ralha_peers(F)
+    for p in range(P):
+        for key in ("alpha", "beta", "tau", "U", "house"):
+            assert np.array_equal(a[p][key], b[p][key]), key
+
+
+def _loop_setup(hip, m, c):
+    import ctypes
+    rng = np.random.default_rng(7)
+    F = np.linalg.qr(rng.standard_normal((2 * m, m)))[1][:, :c]
+    dF = hip.DeviceArray.from_host(np.ascontiguousarray(F.T))
+    bufs = [hip.DeviceArray((c, m)).zero(), hip.DeviceArray((m, m)).zero()] + [hip.DeviceArray((m,)).zero()
+                                                                              for _ in range(3)]
+    ws = hip.workspace(hip.lib.exc_ralha_loop_workspace(m, c))
+    ln = ctypes.c_void_p()
+    hip.call("exc_ralha_link_alloc", m, ctypes.byref(ln))
+    return dF, bufs, ws, ln.value
+
+
+def test_ralha_loop_abort_ends_the_kernel(hip):
+    """The host raises the abort flag after the first message; every wave leaves at its next wait
+    and finish reports the stop (no message is ever posted)."""
+    import ctypes
+    banner("ralha loop abort")
+    m, c = 64, 40
+    dF, (dU, dh, dt, da, db), ws, link = _loop_setup(hip, m, c)
+    try:
+        hip.call("exc_ralha_loop_start", m, c, dF.ptr, dU.ptr, dh.ptr, dt.ptr, da.ptr, db.ptr, link, ws.ptr,
+                 ws.nbytes, 0, 20.0, None)
+        ptr = ctypes.c_void_p()
+        hip.call("exc_ralha_loop_wait", link, 0, 20.0, ctypes.byref(ptr))
+        hip.call("exc_ralha_loop_abort", link)
+        with pytest.raises(hip.HipError, match="stopped"):
+            hip.call("exc_ralha_loop_finish", link, None)
+    finally:
+        hip.lib.exc_ralha_link_free(link)
+
+
+def test_ralha_loop_times_out_without_the_host(hip):
+    """With a 0.2 s limit and no message posted, the kernel's own bounded wait ends the launch."""
+    import ctypes
+    import time
+    banner("ralha loop time limit")
+    m, c = 64, 40
+    dF, (dU, dh, dt, da, db), ws, link = _loop_setup(hip, m, c)
+    try:
+        hip.call("exc_ralha_loop_start", m, c, dF.ptr, dU.ptr, dh.ptr, dt.ptr, da.ptr, db.ptr, link, ws.ptr,
+                 ws.nbytes, 0, 0.2, None)
+        ptr = ctypes.c_void_p()
+        hip.call("exc_ralha_loop_wait", link, 0, 20.0, ctypes.byref(ptr))
+        time.sleep(0.5)
+        with pytest.raises(hip.HipError, match="ran out of time"):
+            hip.call("exc_ralha_loop_finish", link, None)
+    finally:
+        hip.lib.exc_ralha_link_free(link)
+
+
 def test_bidiag_helpers_vs_oracle(hip, oracle):
     banner("lower->upper, sign rule, back rotation")
     rng = np.random.default_rng(4)
@@ -158,6 +262,20 @@ def test_two_peer_run_with_local_qr(hip, oracle):
     xs = [rng.standard_normal((60, 400)), rng.standard_normal((60, 400))]
     res = run_peers(xs, seed=3)
     run = oracle.Run(xs, [3, 3])
+    st = compare_runs(res, run, xs)
+    print(st)
+    assert st["resid"] <= 1e-12 and st["orth_u"] <= 1e-12 and st["orth_v"] <= 1e-12
+    assert st["du_separated"] <= 1e-9
+
+
+def test_two_peer_run_step_kernels(hip, oracle):
+    """The same run with the bidiagonalization as one launch per step (--ralha-loop 0)."""
+    from excalibur_amd.launch import run_peers
+    banner("2-peer run, per-step Ralha kernels")
+    rng = np.random.default_rng(6)
+    xs = [rng.standard_normal((80, 300)), rng.standard_normal((80, 260))]
+    res = run_peers(xs, seed=4, extra=["--ralha-loop", "0"])
+    run = oracle.Run(xs, [4, 4])
     st = compare_runs(res, run, xs)
     print(st)
     assert st["resid"] <= 1e-12 and st["orth_u"] <= 1e-12 and st["orth_v"] <= 1e-12
