@@ -97,6 +97,44 @@ __device__ double ralha_reflector(int i, int m, const double* __restrict__ g, do
   return t;
 }
 
+// Sum over the workgroups' partial rows of entry e (partial[v * ld + e], v < nwg) in a fixed
+// order: sixteen groups of 16 threads (group g = threadIdx.x / 16, entry lane threadIdx.x % 16)
+// each add the partials v = g, g + 16, g + 32, ... in turn, four loads in flight; then the
+// sixteen group sums are added in group order. A 256-thread workgroup handles 16 entries per
+// call; the result is returned to threads 0..15 (entry lane threadIdx.x). With COHERENT the
+// partials are read with device-scope atomics (written by other workgroups of a running grid).
+template <bool COHERENT>
+__device__ __forceinline__ double partials_sum(const double* __restrict__ partial, long ld, int nwg, int e,
+                                               bool active, double (*acc)[16]) {
+  const int g = threadIdx.x >> 4, l = threadIdx.x & 15;
+  double s = 0.0;
+  if (active) {
+    auto ld1 = [&](int v) {
+      const double* p = partial + (long)v * ld + e;
+      return COHERENT ? __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : *p;
+    };
+    int v = g;
+    for (; v + 48 < nwg; v += 64) {
+      double x[4];
+#pragma unroll
+      for (int q = 0; q < 4; q++) x[q] = ld1(v + 16 * q);
+#pragma unroll
+      for (int q = 0; q < 4; q++) s += x[q];
+    }
+    for (; v < nwg; v += 16) s += ld1(v);
+  }
+  acc[g][l] = s;
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x < 16) {
+    t = acc[0][l];
+#pragma unroll
+    for (int q = 1; q < 16; q++) t += acc[q][l];
+  }
+  __syncthreads();
+  return t;
+}
+
 // Register-resident columns for m <= 64 * KM: each wavefront holds CPW whole columns of F (row
 // lane + 64q in slot q), with u_{i-1} and u_i of each column.
 template <int KM, int CPW>
@@ -130,6 +168,8 @@ __device__ __forceinline__ void reg_load(RegCols<KM, CPW>& R, int i, int m, int 
 // Step i >= 0 on the registers (client.cpp:858-891): the reflector from the reduced message g,
 // F[i+1:, col] -= tau h (h . F[i+1:, col]), then f_i -= beta u_{i-1} and u_i = f_i / alpha_i
 // (stored to U); for i = 0, u_0 comes from the tail kernels. Leaves u_i in R.ui on every lane.
+// The CPW columns' dot products run side by side (four partial sums each, one butterfly over
+// all of them), so their latencies overlap.
 template <int KM, int CPW>
 __device__ __forceinline__ void reg_apply(RegCols<KM, CPW>& R, int i, int m, int c, const double* __restrict__ g,
                                           double* hbuf, double* red, double* __restrict__ Uc,
@@ -138,40 +178,65 @@ __device__ __forceinline__ void reg_apply(RegCols<KM, CPW>& R, int i, int m, int
   const int lane = threadIdx.x & 63;
   double b, a;
   const double t = ralha_reflector(i, m, g, hbuf, red, house, tau, alpha, beta, &b, &a);
+  if (t != 0.0) {
+    double h[KM];
 #pragma unroll
-  for (int cc = 0; cc < CPW; cc++) {
-    if (R.cols[cc] >= c) continue;  // uniform within the wavefront
-    if (t != 0.0) {
-      double s = 0.0;
-#pragma unroll
-      for (int q = 0; q < KM; q++) {
-        const int row = lane + 64 * q;
-        if (row > i && row < m) s += R.fr[cc][q] * hbuf[row - i - 1];
-      }
-      s = wave_sum(s);
-#pragma unroll
-      for (int q = 0; q < KM; q++) {
-        const int row = lane + 64 * q;
-        if (row > i && row < m) R.fr[cc][q] += s * (-t * hbuf[row - i - 1]);
-      }
+    for (int q = 0; q < KM; q++) {
+      const int row = lane + 64 * q;
+      h[q] = (row > i && row < m) ? hbuf[row - i - 1] : 0.0;
     }
-    if (i > 0) {
-      // row i sits in lane i % 64, slot i / 64
+    double s[CPW];
 #pragma unroll
-      for (int q = 0; q < KM; q++) {
-        if (lane + 64 * q == i) {
+    for (int cc = 0; cc < CPW; cc++) {
+      double p0 = 0.0, p1 = 0.0, p2 = 0.0, p3 = 0.0;
+#pragma unroll
+      for (int q = 0; q < KM; q += 4) {
+        p0 += R.fr[cc][q] * h[q];
+        if (q + 1 < KM) p1 += R.fr[cc][q + 1] * h[q + 1];
+        if (q + 2 < KM) p2 += R.fr[cc][q + 2] * h[q + 2];
+        if (q + 3 < KM) p3 += R.fr[cc][q + 3] * h[q + 3];
+      }
+      s[cc] = (p0 + p1) + (p2 + p3);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      double o[CPW];
+#pragma unroll
+      for (int cc = 0; cc < CPW; cc++) o[cc] = __shfl_xor(s[cc], off, kWave);
+#pragma unroll
+      for (int cc = 0; cc < CPW; cc++) s[cc] += o[cc];
+    }
+#pragma unroll
+    for (int q = 0; q < KM; q++) {
+      const int row = lane + 64 * q;
+      const double th = -t * h[q];
+#pragma unroll
+      for (int cc = 0; cc < CPW; cc++)
+        if (R.cols[cc] < c && row > i && row < m) R.fr[cc][q] += s[cc] * th;
+    }
+  }
+  const int qi = i >> 6, li = i & 63;
+  if (i > 0) {
+    const double inv_a = 1.0 / a;
+#pragma unroll
+    for (int q = 0; q < KM; q++) {
+      if (q == qi) {  // uniform; row i sits in lane i % 64 of slot i / 64
+#pragma unroll
+        for (int cc = 0; cc < CPW; cc++) {
           const double fi = R.fr[cc][q] + (-b) * R.um1[cc];
-          R.fr[cc][q] = fi;
-          R.ui[cc] = (1.0 / a) * fi;
-          Uc[(long)R.cols[cc] * m + i] = R.ui[cc];
+          const double ui = inv_a * fi;
+          if (lane == li && R.cols[cc] < c) {
+            R.fr[cc][q] = fi;
+            Uc[(long)R.cols[cc] * m + i] = ui;
+          }
+          const double u = __shfl(ui, li, kWave);
+          R.ui[cc] = R.cols[cc] < c ? u : 0.0;
         }
       }
-    } else {
-#pragma unroll
-      for (int q = 0; q < KM; q++)
-        if (lane + 64 * q == 0) R.ui[cc] = Uc[(long)R.cols[cc] * m];
     }
-    R.ui[cc] = __shfl(R.ui[cc], i & 63, kWave);
+  } else {
+#pragma unroll
+    for (int cc = 0; cc < CPW; cc++) R.ui[cc] = R.cols[cc] < c ? Uc[(long)R.cols[cc] * m] : 0.0;
   }
 }
 
@@ -390,13 +455,12 @@ __device__ __forceinline__ bool loop_wait_host(LoopLink* link, int target, int* 
 }
 
 // Publishes message j. This workgroup's partial row is in `partial` (stored with st_dev). After a
-// grid barrier each workgroup adds the partials of its slice of the message (fixed order: four
-// wavefronts take every fourth partial, then their sums are added in wavefront order) into
-// link.msg, zeros after the valid entries, and sets its done word.
+// grid barrier each workgroup adds the partials of its slice of the message (partials_sum, the
+// order of k_ralha_reduce) into link.msg, zeros after the valid entries, and sets its done word.
 __device__ __forceinline__ bool loop_publish(int j, int m, const double* __restrict__ partial, LoopLink* link,
                                              double* __restrict__ hmsg, int* dsync, long long limit, int* sh,
-                                             double (*acc)[64]) {
-  const int G = gridDim.x, w = blockIdx.x, lane = threadIdx.x & 63, grp = threadIdx.x >> 6;
+                                             double (*acc)[16]) {
+  const int G = gridDim.x, w = blockIdx.x;
   stores_done();
   __syncthreads();
   if (threadIdx.x == 0) __hip_atomic_fetch_add(&dsync[kBarrier], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -405,25 +469,10 @@ __device__ __forceinline__ bool loop_publish(int j, int m, const double* __restr
   const int valid = (m - 1 - j) + (j > 0 ? 4 : 0);
   const int per = (len + G - 1) / G;
   const int r0 = w * per, r1 = min(len, r0 + per);
-  const long ld = m + 3;
-  for (int base = r0; base < r1; base += 64) {
-    const int r = base + lane;
-    double s = 0.0;
-    if (r < r1 && r < valid) {
-      int v = grp;
-      for (; v + 28 < G; v += 32) {
-        double x[8];
-#pragma unroll
-        for (int q = 0; q < 8; q++) x[q] = ld_dev(&partial[(long)(v + 4 * q) * ld + r]);
-#pragma unroll
-        for (int q = 0; q < 8; q++) s += x[q];
-      }
-      for (; v < G; v += 4) s += ld_dev(&partial[(long)v * ld + r]);
-    }
-    acc[grp][lane] = s;
-    __syncthreads();
-    if (grp == 0 && r < r1) st_sys(&hmsg[r], ((acc[0][lane] + acc[1][lane]) + acc[2][lane]) + acc[3][lane]);
-    __syncthreads();
+  for (int base = r0; base < r1; base += 16) {
+    const int e = base + (threadIdx.x & 15);
+    const double v = partials_sum<true>(partial, m + 3, G, e, e < r1 && e < valid, acc);
+    if (threadIdx.x < 16 && e < r1) st_sys(&hmsg[e], v);
   }
   stores_done();
   __syncthreads();
@@ -433,16 +482,27 @@ __device__ __forceinline__ bool loop_publish(int j, int m, const double* __restr
 
 // Brings the reduced message i into gl (LDS, m + 3 doubles): workgroup 0 copies it from the host
 // into gdev and sets dsync[kGin]; every workgroup then loads gdev.
+template <int KM>
 __device__ __forceinline__ bool loop_receive(int i, int m, LoopLink* link, const double* hmsg,
                                              double* __restrict__ gdev, double* gl, int* dsync, long long limit,
                                              int* sh) {
   const int len = m + 3 - i;
   if (blockIdx.x == 0) {
     if (!loop_wait_host(link, i + 1, dsync, limit, sh)) return false;
-    for (int r = threadIdx.x; r < len; r += blockDim.x) {
-      const double v = ld_sys(&hmsg[r]);
-      st_dev(&gdev[r], v);
-      gl[r] = v;
+    constexpr int kPer = (64 * KM + 3 + 255) / 256;  // m <= 64 KM
+    double v[kPer];
+#pragma unroll
+    for (int q = 0; q < kPer; q++) {
+      const int r = threadIdx.x + 256 * q;
+      v[q] = r < len ? ld_sys(&hmsg[r]) : 0.0;  // all loads in flight before the first store
+    }
+#pragma unroll
+    for (int q = 0; q < kPer; q++) {
+      const int r = threadIdx.x + 256 * q;
+      if (r < len) {
+        st_dev(&gdev[r], v[q]);
+        gl[r] = v[q];
+      }
     }
     stores_done();
     __syncthreads();
@@ -450,7 +510,18 @@ __device__ __forceinline__ bool loop_receive(int i, int m, LoopLink* link, const
     return true;
   }
   if (!loop_wait_dev(&dsync[kGin], i + 1, dsync, link, limit, sh)) return false;
-  for (int r = threadIdx.x; r < len; r += blockDim.x) gl[r] = ld_dev(&gdev[r]);
+  constexpr int kPer = (64 * KM + 3 + 255) / 256;
+  double v[kPer];
+#pragma unroll
+  for (int q = 0; q < kPer; q++) {
+    const int r = threadIdx.x + 256 * q;
+    v[q] = r < len ? ld_dev(&gdev[r]) : 0.0;
+  }
+#pragma unroll
+  for (int q = 0; q < kPer; q++) {
+    const int r = threadIdx.x + 256 * q;
+    if (r < len) gl[r] = v[q];
+  }
   __syncthreads();
   return true;
 }
@@ -466,7 +537,7 @@ __global__ __launch_bounds__(256) void k_ralha_loop(int m, int c, double* __rest
   double* mp = lds + m;
   double* gl = mp + 4 * (m + 4);
   __shared__ double red[8];
-  __shared__ double acc[4][64];
+  __shared__ double acc[16][16];
   __shared__ int sh[2];
   LoopLink* link = (LoopLink*)link_bytes;
   double* hmsg = (double*)(link_bytes + kLinkMsgOffset);
@@ -476,7 +547,7 @@ __global__ __launch_bounds__(256) void k_ralha_loop(int m, int c, double* __rest
   reg_share<KM, CPW, true>(R, 0, m, c, mp, mine);
   if (!loop_publish(0, m, partial, link, hmsg, dsync, limit, sh, acc)) return;
   for (int i = 0; i <= m - 3; i++) {
-    if (!loop_receive(i, m, link, hmsg, gdev, gl, dsync, limit, sh)) return;
+    if (!loop_receive<KM>(i, m, link, hmsg, gdev, gl, dsync, limit, sh)) return;
     reg_apply(R, i, m, c, gl, hbuf, red, Uc, house, tau, alpha, beta);
 #pragma unroll
     for (int cc = 0; cc < CPW; cc++) R.um1[cc] = R.ui[cc];
@@ -551,31 +622,15 @@ __global__ __launch_bounds__(256) void k_ralha_step(int i, int m, int c, double*
 }
 
 // msg[r] = sum over the step workgroups of partial[wg][r]; zeros after the valid entries. A
-// workgroup owns 64 message entries; its four wavefronts each add every fourth partial (eight
-// loads in flight per lane), then the four sums are added in wavefront order.
+// workgroup owns 16 message entries (partials_sum gives the order).
 __global__ __launch_bounds__(256) void k_ralha_reduce(int j, int m, int nwg, const double* __restrict__ partial,
                                                       double* __restrict__ msg) {
-  __shared__ double acc[4][64];
-  const int lane = threadIdx.x & 63, grp = threadIdx.x >> 6;
-  const int r = blockIdx.x * 64 + lane;
+  __shared__ double acc[16][16];
+  const int e = blockIdx.x * 16 + (threadIdx.x & 15);
   const int len = m + 3 - j;
   const int valid = (m - 1 - j) + (j > 0 ? 4 : 0);
-  const long ld = m + 3;
-  double s = 0.0;
-  if (r < valid) {
-    int w = grp;
-    for (; w + 28 < nwg; w += 32) {
-      double v[8];
-#pragma unroll
-      for (int q = 0; q < 8; q++) v[q] = partial[(long)(w + 4 * q) * ld + r];
-#pragma unroll
-      for (int q = 0; q < 8; q++) s += v[q];
-    }
-    for (; w < nwg; w += 4) s += partial[(long)w * ld + r];
-  }
-  acc[grp][lane] = s;
-  __syncthreads();
-  if (grp == 0 && r < len) msg[r] = ((acc[0][lane] + acc[1][lane]) + acc[2][lane]) + acc[3][lane];
+  const double v = partials_sum<false>(partial, m + 3, nwg, e, e < valid, acc);
+  if (threadIdx.x < 16 && e < len) msg[e] = v;
 }
 
 // dot = f_i . u_{i-1} over the columns (column-major F and U)
@@ -771,7 +826,7 @@ int ralha_step(bool generic, int i, int m, int c, double* d_fc, double* d_uc, co
   }
   if (rc) return rc;
   if (i + 1 <= m - 3) {
-    hipLaunchKernelGGL(k_ralha_reduce, dim3(exc_detail::ceil_div(m + 3, 64)), dim3(256), 0, s, i + 1, m, (int)nwg,
+    hipLaunchKernelGGL(k_ralha_reduce, dim3(exc_detail::ceil_div(m + 3, 16)), dim3(256), 0, s, i + 1, m, (int)nwg,
                        partial, d_msg);
     EXC_CHECK_HIP(hipGetLastError());
   }

@@ -62,10 +62,11 @@ def test_ralha_vs_oracle(hip, oracle, P, m, cols, generic):
     F = [Rf[:, sum(cols[:p]):sum(cols[:p + 1])].copy() for p in range(P)]
     got = hip.ralha_peers(F, generic=generic)
     want = oracle.ralha(F)
+    tol = max(1e-11, 1e-13 * m)  # m dependent steps; the last reflectors act on 2-4 entries
     for p in range(P):
         for key in ("alpha", "beta", "tau", "U"):
             a, b = got[p][key], want[p][key]
-            assert np.max(np.abs(a - b)) <= 1e-11 * max(1.0, np.abs(b).max()), key
+            assert np.max(np.abs(a - b)) <= tol * max(1.0, np.abs(b).max()), key
         ho, hg = want[p]["house"], got[p]["house"]
         # reflector columns (below the diagonal) of columns with tau != 0
         for j in range(1, m - 1):
