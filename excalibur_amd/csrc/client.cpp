@@ -6,6 +6,7 @@
 
 #include <hip/hip_runtime.h>
 #include <sys/stat.h>
+#include <unistd.h>
 
 #include <algorithm>
 #include <cmath>
@@ -93,7 +94,15 @@ struct ClientDevice {
   HBuf h_masked, h_repart, h_msg, h_U;
   void* link = nullptr;  // message link of the one-launch Ralha loop (coherent pinned memory)
   int link_m = 0;
+  // Device-memory message exchange among peers on one device: -1 undecided, 0 off, 1 on.
+  int boxes_state = -1;
+  void* box = nullptr;                // this peer's box
+  std::vector<void*> boxes;           // every peer's box, opened through IPC (own box at cid)
+  std::vector<int> box_nwg;           // every peer's loop grid size
   ~ClientDevice() {
+    for (size_t q = 0; q < boxes.size(); q++)
+      if (boxes[q] && boxes[q] != box) exc_ipc_close(boxes[q]);
+    if (box) exc_free(box);
     if (link) exc_ralha_link_free(link);
     if (s) hipStreamDestroy(s);
   }
@@ -202,6 +211,83 @@ DecentralizedClient::~DecentralizedClient() = default;
 void DecentralizedClient::log(const std::string& line) {
   std::ofstream f(log_dir + "/Client" + std::to_string(client_id) + ".log", std::ios::app);
   f << line << "\n";
+}
+
+void DecentralizedClient::decide_device_exchange(int mi, int ci, bool one_launch) {
+  exc::ClientDevice& D = *dev_;
+  exc::PeerNet& net = *net_;
+  const int P = num_client, cid = client_id;
+  // this peer's record: the loop applies, its grid size, and the identity of its device
+  struct Rec {
+    int usable, nwg;
+    char where[160];
+  } me{};
+  int dev = 0;
+  char bus[64] = {0};
+  ckh(hipGetDevice(&dev), "device");
+  ckh(hipDeviceGetPCIBusId(bus, sizeof bus, dev), "device");
+  char host[64] = {0};
+  gethostname(host, sizeof host - 1);
+  std::string boot;
+  {
+    std::ifstream f("/proc/sys/kernel/random/boot_id");
+    std::getline(f, boot);
+  }
+  snprintf(me.where, sizeof me.where, "%s|%s|%s", host, boot.c_str(), bus);
+  me.usable = (opt_.ralha_device_exchange && one_launch && P <= 16) ? 1 : 0;
+  me.nwg = one_launch ? exc_ralha_loop_grid(mi, ci, 0) : 0;
+  net.broadcast(&me, sizeof me);
+  std::vector<Rec> all(P);
+  all[cid] = me;
+  for (int j = 0; j < P; j++)
+    if (j != cid) net.receive_into(j, &all[j], sizeof(Rec));
+  bool ok = true;
+  for (int j = 0; j < P; j++)
+    ok = ok && all[j].usable == 1 && all[j].nwg > 0 && strncmp(all[j].where, me.where, sizeof me.where) == 0;
+  if (!ok) {
+    D.boxes_state = 0;
+    return;
+  }
+  // every peer allocates its box and sends the IPC handle to the others
+  ck(exc_malloc(&D.box, exc_ralha_box_bytes(mi)), "ralha box");
+  char handle[EXC_IPC_HANDLE_BYTES];
+  ck(exc_ipc_handle(D.box, handle), "ralha box handle");
+  net.broadcast(handle, sizeof handle);
+  D.boxes.assign(P, nullptr);
+  D.box_nwg.assign(P, 0);
+  D.boxes[cid] = D.box;
+  int opened = 1;
+  for (int j = 0; j < P; j++) {
+    D.box_nwg[j] = all[j].nwg;
+    if (j == cid) continue;
+    char h[EXC_IPC_HANDLE_BYTES];
+    net.receive_into(j, h, sizeof h);
+    if (opened && exc_ipc_open(h, &D.boxes[j]) != EXC_OK) {
+      log(std::string("could not open a peer's box: ") + exc_last_error());
+      D.boxes[j] = nullptr;
+      opened = 0;
+    }
+  }
+  // all peers switch to the device exchange, or none does
+  net.broadcast(&opened, sizeof opened);
+  int all_opened = opened;
+  for (int j = 0; j < P; j++) {
+    if (j == cid) continue;
+    int o = 0;
+    net.receive_into(j, &o, sizeof o);
+    all_opened &= o;
+  }
+  if (!all_opened) {
+    for (int j = 0; j < P; j++)
+      if (j != cid && D.boxes[j]) exc_ipc_close(D.boxes[j]);
+    D.boxes.clear();
+    exc_free(D.box);
+    D.box = nullptr;
+    D.boxes_state = 0;
+    return;
+  }
+  D.boxes_state = 1;
+  log("bidiagonalization messages exchanged in device memory among " + std::to_string(P) + " peers");
 }
 
 void DecentralizedClient::load_data() {
@@ -441,8 +527,11 @@ void DecentralizedClient::run() {
     ck(exc_ralha_tail_update(-1, mi, ci, D.F.p, D.U.p, nullptr, D.beta.p, D.scal.p, s), "ralha init");
     allreduce_dev(D.scal.p, D.scal.p + 1, 1);
     ck(exc_ralha_tail_scale(0, mi, ci, D.F.p, D.U.p, D.scal.p + 1, D.alpha.p, s), "ralha init");
-    if (opt_.ralha_one_launch && exc_ralha_loop_fits(mi, ci, 0) == 1) {
-      // steps -1 .. m-3 in one launch; this thread only all-reduces the messages it publishes
+    const bool one_launch = opt_.ralha_one_launch && exc_ralha_loop_fits(mi, ci, 0) == 1;
+    if (D.boxes_state < 0) decide_device_exchange(mi, ci, one_launch);
+    if (one_launch) {
+      // steps -1 .. m-3 in one launch; the messages go through device-memory boxes when every
+      // peer shares this device, else this thread all-reduces them over the network
       const double tmo = std::max(30.0, opt_.recv_timeout_s);
       D.work.alloc((exc_ralha_loop_workspace(mi, ci) + 7) / 8, s);
       if (!D.link || D.link_m != mi) {
@@ -451,17 +540,39 @@ void DecentralizedClient::run() {
         ck(exc_ralha_link_alloc(mi, &D.link), "ralha link");
         D.link_m = mi;
       }
-      ck(exc_ralha_loop_start(mi, ci, D.F.p, D.U.p, D.house.p, D.tau_h.p, D.alpha.p, D.beta.p, D.link, D.work.p,
-                              D.work.n * 8, 0, tmo, s),
-         "ralha loop");
+      if (D.boxes_state == 1) {
+        ck(exc_ralha_box_clear(D.box, s), "ralha box");
+        sync();
+        net.barrier();  // every peer's box counter is zero before any loop starts
+        ck(exc_ralha_loop_start_boxes(mi, ci, D.F.p, D.U.p, D.house.p, D.tau_h.p, D.alpha.p, D.beta.p, D.link,
+                                      D.work.p, D.work.n * 8, 0, tmo, (int)P, (int)cid, D.boxes.data(),
+                                      D.box_nwg.data(), s),
+           "ralha loop");
+      } else {
+        ck(exc_ralha_loop_start(mi, ci, D.F.p, D.U.p, D.house.p, D.tau_h.p, D.alpha.p, D.beta.p, D.link, D.work.p,
+                                D.work.n * 8, 0, tmo, s),
+           "ralha loop");
+      }
       try {
-        for (int j = 0; j + 2 < mi; j++) {
-          double* msg = nullptr;
-          ck(exc_ralha_loop_wait(D.link, j, tmo, &msg), "ralha loop");
-          const double ta = now_seconds();
-          net.ring_all_reduce_sum(msg, (long)m + 3 - j);
-          ar += now_seconds() - ta;
-          ck(exc_ralha_loop_post(D.link, j), "ralha loop");
+        if (D.boxes_state == 1) {
+          const double t0w = now_seconds();
+          while (true) {
+            const int r = exc_ralha_loop_running(s);
+            if (r < 0) ck(r, "ralha loop");
+            if (r == 0) break;
+            if (net.any_closed()) throw std::runtime_error("a peer disconnected during the bidiagonalization");
+            if (now_seconds() - t0w > tmo) throw std::runtime_error("the bidiagonalization loop timed out");
+            usleep(50);
+          }
+        } else {
+          for (int j = 0; j + 2 < mi; j++) {
+            double* msg = nullptr;
+            ck(exc_ralha_loop_wait(D.link, j, tmo, &msg), "ralha loop");
+            const double ta = now_seconds();
+            net.ring_all_reduce_sum(msg, (long)m + 3 - j);
+            ar += now_seconds() - ta;
+            ck(exc_ralha_loop_post(D.link, j), "ralha loop");
+          }
         }
       } catch (...) {
         exc_ralha_loop_abort(D.link);

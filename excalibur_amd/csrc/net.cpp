@@ -262,6 +262,21 @@ void PeerNet::receive_into(int peer, void* buf, size_t bytes) {
   if (bytes) memcpy(buf, m.data(), bytes);
 }
 
+bool PeerNet::any_closed() {
+  for (int j = 0; j < P_; j++) {
+    if (j == cid_) continue;
+    std::lock_guard<std::mutex> g(inbox_[j]->mu);
+    if (inbox_[j]->closed) return true;
+  }
+  return false;
+}
+
+void PeerNet::barrier() {
+  broadcast(nullptr, 0);
+  for (int j = 0; j < P_; j++)
+    if (j != cid_) receive_into(j, nullptr, 0);
+}
+
 void PeerNet::broadcast(const void* buf, size_t bytes) {
   for (int j = 0; j < P_; j++)
     if (j != cid_) send_to(j, buf, bytes);

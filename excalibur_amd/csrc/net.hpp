@@ -42,6 +42,11 @@ class PeerNet {
   // the receivers).
   void all_gather(std::vector<char>& buf, size_t bytes, const std::vector<int>& grp, int src);
 
+  // True once the connection from any peer has closed (the peer ended or failed).
+  bool any_closed();
+  // Every peer has reached this point (one empty frame to and from each peer).
+  void barrier();
+
   double comm_seconds() const { return comm_s_; }
   void add_comm_seconds(double s) { comm_s_ += s; }
   void reset_comm() { comm_s_ = 0.0; }

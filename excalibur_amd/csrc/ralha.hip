@@ -179,22 +179,21 @@ __device__ __forceinline__ void reg_apply(RegCols<KM, CPW>& R, int i, int m, int
   double b, a;
   const double t = ralha_reflector(i, m, g, hbuf, red, house, tau, alpha, beta, &b, &a);
   if (t != 0.0) {
-    double h[KM];
-#pragma unroll
-    for (int q = 0; q < KM; q++) {
+    // h from LDS (zero outside rows i+1 .. m-1); read twice rather than held in KM registers
+    auto h = [&](int q) {
       const int row = lane + 64 * q;
-      h[q] = (row > i && row < m) ? hbuf[row - i - 1] : 0.0;
-    }
+      return (row > i && row < m) ? hbuf[row - i - 1] : 0.0;
+    };
     double s[CPW];
 #pragma unroll
     for (int cc = 0; cc < CPW; cc++) {
       double p0 = 0.0, p1 = 0.0, p2 = 0.0, p3 = 0.0;
 #pragma unroll
       for (int q = 0; q < KM; q += 4) {
-        p0 += R.fr[cc][q] * h[q];
-        if (q + 1 < KM) p1 += R.fr[cc][q + 1] * h[q + 1];
-        if (q + 2 < KM) p2 += R.fr[cc][q + 2] * h[q + 2];
-        if (q + 3 < KM) p3 += R.fr[cc][q + 3] * h[q + 3];
+        p0 += R.fr[cc][q] * h(q);
+        if (q + 1 < KM) p1 += R.fr[cc][q + 1] * h(q + 1);
+        if (q + 2 < KM) p2 += R.fr[cc][q + 2] * h(q + 2);
+        if (q + 3 < KM) p3 += R.fr[cc][q + 3] * h(q + 3);
       }
       s[cc] = (p0 + p1) + (p2 + p3);
     }
@@ -209,7 +208,7 @@ __device__ __forceinline__ void reg_apply(RegCols<KM, CPW>& R, int i, int m, int
 #pragma unroll
     for (int q = 0; q < KM; q++) {
       const int row = lane + 64 * q;
-      const double th = -t * h[q];
+      const double th = -t * h(q);
 #pragma unroll
       for (int cc = 0; cc < CPW; cc++)
         if (R.cols[cc] < c && row > i && row < m) R.fr[cc][q] += s[cc] * th;
@@ -373,6 +372,23 @@ struct LoopLink {  // pinned, coherent host memory
 };
 static_assert(sizeof(LoopLink) <= kLinkMsgOffset, "link header");
 
+// Peers that share the device can exchange the messages through device memory instead of the
+// host: every peer owns a box (a counter, then two message slots used by step parity), opened by
+// the other peers' processes through IPC handles. Peer p's workgroups publish their slices of
+// message j into slot j % 2 of p's box and add 1 to its counter; message j of peer q is complete
+// when q's counter reaches nwg[q] (j + 1). The reduced message is then, on every peer, its own
+// message plus the others' in peer order, the order of the host all-reduce
+// (client.cpp:313-327), so both exchanges give the same bits. Two slots suffice: a peer writes
+// message j + 2 only after every peer has published message j + 1, which each does only after
+// reading all of message j.
+constexpr int kMaxPeers = 16;
+constexpr int kBoxMsgOffset = 256;
+struct PeerBoxes {  // kernel argument; P = 0 selects the host link
+  int P, me;
+  int nwg[kMaxPeers];
+  char* box[kMaxPeers];
+};
+
 enum { kGin = 0, kBarrier = 1, kStop = 2 };  // device words
 
 __device__ __forceinline__ double ld_dev(const double* p) {
@@ -526,12 +542,104 @@ __device__ __forceinline__ bool loop_receive(int i, int m, LoopLink* link, const
   return true;
 }
 
+// Message j through the boxes: the slices go to this peer's box, then its counter is raised.
+__device__ __forceinline__ bool box_publish(int j, int m, const double* __restrict__ partial, const PeerBoxes& B,
+                                            LoopLink* link, int* dsync, long long limit, int* sh,
+                                            double (*acc)[16]) {
+  const int G = gridDim.x, w = blockIdx.x;
+  stores_done();
+  __syncthreads();
+  if (threadIdx.x == 0) __hip_atomic_fetch_add(&dsync[kBarrier], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (!loop_wait_dev(&dsync[kBarrier], G * (j + 1), dsync, link, limit, sh)) return false;
+  const int len = m + 3 - j;
+  const int valid = (m - 1 - j) + (j > 0 ? 4 : 0);
+  const int per = (len + G - 1) / G;
+  const int r0 = w * per, r1 = min(len, r0 + per);
+  double* slot = (double*)(B.box[B.me] + kBoxMsgOffset) + (long)(j & 1) * (m + 3);
+  for (int base = r0; base < r1; base += 16) {
+    const int e = base + (threadIdx.x & 15);
+    const double v = partials_sum<true>(partial, m + 3, G, e, e < r1 && e < valid, acc);
+    if (threadIdx.x < 16 && e < r1) st_dev(&slot[e], v);
+  }
+  stores_done();
+  __syncthreads();
+  if (threadIdx.x == 0)
+    __hip_atomic_fetch_add((int*)B.box[B.me], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return true;
+}
+
+// Waits until every peer has published message i, then forms the reduced message in gl.
+template <int KM>
+__device__ __forceinline__ bool box_receive(int i, int m, const PeerBoxes& B, LoopLink* link, double* gl, int* dsync,
+                                            long long limit, int* sh) {
+  if (threadIdx.x == 0) {
+    int ok = 1;
+    const long long t0 = wall_clock64();
+    for (int q = 0; q < B.P && ok; q++) {
+      const int* cnt = (const int*)B.box[q];
+      const int target = B.nwg[q] * (i + 1);
+      for (unsigned k = 0; __hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target; k++) {
+        if ((k & 15) == 15) {
+          if (__hip_atomic_load(&dsync[kStop], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+            ok = 0;
+            break;
+          }
+          if (__hip_atomic_load(&link->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) != 0) {
+            loop_fail(dsync, link, 2);
+            ok = 0;
+            break;
+          }
+          if (wall_clock64() - t0 > limit) {
+            loop_fail(dsync, link, 1);
+            ok = 0;
+            break;
+          }
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+    }
+    sh[0] = ok;
+  }
+  __syncthreads();
+  const bool ok = sh[0] != 0;
+  __syncthreads();
+  if (!ok) return false;
+  const int len = m + 3 - i;
+  const long off = (long)(i & 1) * (m + 3);
+  constexpr int kPer = (64 * KM + 3 + 255) / 256;
+  double v[kPer];
+#pragma unroll
+  for (int qq = 0; qq < kPer; qq++) {
+    const int r = threadIdx.x + 256 * qq;
+    v[qq] = r < len ? ld_dev((const double*)(B.box[B.me] + kBoxMsgOffset) + off + r) : 0.0;
+  }
+  for (int q = 0; q < B.P; q++) {
+    if (q == B.me) continue;
+    const double* x = (const double*)(B.box[q] + kBoxMsgOffset) + off;
+    double y[kPer];
+#pragma unroll
+    for (int qq = 0; qq < kPer; qq++) {
+      const int r = threadIdx.x + 256 * qq;
+      y[qq] = r < len ? ld_dev(x + r) : 0.0;
+    }
+#pragma unroll
+    for (int qq = 0; qq < kPer; qq++) v[qq] = v[qq] + y[qq];
+  }
+#pragma unroll
+  for (int qq = 0; qq < kPer; qq++) {
+    const int r = threadIdx.x + 256 * qq;
+    if (r < len) gl[r] = v[qq];
+  }
+  __syncthreads();
+  return true;
+}
+
 template <int KM, int CPW>
-__global__ __launch_bounds__(256) void k_ralha_loop(int m, int c, double* __restrict__ Fc, double* __restrict__ Uc,
+__global__ __launch_bounds__(256, KM * CPW <= 32 ? 2 : 1) void k_ralha_loop(int m, int c, double* __restrict__ Fc, double* __restrict__ Uc,
                                                     double* __restrict__ house, double* __restrict__ tau,
                                                     double* __restrict__ alpha, double* __restrict__ beta,
                                                     char* link_bytes, int* dsync, double* __restrict__ gdev,
-                                                    double* __restrict__ partial, long long limit) {
+                                                    double* __restrict__ partial, long long limit, PeerBoxes B) {
   extern __shared__ double lds[];  // hbuf: m, mp: 4 x (m + 4), gl: m + 3
   double* hbuf = lds;
   double* mp = lds + m;
@@ -544,16 +652,23 @@ __global__ __launch_bounds__(256) void k_ralha_loop(int m, int c, double* __rest
   double* mine = partial + (long)blockIdx.x * (m + 3);
   RegCols<KM, CPW> R;
   reg_load(R, -1, m, c, Fc, Uc);
+  const bool boxes = B.P > 0;  // uniform
   reg_share<KM, CPW, true>(R, 0, m, c, mp, mine);
-  if (!loop_publish(0, m, partial, link, hmsg, dsync, limit, sh, acc)) return;
+  if (!(boxes ? box_publish(0, m, partial, B, link, dsync, limit, sh, acc)
+              : loop_publish(0, m, partial, link, hmsg, dsync, limit, sh, acc)))
+    return;
   for (int i = 0; i <= m - 3; i++) {
-    if (!loop_receive<KM>(i, m, link, hmsg, gdev, gl, dsync, limit, sh)) return;
+    if (!(boxes ? box_receive<KM>(i, m, B, link, gl, dsync, limit, sh)
+                : loop_receive<KM>(i, m, link, hmsg, gdev, gl, dsync, limit, sh)))
+      return;
     reg_apply(R, i, m, c, gl, hbuf, red, Uc, house, tau, alpha, beta);
 #pragma unroll
     for (int cc = 0; cc < CPW; cc++) R.um1[cc] = R.ui[cc];
     if (i + 1 <= m - 3) {
       reg_share<KM, CPW, true>(R, i + 1, m, c, mp, mine);
-      if (!loop_publish(i + 1, m, partial, link, hmsg, dsync, limit, sh, acc)) return;
+      if (!(boxes ? box_publish(i + 1, m, partial, B, link, dsync, limit, sh, acc)
+                  : loop_publish(i + 1, m, partial, link, hmsg, dsync, limit, sh, acc)))
+        return;
     }
   }
   reg_store(R, 0, m, c, Fc);
@@ -947,9 +1062,17 @@ int exc_ralha_loop_fits(int m, int c, int cols_per_wave) {
   return fits ? 1 : 0;
 }
 
-int exc_ralha_loop_start(int m, int c, double* d_fc, double* d_uc, double* d_house, double* d_tau, double* d_alpha,
-                         double* d_beta, void* h_link, void* d_work, size_t work_bytes, int cols_per_wave,
-                         double timeout_s, exc_stream_t stream) {
+}  // extern "C"
+
+namespace {
+
+__global__ void k_box_clear(int* box) {
+  if (threadIdx.x == 0) box[0] = 0;
+}
+
+int loop_start(int m, int c, double* d_fc, double* d_uc, double* d_house, double* d_tau, double* d_alpha,
+               double* d_beta, void* h_link, void* d_work, size_t work_bytes, int cols_per_wave, double timeout_s,
+               const PeerBoxes& B, exc_stream_t stream) {
   EXC_REQUIRE(m >= 3 && c >= 1 && d_fc && d_uc && d_house && d_tau && d_alpha && d_beta && h_link && d_work &&
                   timeout_s > 0,
               EXC_ERR_ARG, "exc_ralha_loop_start: bad arguments");
@@ -962,6 +1085,8 @@ int exc_ralha_loop_start(int m, int c, double* d_fc, double* d_uc, double* d_hou
   const int rc = loop_resident(sh, fn, &fits);
   if (rc) return rc;
   EXC_REQUIRE(fits, EXC_ERR_ARG, "exc_ralha_loop_start: the grid would not be resident (see exc_ralha_loop_fits)");
+  if (B.P > 0)
+    EXC_REQUIRE(B.nwg[B.me] == sh.nwg, EXC_ERR_ARG, "exc_ralha_loop_start_boxes: nwg[me] is not this grid's size");
   hipStream_t s = exc_detail::stream(stream);
   EXC_CHECK_HIP(hipStreamSynchronize(s));  // no earlier loop still uses the link
   memset(link, 0, kLinkMsgOffset);
@@ -980,9 +1105,83 @@ int exc_ralha_loop_start(int m, int c, double* d_fc, double* d_uc, double* d_hou
   double* gdev = (double*)(w + 256);
   double* partial = (double*)(w + 256 + ((size_t)(m + 3) * sizeof(double) + 255) / 256 * 256);
   hipLaunchKernelGGL(k_loop_clear, dim3(1), dim3(64), 0, s, dsync);
+  PeerBoxes b = B;
   void* args[] = {&m, &c, &d_fc, &d_uc, &d_house, &d_tau, &d_alpha, &d_beta, &d_link, &dsync, &gdev, &partial,
-                  (void*)&limit};
+                  (void*)&limit, &b};
   EXC_CHECK_HIP(hipLaunchKernel(fn, dim3(sh.nwg), dim3(256), args, sh.lds, s));
+  return EXC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int exc_ralha_loop_start(int m, int c, double* d_fc, double* d_uc, double* d_house, double* d_tau, double* d_alpha,
+                         double* d_beta, void* h_link, void* d_work, size_t work_bytes, int cols_per_wave,
+                         double timeout_s, exc_stream_t stream) {
+  PeerBoxes none{};
+  return loop_start(m, c, d_fc, d_uc, d_house, d_tau, d_alpha, d_beta, h_link, d_work, work_bytes, cols_per_wave,
+                    timeout_s, none, stream);
+}
+
+int exc_ralha_loop_grid(int m, int c, int cols_per_wave) {
+  return exc_ralha_loop_fits(m, c, cols_per_wave) ? loop_shape(m, c, cols_per_wave).nwg : 0;
+}
+
+size_t exc_ralha_box_bytes(int m) { return m < 3 ? 0 : kBoxMsgOffset + 2 * (size_t)(m + 3) * sizeof(double); }
+
+int exc_ralha_box_clear(void* d_box, exc_stream_t stream) {
+  EXC_REQUIRE(d_box, EXC_ERR_ARG, "exc_ralha_box_clear: null box");
+  hipLaunchKernelGGL(k_box_clear, dim3(1), dim3(64), 0, exc_detail::stream(stream), (int*)d_box);
+  EXC_CHECK_HIP(hipGetLastError());
+  return EXC_OK;
+}
+
+int exc_ralha_loop_start_boxes(int m, int c, double* d_fc, double* d_uc, double* d_house, double* d_tau,
+                               double* d_alpha, double* d_beta, void* h_link, void* d_work, size_t work_bytes,
+                               int cols_per_wave, double timeout_s, int npeers, int me, void* const* d_boxes,
+                               const int* nwg, exc_stream_t stream) {
+  EXC_REQUIRE(npeers >= 1 && npeers <= kMaxPeers && me >= 0 && me < npeers && d_boxes && nwg, EXC_ERR_ARG,
+              "exc_ralha_loop_start_boxes: bad peer arguments (1 <= npeers <= 16)");
+  PeerBoxes B{};
+  B.P = npeers;
+  B.me = me;
+  for (int q = 0; q < npeers; q++) {
+    EXC_REQUIRE(d_boxes[q] && nwg[q] >= 1 && nwg[q] <= kLoopMaxWg, EXC_ERR_ARG,
+                "exc_ralha_loop_start_boxes: bad box or grid size");
+    B.box[q] = (char*)d_boxes[q];
+    B.nwg[q] = nwg[q];
+  }
+  return loop_start(m, c, d_fc, d_uc, d_house, d_tau, d_alpha, d_beta, h_link, d_work, work_bytes, cols_per_wave,
+                    timeout_s, B, stream);
+}
+
+int exc_ralha_loop_running(exc_stream_t stream) {
+  const hipError_t e = hipStreamQuery(exc_detail::stream(stream));
+  if (e == hipErrorNotReady) return 1;
+  EXC_CHECK_HIP(e);
+  return 0;
+}
+
+int exc_ipc_handle(void* d_ptr, void* handle) {
+  EXC_REQUIRE(d_ptr && handle, EXC_ERR_ARG, "exc_ipc_handle: bad arguments");
+  static_assert(sizeof(hipIpcMemHandle_t) == EXC_IPC_HANDLE_BYTES, "IPC handle size");
+  hipIpcMemHandle_t h;
+  EXC_CHECK_HIP(hipIpcGetMemHandle(&h, d_ptr));
+  memcpy(handle, &h, sizeof h);
+  return EXC_OK;
+}
+
+int exc_ipc_open(const void* handle, void** d_ptr) {
+  EXC_REQUIRE(handle && d_ptr, EXC_ERR_ARG, "exc_ipc_open: bad arguments");
+  hipIpcMemHandle_t h;
+  memcpy(&h, handle, sizeof h);
+  EXC_CHECK_HIP(hipIpcOpenMemHandle(d_ptr, h, hipIpcMemLazyEnablePeerAccess));
+  return EXC_OK;
+}
+
+int exc_ipc_close(void* d_ptr) {
+  if (d_ptr) EXC_CHECK_HIP(hipIpcCloseMemHandle(d_ptr));
   return EXC_OK;
 }
 

@@ -79,6 +79,15 @@ SIGNATURES = {
     "exc_ralha_loop_post": (_I, [_vp, _I]),
     "exc_ralha_loop_abort": (_I, [_vp]),
     "exc_ralha_loop_finish": (_I, [_vp, _vp]),
+    "exc_ralha_loop_grid": (_I, [_I, _I, _I]),
+    "exc_ralha_box_bytes": (_S, [_I]),
+    "exc_ralha_box_clear": (_I, [_dp, _vp]),
+    "exc_ralha_loop_start_boxes": (_I, [_I, _I, _dp, _dp, _dp, _dp, _dp, _dp, _vp, _vp, _S, _I, _D, _I, _I,
+                                        ctypes.POINTER(_vp), ctypes.POINTER(_I), _vp]),
+    "exc_ralha_loop_running": (_I, [_vp]),
+    "exc_ipc_handle": (_I, [_dp, _vp]),
+    "exc_ipc_open": (_I, [_vp, ctypes.POINTER(_vp)]),
+    "exc_ipc_close": (_I, [_dp]),
     "exc_ralha_dot": (_I, [_I, _I, _I, _dp, _dp, _dp, _vp]),
     "exc_ralha_tail_update": (_I, [_I, _I, _I, _dp, _dp, _dp, _dp, _dp, _vp]),
     "exc_ralha_tail_scale": (_I, [_I, _I, _I, _dp, _dp, _dp, _dp, _vp]),
@@ -372,13 +381,14 @@ def ormqr_house_ln(house, tau, C):
     return dC.to_host()
 
 
-def ralha_peers(F_list, generic=False, loop=False, cols_per_wave=0, timeout_s=30.0):
+def ralha_peers(F_list, generic=False, loop=False, cols_per_wave=0, timeout_s=30.0, boxes=False):
     """The overlapped Ralha loop over P simulated peers in this process: GPU kernels per peer,
     the all-reduce done here in each peer's own order (own buffer, then the others by id).
     F_list[p] is peer p's m x c_p share of the rows f_j; U is returned in the same orientation.
     generic selects the streamed (HBM) form of the step kernel instead of the register form;
     loop runs steps -1 .. m-3 as one launch per peer (one stream each, all running at once) with
-    this thread all-reducing the messages through the link buffers."""
+    this thread all-reducing the messages through the link buffers; with boxes the launches
+    exchange the messages through device-memory boxes instead and this thread only waits."""
     P = len(F_list)
     m = F_list[0].shape[0]
     cs = [f.shape[1] for f in F_list]
@@ -423,7 +433,7 @@ def ralha_peers(F_list, generic=False, loop=False, cols_per_wave=0, timeout_s=30
         if not loop:
             step(-1, p)
     if loop:
-        _ralha_loop_run(m, cs, dF, dU, dh, dt, da, db, cols_per_wave, timeout_s)
+        _ralha_loop_run(m, cs, dF, dU, dh, dt, da, db, cols_per_wave, timeout_s, boxes)
     else:
         for i in range(m - 2):
             sync()
@@ -455,14 +465,23 @@ def ralha_peers(F_list, generic=False, loop=False, cols_per_wave=0, timeout_s=30
                  U=np.ascontiguousarray(dU[p].to_host().T)) for p in range(P)]
 
 
-def _ralha_loop_run(m, cs, dF, dU, dh, dt, da, db, cols_per_wave, timeout_s):
-    """Steps -1 .. m-3 of every peer as one launch each, on P streams; messages all-reduced here."""
+def _ralha_loop_run(m, cs, dF, dU, dh, dt, da, db, cols_per_wave, timeout_s, boxes=False):
+    """Steps -1 .. m-3 of every peer as one launch each, on P streams; messages all-reduced here
+    (or, with boxes, exchanged by the launches themselves through device-memory boxes)."""
+    import time
     P = len(cs)
     sync()
     for c in cs:
         if not _lib.exc_ralha_loop_fits(m, c, cols_per_wave):
             raise HipError(-1, "exc_ralha_loop_fits")
     streams, links, ws = [], [], []
+    if boxes:
+        box = [DeviceArray((_lib.exc_ralha_box_bytes(m),), np.uint8) for _ in range(P)]
+        for b in box:
+            call("exc_ralha_box_clear", b.ptr, None)
+        sync()
+        box_ptrs = (_vp * P)(*[b.ptr for b in box])
+        nwg = (_I * P)(*[_lib.exc_ralha_loop_grid(m, c, cols_per_wave) for c in cs])
     try:
         for p in range(P):
             st = _vp()
@@ -475,10 +494,20 @@ def _ralha_loop_run(m, cs, dF, dU, dh, dt, da, db, cols_per_wave, timeout_s):
         started = []
         try:
             for p in range(P):
-                call("exc_ralha_loop_start", m, cs[p], dF[p].ptr, dU[p].ptr, dh[p].ptr, dt[p].ptr, da[p].ptr,
-                     db[p].ptr, links[p], ws[p].ptr, ws[p].nbytes, cols_per_wave, timeout_s, streams[p])
+                if boxes:
+                    call("exc_ralha_loop_start_boxes", m, cs[p], dF[p].ptr, dU[p].ptr, dh[p].ptr, dt[p].ptr,
+                         da[p].ptr, db[p].ptr, links[p], ws[p].ptr, ws[p].nbytes, cols_per_wave, timeout_s, P, p,
+                         box_ptrs, nwg, streams[p])
+                else:
+                    call("exc_ralha_loop_start", m, cs[p], dF[p].ptr, dU[p].ptr, dh[p].ptr, dt[p].ptr, da[p].ptr,
+                         db[p].ptr, links[p], ws[p].ptr, ws[p].nbytes, cols_per_wave, timeout_s, streams[p])
                 started.append(p)
-            for j in range(m - 2):
+            t0 = time.time()
+            while boxes and any(_lib.exc_ralha_loop_running(st) == 1 for st in streams):
+                if time.time() - t0 > timeout_s:
+                    raise HipError(-4, "exc_ralha_loop_running")
+                time.sleep(2e-4)
+            for j in range(0 if not boxes else m - 2, m - 2):
                 n = m + 3 - j
                 views = []
                 for p in range(P):

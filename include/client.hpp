@@ -38,6 +38,7 @@ struct ClientOptions {
   int repeat = 1;                 // run() this many times on the same input (benchmarking)
   bool profile = false;           // time the MFMA GEMM / Givens launches with HIP events
   bool ralha_one_launch = true;   // the bidiagonalization loop as one kernel launch where it fits
+  bool ralha_device_exchange = true;  // peers sharing the device exchange its messages in device memory
 };
 
 class DecentralizedClient {
@@ -77,6 +78,9 @@ class DecentralizedClient {
 
  private:
   void log(const std::string& line);
+  // Once per client: whether all peers share this device (and each runs the one-launch
+  // bidiagonalization), in which case they exchange IPC handles of their message boxes.
+  void decide_device_exchange(int m, int c, bool one_launch);
   ClientOptions opt_;
   std::unique_ptr<exc::PeerNet> net_;
   std::unique_ptr<exc::ClientDevice> dev_;

@@ -231,6 +231,30 @@ int exc_ralha_loop_wait(void* h_link, int j, double timeout_s, double** h_msg);
 int exc_ralha_loop_post(void* h_link, int j);
 int exc_ralha_loop_abort(void* h_link);
 int exc_ralha_loop_finish(void* h_link, exc_stream_t stream);
+/* Number of workgroups of this peer's loop grid (0 when exc_ralha_loop_fits is 0). */
+int exc_ralha_loop_grid(int m, int c, int cols_per_wave);
+/* The same loop for peers that share one device: the messages go through per-peer boxes in
+ * device memory (exc_ralha_box_bytes(m) each; peer q's box opened in the other peers' processes
+ * with exc_ipc_open), and the host does nothing per step. Every peer q's box counter must have
+ * been cleared (exc_ralha_box_clear, then a barrier among the peers) before any peer starts;
+ * nwg[q] = exc_ralha_loop_grid of peer q. The reduced message is each peer's own plus the others'
+ * in peer order, as the host all-reduce adds them, so the results equal the host-link loop's.
+ * The host waits with exc_ralha_loop_running (1 while the launch runs) and ends with
+ * exc_ralha_loop_finish; exc_ralha_loop_abort ends it early (e.g. when a peer's connection
+ * drops). At most 16 peers. */
+size_t exc_ralha_box_bytes(int m);
+int exc_ralha_box_clear(void* d_box, exc_stream_t stream);
+int exc_ralha_loop_start_boxes(int m, int c, double* d_fc, double* d_uc, double* d_house, double* d_tau,
+                               double* d_alpha, double* d_beta, void* h_link, void* d_work, size_t work_bytes,
+                               int cols_per_wave, double timeout_s, int npeers, int me, void* const* d_boxes,
+                               const int* nwg, exc_stream_t stream);
+int exc_ralha_loop_running(exc_stream_t stream);
+/* Device memory shared between the processes of peers on one device (hipIpcGetMemHandle and
+ * hipIpcOpenMemHandle): a handle is EXC_IPC_HANDLE_BYTES opaque bytes. */
+#define EXC_IPC_HANDLE_BYTES 64
+int exc_ipc_handle(void* d_ptr, void* handle);
+int exc_ipc_open(const void* handle, void** d_ptr);
+int exc_ipc_close(void* d_ptr);
 /* The last two columns and the first (client.cpp:830-833, 897-905): d_out = f_i.u_{i-1};
  * tail_update: f_i -= beta u_{i-1} (beta = reduced value, stored in d_beta[i-1]), d_out =
  * ||f_i||^2 (i = -1: ||f_0||^2 only); tail_scale: alpha_i = sqrt(reduced), u_i = f_i / alpha_i. */

@@ -127,6 +127,24 @@ def test_ralha_loop_equals_step_kernels(hip, P, m, cols):
             assert np.array_equal(a[p][key], b[p][key]), key
 
 
+@pytest.mark.parametrize("P,m,cols", [(1, 12, [12]), (2, 60, [30, 30]), (3, 101, [34, 34, 33]),
+                                      (2, 600, [301, 299]), (2, 1100, [550, 550])])
+def test_ralha_loop_boxes_equal_host_link(hip, oracle, P, m, cols):
+    """The loops exchanging messages through device-memory boxes give the host-link loop's bits
+    (same partial sums, own message plus the others in peer order), and match the oracle."""
+    banner(f"ralha loop boxes P={P} m={m}")
+    F = _ralha_input(P, m, cols)
+    a = hip.ralha_peers(F, loop=True, boxes=True)
+    b = hip.ralha_peers(F, loop=True)
+    for p in range(P):
+        for key in ("alpha", "beta", "tau", "U", "house"):
+            assert np.array_equal(a[p][key], b[p][key]), key
+    want = oracle.ralha(F)
+    tol = max(1e-11, 1e-13 * m)
+    for p in range(P):
+        assert np.max(np.abs(a[p]["alpha"] - want[p]["alpha"])) <= tol * np.abs(want[p]["alpha"]).max()
+
+
 def _loop_setup(hip, m, c):
     import ctypes
     rng = np.random.default_rng(7)
@@ -281,6 +299,28 @@ def test_two_peer_run_step_kernels(hip, oracle):
     print(st)
     assert st["resid"] <= 1e-12 and st["orth_u"] <= 1e-12 and st["orth_v"] <= 1e-12
     assert st["du_separated"] <= 1e-9
+
+
+def test_run_device_exchange_equals_network_exchange(hip, oracle):
+    """Peers on one device exchange the bidiagonalization messages in device memory (IPC boxes);
+    with --ralha-device-exchange 0 they all-reduce them over TCP. Both add the same values in the
+    same order, so the results are identical."""
+    import os
+    from excalibur_amd.launch import run_peers
+    banner("3-peer run, device exchange vs network exchange")
+    rng = np.random.default_rng(12)
+    m = 70
+    xs = [rng.standard_normal((m, w)) for w in (200, 150, 250)]
+    a = run_peers(xs, seed=5)
+    b = run_peers(xs, seed=5, extra=["--ralha-device-exchange", "0"])
+    logs = open(os.path.join(a["workdir"], "logs", "Client0.log")).read()
+    assert "exchanged in device memory" in logs
+    assert "exchanged in device memory" not in open(os.path.join(b["workdir"], "logs", "Client0.log")).read()
+    for pa, pb in zip(a["peers"], b["peers"]):
+        for key in ("sigma", "FinalU", "FinalVT"):
+            assert np.array_equal(pa[key], pb[key]), key
+    st = compare_runs(a, oracle.Run(xs, [5, 5, 5]), xs)
+    assert st["resid"] <= 1e-12 and st["orth_u"] <= 1e-12 and st["orth_v"] <= 1e-12
 
 
 def test_three_peer_run_uneven(hip, oracle):
