@@ -99,7 +99,18 @@ struct ClientDevice {
   void* box = nullptr;                // this peer's box
   std::vector<void*> boxes;           // every peer's box, opened through IPC (own box at cid)
   std::vector<int> box_nwg;           // every peer's loop grid size
+  // Bulk exchange among peers on one device: this peer's outbox (device memory, exported by IPC)
+  // and every other peer's outbox as last opened here (by handle).
+  int colocated = 0;
+  double* outbox = nullptr;
+  size_t outbox_cap = 0;  // doubles
+  char outbox_handle[EXC_IPC_HANDLE_BYTES] = {0};
+  std::vector<std::string> peer_handle;
+  std::vector<void*> peer_outbox;
   ~ClientDevice() {
+    for (void* q : peer_outbox)
+      if (q) exc_ipc_close(q);
+    if (outbox) exc_free(outbox);
     for (size_t q = 0; q < boxes.size(); q++)
       if (boxes[q] && boxes[q] != box) exc_ipc_close(boxes[q]);
     if (box) exc_free(box);
@@ -107,6 +118,54 @@ struct ClientDevice {
     if (s) hipStreamDestroy(s);
   }
 };
+
+// Bulk transfers between peers on one device. The sender packs the data into its outbox and
+// sends a note (outbox handle, offset, count) over the network; the receiver copies from the
+// outbox, mapped into its process, device to device, then acknowledges, after which the sender
+// may reuse its outbox.
+struct DmaNote {
+  char handle[EXC_IPC_HANDLE_BYTES];
+  uint64_t offset, count;  // doubles
+};
+
+double* outbox_reserve(ClientDevice& D, size_t count) {
+  if (D.outbox_cap < count) {
+    // every earlier transfer from the outbox has been acknowledged
+    if (D.outbox) ck(exc_free(D.outbox), "outbox");
+    D.outbox = nullptr;
+    void* q = nullptr;
+    ck(exc_malloc(&q, count * sizeof(double)), "outbox");
+    D.outbox = (double*)q;
+    D.outbox_cap = count;
+    ck(exc_ipc_handle(D.outbox, D.outbox_handle), "outbox handle");
+  }
+  return D.outbox;
+}
+
+void dma_offer(PeerNet& net, ClientDevice& D, int to, size_t offset, size_t count) {
+  DmaNote n;
+  memcpy(n.handle, D.outbox_handle, sizeof n.handle);
+  n.offset = offset;
+  n.count = count;
+  net.send_to(to, &n, sizeof n);
+}
+
+const double* dma_take(PeerNet& net, ClientDevice& D, int from, size_t count) {
+  DmaNote n;
+  net.receive_into(from, &n, sizeof n);
+  if (n.count != count) throw std::runtime_error("device transfer of unexpected size");
+  std::string key(n.handle, sizeof n.handle);
+  if (D.peer_handle[from] != key) {
+    if (D.peer_outbox[from]) exc_ipc_close(D.peer_outbox[from]);
+    D.peer_outbox[from] = nullptr;
+    ck(exc_ipc_open(n.handle, &D.peer_outbox[from]), "open a peer's outbox");
+    D.peer_handle[from] = key;
+  }
+  return (const double*)D.peer_outbox[from] + n.offset;
+}
+
+void dma_ack(PeerNet& net, int to) { net.send_to(to, nullptr, 0); }
+void dma_wait_ack(PeerNet& net, int from) { net.receive_into(from, nullptr, 0); }
 
 }  // namespace exc
 
@@ -234,17 +293,23 @@ void DecentralizedClient::decide_device_exchange(int mi, int ci, bool one_launch
     std::getline(f, boot);
   }
   snprintf(me.where, sizeof me.where, "%s|%s|%s", host, boot.c_str(), bus);
-  me.usable = (opt_.ralha_device_exchange && one_launch && P <= 16) ? 1 : 0;
+  me.usable = (opt_.device_exchange && P <= 16) ? 1 : 0;
   me.nwg = one_launch ? exc_ralha_loop_grid(mi, ci, 0) : 0;
   net.broadcast(&me, sizeof me);
   std::vector<Rec> all(P);
   all[cid] = me;
   for (int j = 0; j < P; j++)
     if (j != cid) net.receive_into(j, &all[j], sizeof(Rec));
-  bool ok = true;
-  for (int j = 0; j < P; j++)
-    ok = ok && all[j].usable == 1 && all[j].nwg > 0 && strncmp(all[j].where, me.where, sizeof me.where) == 0;
-  if (!ok) {
+  bool same = true, loops = true;
+  for (int j = 0; j < P; j++) {
+    same = same && all[j].usable == 1 && strncmp(all[j].where, me.where, sizeof me.where) == 0;
+    loops = loops && all[j].nwg > 0;
+  }
+  D.colocated = (same && P > 1) ? 1 : 0;
+  D.peer_handle.assign(P, std::string());
+  D.peer_outbox.assign(P, nullptr);
+  if (D.colocated) log("bulk exchanges among " + std::to_string(P) + " peers in device memory");
+  if (!same || !loops) {
     D.boxes_state = 0;
     return;
   }
@@ -314,6 +379,9 @@ void DecentralizedClient::load_data() {
 void DecentralizedClient::init_server_and_client() {
   net_.reset(new exc::PeerNet(client_id, num_client, listen, base_server_port, opt_.recv_timeout_s));
   net_->connect_all(std::max(60.0, opt_.recv_timeout_s));
+  // this peer's share of the bidiagonalization columns (client.cpp:613-620)
+  const size_t c = m / num_client + (client_id < (int)(m % num_client) ? 1 : 0);
+  decide_device_exchange((int)m, (int)c, opt_.ralha_one_launch && exc_ralha_loop_fits((int)m, (int)c, 0) == 1);
 }
 
 void DecentralizedClient::run() {
@@ -408,7 +476,27 @@ void DecentralizedClient::run() {
   const size_t c = m_part[cid];
   const size_t R = reduced_global_n;
   t0 = now_seconds();
-  {
+  if (D.colocated) {
+    // row slab j of the masked block goes to peer j, device to device
+    double* ob = exc::outbox_reserve(D, m * rl);
+    ckh(hipMemcpyAsync(ob, masked, m * rl * sizeof(double), hipMemcpyDeviceToDevice, s), "pack");
+    sync();
+    for (int j = 0; j < P; j++)
+      if (j != cid) exc::dma_offer(net, D, j, m_part_pos[j] * rl, m_part[j] * rl);
+    D.P.alloc(c * R, s);
+    for (int j = 0; j < P; j++) {
+      const size_t rj = reduced_local_ns[j], off = reduced_n_pos[j];
+      const double* src = j == cid ? masked + m_part_pos[cid] * rl : exc::dma_take(net, D, j, c * rj);
+      ckh(hipMemcpy2DAsync(D.P.p + off, R * sizeof(double), src, rj * sizeof(double), rj * sizeof(double), c,
+                           hipMemcpyDeviceToDevice, s),
+          "re-partition");
+    }
+    sync();
+    for (int j = 0; j < P; j++)
+      if (j != cid) exc::dma_ack(net, j);
+    for (int j = 0; j < P; j++)
+      if (j != cid) exc::dma_wait_ack(net, j);
+  } else {
     HBuf& hm = D.h_masked;
     HBuf& hr = D.h_repart;
     hm.alloc(m * rl);
@@ -460,18 +548,35 @@ void DecentralizedClient::run() {
            "global QR panel");
         ck(exc_copy2d(ci, ci, rt.p, ci, D.P.p + pos, (long)R, s), "panel R");
         ck(exc_fill2d(ci, L - ci, 0.0, D.P.p + pos + ci, (long)R, s), "panel zero");
-        msgd.alloc(len, s);
-        ck(exc_pack_reflectors(ci, (long)R, pos, D.Vt.p, (long)R, D.tau_g.p, msgd.p, s), "pack");
-        std::vector<char> hmsg(len * sizeof(double));
-        ckh(hipMemcpyAsync(hmsg.data(), msgd.p, len * sizeof(double), hipMemcpyDeviceToHost, s), "d2h");
-        sync();
-        if ((int)grp.size() > 1) net.all_gather(hmsg, len * sizeof(double), grp, cid);
+        if (D.colocated) {
+          double* ob = exc::outbox_reserve(D, len);
+          ck(exc_pack_reflectors(ci, (long)R, pos, D.Vt.p, (long)R, D.tau_g.p, ob, s), "pack");
+          sync();
+          for (int j : grp)
+            if (j != cid) exc::dma_offer(net, D, j, 0, len);
+          for (int j : grp)
+            if (j != cid) exc::dma_wait_ack(net, j);
+        } else {
+          msgd.alloc(len, s);
+          ck(exc_pack_reflectors(ci, (long)R, pos, D.Vt.p, (long)R, D.tau_g.p, msgd.p, s), "pack");
+          std::vector<char> hmsg(len * sizeof(double));
+          ckh(hipMemcpyAsync(hmsg.data(), msgd.p, len * sizeof(double), hipMemcpyDeviceToHost, s), "d2h");
+          sync();
+          if ((int)grp.size() > 1) net.all_gather(hmsg, len * sizeof(double), grp, cid);
+        }
       } else if (compute_v || cid > i) {
-        std::vector<char> hmsg;
-        net.all_gather(hmsg, len * sizeof(double), grp, i);
-        if ((long)hmsg.size() != len * 8) throw std::runtime_error("global QR message of unexpected size");
         msgd.alloc(len, s);
-        ckh(hipMemcpyAsync(msgd.p, hmsg.data(), len * sizeof(double), hipMemcpyHostToDevice, s), "h2d");
+        if (D.colocated) {
+          const double* src = exc::dma_take(net, D, i, len);
+          ckh(hipMemcpyAsync(msgd.p, src, len * sizeof(double), hipMemcpyDeviceToDevice, s), "reflectors");
+          sync();
+          exc::dma_ack(net, i);
+        } else {
+          std::vector<char> hmsg;
+          net.all_gather(hmsg, len * sizeof(double), grp, i);
+          if ((long)hmsg.size() != len * 8) throw std::runtime_error("global QR message of unexpected size");
+          ckh(hipMemcpyAsync(msgd.p, hmsg.data(), len * sizeof(double), hipMemcpyHostToDevice, s), "h2d");
+        }
         ck(exc_unpack_reflectors(ci, (long)R, pos, msgd.p, D.Vt.p, (long)R, D.tau_g.p, s), "unpack");
         if (cid > i) {
           const size_t wb = exc_wy_workspace((int)ci, L, (long)c);
@@ -528,7 +633,6 @@ void DecentralizedClient::run() {
     allreduce_dev(D.scal.p, D.scal.p + 1, 1);
     ck(exc_ralha_tail_scale(0, mi, ci, D.F.p, D.U.p, D.scal.p + 1, D.alpha.p, s), "ralha init");
     const bool one_launch = opt_.ralha_one_launch && exc_ralha_loop_fits(mi, ci, 0) == 1;
-    if (D.boxes_state < 0) decide_device_exchange(mi, ci, one_launch);
     if (one_launch) {
       // steps -1 .. m-3 in one launch; the messages go through device-memory boxes when every
       // peer shares this device, else this thread all-reduces them over the network
@@ -656,21 +760,42 @@ void DecentralizedClient::run() {
        "part final U");
     // ring all-gather of the row blocks of FinalU (client.cpp:1062-1096)
     const double tc = now_seconds();
-    HBuf& hU = D.h_U;
-    hU.alloc(m * (size_t)k);
-    ckh(hipMemcpyAsync(hU.p + m_part_pos[cid] * k, D.FinalU.p + m_part_pos[cid] * k, c * (size_t)k * 8,
-                       hipMemcpyDeviceToHost, s),
-        "d2h");
-    sync();
-    const int next = (cid + 1) % P, front = (cid - 1 + P) % P;
-    for (int st = 0; st < P - 1; st++) {
-      int di = (cid - st + P) % P;
-      net.send_to(next, hU.p + m_part_pos[di] * k, m_part[di] * (size_t)k * 8);
-      di = (di - 1 + P) % P;
-      net.receive_into(front, hU.p + m_part_pos[di] * k, m_part[di] * (size_t)k * 8);
+    if (D.colocated) {
+      double* ob = exc::outbox_reserve(D, c * (size_t)k);
+      ckh(hipMemcpyAsync(ob, D.FinalU.p + m_part_pos[cid] * k, c * (size_t)k * 8, hipMemcpyDeviceToDevice, s),
+          "pack");
+      sync();
+      for (int j = 0; j < P; j++)
+        if (j != cid) exc::dma_offer(net, D, j, 0, c * (size_t)k);
+      for (int j = 0; j < P; j++) {
+        if (j == cid) continue;
+        const double* src = exc::dma_take(net, D, j, m_part[j] * (size_t)k);
+        ckh(hipMemcpyAsync(D.FinalU.p + m_part_pos[j] * k, src, m_part[j] * (size_t)k * 8, hipMemcpyDeviceToDevice,
+                           s),
+            "gather");
+      }
+      sync();
+      for (int j = 0; j < P; j++)
+        if (j != cid) exc::dma_ack(net, j);
+      for (int j = 0; j < P; j++)
+        if (j != cid) exc::dma_wait_ack(net, j);
+    } else {
+      HBuf& hU = D.h_U;
+      hU.alloc(m * (size_t)k);
+      ckh(hipMemcpyAsync(hU.p + m_part_pos[cid] * k, D.FinalU.p + m_part_pos[cid] * k, c * (size_t)k * 8,
+                         hipMemcpyDeviceToHost, s),
+          "d2h");
+      sync();
+      const int next = (cid + 1) % P, front = (cid - 1 + P) % P;
+      for (int st = 0; st < P - 1; st++) {
+        int di = (cid - st + P) % P;
+        net.send_to(next, hU.p + m_part_pos[di] * k, m_part[di] * (size_t)k * 8);
+        di = (di - 1 + P) % P;
+        net.receive_into(front, hU.p + m_part_pos[di] * k, m_part[di] * (size_t)k * 8);
+      }
+      ckh(hipMemcpyAsync(D.FinalU.p, hU.p, m * (size_t)k * 8, hipMemcpyHostToDevice, s), "h2d");
+      sync();
     }
-    ckh(hipMemcpyAsync(D.FinalU.p, hU.p, m * (size_t)k * 8, hipMemcpyHostToDevice, s), "h2d");
-    sync();
     net.add_comm_seconds(now_seconds() - tc);
     givens_mask(D.FinalU.p, (long)m, k, 0, 1, public_seed, D.plan, s);
   }

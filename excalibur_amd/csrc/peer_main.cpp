@@ -29,7 +29,7 @@ int main(int argc, char** argv) {
       {"dump", required_argument, nullptr, 1001},  {"recv-timeout", required_argument, nullptr, 1002},
       {"refine-iters", required_argument, nullptr, 1003}, {"repeat", required_argument, nullptr, 1004},
       {"profile", required_argument, nullptr, 1005},      {"ralha-loop", required_argument, nullptr, 1006},
-      {"ralha-device-exchange", required_argument, nullptr, 1007},
+      {"device-exchange", required_argument, nullptr, 1007},
       {nullptr, 0, nullptr, 0}};
   int c, index = 0;
   while ((c = getopt_long(argc, argv, "l:p:m:n:k:o:d:g:t:c:e:f:s:a:h", long_options, &index)) != -1) {
@@ -59,13 +59,13 @@ int main(int argc, char** argv) {
       case 1004: options.repeat = atoi(optarg); break;
       case 1005: options.profile = atoi(optarg) == 1; break;
       case 1006: options.ralha_one_launch = atoi(optarg) == 1; break;
-      case 1007: options.ralha_device_exchange = atoi(optarg) == 1; break;
+      case 1007: options.device_exchange = atoi(optarg) == 1; break;
       case 'h':
         printf(" -l --listen base server IP\n -p --port base server port\n -m --drow rows\n -n --dcol local columns\n"
                " -d --dpath data dir\n -g --logdir log dir\n -k --topk top-k (-1 all)\n -o --mode 0..3\n"
                " -t --nclient peers\n -c --cid my id\n -e --evaluate 1 dumps results\n -s --seed\n -a --opt 1|3\n"
                " --form fp64|mixed  --dump 0|1  --recv-timeout seconds  --ralha-loop 0|1\n"
-               " --ralha-device-exchange 0|1\n");
+               " --device-exchange 0|1\n");
         return 0;
       default: return 2;
     }

@@ -38,7 +38,7 @@ struct ClientOptions {
   int repeat = 1;                 // run() this many times on the same input (benchmarking)
   bool profile = false;           // time the MFMA GEMM / Givens launches with HIP events
   bool ralha_one_launch = true;   // the bidiagonalization loop as one kernel launch where it fits
-  bool ralha_device_exchange = true;  // peers sharing the device exchange its messages in device memory
+  bool device_exchange = true;  // peers sharing one device exchange data through device memory (IPC)
 };
 
 class DecentralizedClient {

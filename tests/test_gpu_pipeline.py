@@ -312,7 +312,7 @@ def test_run_device_exchange_equals_network_exchange(hip, oracle):
     m = 70
     xs = [rng.standard_normal((m, w)) for w in (200, 150, 250)]
     a = run_peers(xs, seed=5)
-    b = run_peers(xs, seed=5, extra=["--ralha-device-exchange", "0"])
+    b = run_peers(xs, seed=5, extra=["--device-exchange", "0"])
     logs = open(os.path.join(a["workdir"], "logs", "Client0.log")).read()
     assert "exchanged in device memory" in logs
     assert "exchanged in device memory" not in open(os.path.join(b["workdir"], "logs", "Client0.log")).read()
