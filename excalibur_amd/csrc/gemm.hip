@@ -52,7 +52,39 @@ struct GemmArgs {
   int lower_only; // compute only tiles with tile_row >= tile_col
   long kchunk;    // split-K chunk (multiple of BK); 0 = no split
   void* part;     // split-K partials: splits x M x N (type T)
+  long tm;        // tile rows
+  long ntile;     // tiles per split (tm x tn, or the lower tiles tm (tm + 1) / 2)
+  long total;     // ntile x splits = the 1-D grid
 };
+
+constexpr int kXcds = 8;  // dies of the device; the dispatcher deals workgroups round-robin over them
+
+// Tile of workgroup blockIdx.x. The dispatcher sends linear id L to die L % 8, so consecutive tiles
+// of the natural order would land on eight dies with eight separate L2s. The remap gives die x a
+// contiguous range of the tile order instead, and that order runs the row tiles of one column
+// (m fastest) back to back, so their shared B column tile is fetched from HBM once per die and
+// the row tiles read it from that die's L2. Splits of K are the slowest index.
+__device__ inline void tile_of(const GemmArgs& g, long& tile_m, long& tile_n, long& split) {
+  const long L = blockIdx.x;
+  const long q = g.total / kXcds, r = g.total % kXcds;
+  const long x = L % kXcds, j = L / kXcds;
+  const long t = x * q + min(x, r) + j;
+  split = t / g.ntile;
+  long u = t - split * g.ntile;
+  if (g.lower_only) {
+    // column n holds the tm - n tiles m = n .. tm - 1
+    long n = 0;
+    while (u >= g.tm - n) {
+      u -= g.tm - n;
+      n++;
+    }
+    tile_n = n;
+    tile_m = n + u;
+  } else {
+    tile_n = u / g.tm;
+    tile_m = u - tile_n * g.tm;
+  }
+}
 
 // K range [k0, k1) that can contribute to output rows [i0, i0+BM) / columns [j0, j0+BN).
 __device__ inline void k_range(const GemmArgs& g, long i0, long j0, long& k0, long& k1) {
@@ -97,20 +129,20 @@ __global__ __launch_bounds__(NT, 2) void k_gemm(GemmArgs g) {
   __shared__ T Bs[2][BK][BN + 1];
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
-  const long tile_m = blockIdx.y, tile_n = blockIdx.x;
-  if (g.lower_only && tile_n > tile_m) return;
+  long tile_m, tile_n, split;
+  tile_of(g, tile_m, tile_n, split);
   const long i0 = tile_m * BM, j0 = tile_n * BN;
   long k0, k1;
   k_range(g, i0, j0, k0, k1);
   if (g.kchunk > 0) {
-    const long ks = (long)blockIdx.z * g.kchunk;
+    const long ks = split * g.kchunk;
     k0 = max(k0, ks);
     k1 = min(k1, ks + g.kchunk);
   }
   const Stager<T, TA, TB> st(g, i0, j0, tid);
-  // interior: no bounds, no triangle, whole chunks
-  const bool interior = i0 + BM <= g.M && j0 + BN <= g.N && g.tri == EXC_TRI_NONE && (k0 % BK) == 0 &&
-                        ((k1 - k0) % BK) == 0;
+  // interior tile: no row or column bounds, no triangle, chunks aligned to BK; every whole chunk
+  // then loads without checks and only a last partial chunk checks its k range
+  const bool interior = i0 + BM <= g.M && j0 + BN <= g.N && g.tri == EXC_TRI_NONE && (k0 % BK) == 0;
   const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
   acc_t acc[4][4];
 #pragma unroll
@@ -150,7 +182,7 @@ __global__ __launch_bounds__(NT, 2) void k_gemm(GemmArgs g) {
   const long kstart = (k0 / BK) * BK;
   int buf = 0;
   if (kstart < k1) {
-    if (interior) load_fast(kstart);
+    if (interior && kstart + BK <= k1) load_fast(kstart);
     else load_checked(kstart);
     store_chunk(0);
   }
@@ -159,7 +191,7 @@ __global__ __launch_bounds__(NT, 2) void k_gemm(GemmArgs g) {
   for (long kb = kstart; kb < k1; kb += BK) {
     const bool more = kb + BK < k1;
     if (more) {
-      if (interior) load_fast(kb + BK);
+      if (interior && kb + 2 * BK <= k1) load_fast(kb + BK);
       else load_checked(kb + BK);
     }
 #pragma unroll
@@ -181,7 +213,7 @@ __global__ __launch_bounds__(NT, 2) void k_gemm(GemmArgs g) {
     }
   }
   // epilogue: operands were supplied as A[lane%16][lane/16], B[lane/16][lane%16]
-  T* C = g.kchunk > 0 ? (T*)g.part + (size_t)blockIdx.z * g.M * g.N : (T*)g.C;
+  T* C = g.kchunk > 0 ? (T*)g.part + (size_t)split * g.M * g.N : (T*)g.C;
   const long ldc = g.kchunk > 0 ? g.N : g.ldc;
   const bool direct = g.kchunk == 0;
   const T alpha = (T)g.alpha, beta = (T)g.beta;
@@ -261,7 +293,11 @@ int gemm_impl(int ta, int tb, long M, long N, long K, double alpha, const T* A, 
     g.kchunk = chunk;
     g.part = o.work;
   }
-  dim3 grid((unsigned)tn, (unsigned)tm, (unsigned)splits);
+  g.tm = tm;
+  g.ntile = tiles;
+  g.total = tiles * splits;
+  EXC_REQUIRE(g.total < (1L << 31), EXC_ERR_ARG, "exc_gemm: too many tiles");
+  dim3 grid((unsigned)g.total);
   hipEvent_t e0 = exc_prof::mark(s);
   if (!ta && !tb) hipLaunchKernelGGL((k_gemm<T, 0, 0>), grid, dim3(NT), 0, s, g);
   else if (!ta && tb) hipLaunchKernelGGL((k_gemm<T, 0, 1>), grid, dim3(NT), 0, s, g);
