@@ -50,6 +50,20 @@ __device__ inline double wave_max(double v) {
   return v;
 }
 
+// Data shared between workgroups of one running grid without cache maintenance: relaxed
+// device-scope atomics compile to sc1 loads and stores, coherent across the XCDs' L2s, so no
+// L2 writeback or invalidate (what a device-scope fence costs on gfx950) is needed. A writer stores
+// with coh_st, waits for its own stores (coh_stores_done, s_waitcnt vmcnt(0)) in every wavefront,
+// meets the others at a workgroup barrier, then one thread raises a flag with a relaxed atomic; a
+// reader spins on the flag with relaxed loads and then reads the data with coh_ld.
+__device__ __forceinline__ double coh_ld(const double* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void coh_st(double* p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void coh_stores_done() { __builtin_amdgcn_s_waitcnt(0x0F70); }  // vmcnt(0)
+
 // Live kernel timing with HIP events on the launch stream (off unless exc_prof_enable(1)).
 namespace exc_prof {
 bool enabled();

@@ -391,12 +391,8 @@ struct PeerBoxes {  // kernel argument; P = 0 selects the host link
 
 enum { kGin = 0, kBarrier = 1, kStop = 2 };  // device words
 
-__device__ __forceinline__ double ld_dev(const double* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void st_dev(double* p, double v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
+__device__ __forceinline__ double ld_dev(const double* p) { return coh_ld(p); }
+__device__ __forceinline__ void st_dev(double* p, double v) { coh_st(p, v); }
 __device__ __forceinline__ double ld_sys(const double* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
@@ -404,7 +400,7 @@ __device__ __forceinline__ void st_sys(double* p, double v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 // This wavefront's stores have completed (vmcnt 0; expcnt and lgkmcnt left alone).
-__device__ __forceinline__ void stores_done() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+__device__ __forceinline__ void stores_done() { coh_stores_done(); }
 
 __device__ __forceinline__ void loop_fail(int* dsync, LoopLink* link, int why) {
   __hip_atomic_store(&dsync[kStop], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);

@@ -24,14 +24,13 @@ constexpr long kSpinLimit = 1L << 22;
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 
-// Waits until *f != 0 (thread 0 spins with acquire loads; the barrier extends the acquire to the
-// workgroup). Returns false when the wait timed out or another workgroup reported an error.
+// Waits until *f != 0: thread 0 spins with relaxed device-scope loads, and the finished tile is
+// then read with coherent loads (coh_ld), so no cache needs invalidating. Returns false when the
+// wait timed out or another workgroup reported an error.
 __device__ __forceinline__ bool wait_flag(int* f, int* error, int* sh) {
   if (threadIdx.x == 0) {
     int ok = 1;
     long spins = 0;
-    // relaxed polls read the coherent value without invalidating the caches on every
-    // iteration; one acquire fence once the flag is seen
     while (__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
       if (__hip_atomic_load(error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) { ok = 0; break; }
       if (++spins > kSpinLimit) {
@@ -41,7 +40,6 @@ __device__ __forceinline__ bool wait_flag(int* f, int* error, int* sh) {
       }
       __builtin_amdgcn_s_sleep(2);
     }
-    if (ok) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     sh[0] = ok;
   }
   __syncthreads();
@@ -50,11 +48,11 @@ __device__ __forceinline__ bool wait_flag(int* f, int* error, int* sh) {
   return ok;
 }
 
-// Publishes a tile: every thread's stores are made visible device-wide before the flag.
+// Publishes a tile: every wavefront's coherent stores of it have completed before the flag.
 __device__ __forceinline__ void signal_flag(int* f) {
-  __threadfence();
+  coh_stores_done();
   __syncthreads();
-  if (threadIdx.x == 0) __hip_atomic_store(f, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  if (threadIdx.x == 0) __hip_atomic_store(f, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Accumulator layout: the wavefront holds the tile as 4 x 4 MFMA blocks of 16 x 16; element v of
@@ -93,11 +91,12 @@ __device__ __forceinline__ void acc_to_lds(const d4 (&acc)[4][4], const Quad& q,
       for (int v = 0; v < 4; v++) S[q.row(a, v) * LD + q.col(b)] = acc[a][b][v];
 }
 
-// T (LDS, 64 x 65) <- the nr x nc tile of A at (r0, c0), zero outside.
+// T (LDS, 64 x 65) <- the nr x nc tile of A at (r0, c0), zero outside (a finished tile of
+// another workgroup: coherent loads).
 __device__ __forceinline__ void tile_to_lds(double* T, const double* A, long lda, long r0, long c0, int nr, int nc) {
   for (int e = threadIdx.x; e < TB * TB; e += TNT) {
     const int r = e >> 6, c = e & 63;
-    T[r * LD + c] = (r < nr && c < nc) ? A[(r0 + r) * lda + c0 + c] : 0.0;
+    T[r * LD + c] = (r < nr && c < nc) ? coh_ld(&A[(r0 + r) * lda + c0 + c]) : 0.0;
   }
 }
 
@@ -110,10 +109,11 @@ __device__ __forceinline__ void tile_to_lds_t(double* T, const double* A, long l
 }
 
 // The nr x nc tile of A at (r0, c0) <- S (LDS); lower_zero writes zeros above the diagonal.
+// Coherent stores: other workgroups read the finished tile.
 __device__ __forceinline__ void lds_to_tile(const double* S, double* A, long lda, long r0, long c0, int nr, int nc, bool lower_zero) {
   for (int e = threadIdx.x; e < TB * TB; e += TNT) {
     const int r = e >> 6, c = e & 63;
-    if (r < nr && c < nc) A[(r0 + r) * lda + c0 + c] = (lower_zero && c > r) ? 0.0 : S[r * LD + c];
+    if (r < nr && c < nc) coh_st(&A[(r0 + r) * lda + c0 + c], (lower_zero && c > r) ? 0.0 : S[r * LD + c]);
   }
 }
 
@@ -231,6 +231,10 @@ __device__ __forceinline__ void tile_solve(double* S, const double* Dg, long lda
   double x[TB];
 #pragma unroll
   for (int k = 0; k < TB; k++) x[k] = MODE == 2 ? S[k * LD + lane] : S[lane * LD + k];
+  // The diagonal tile was finished by another workgroup and is read many times at
+  // wavefront-uniform addresses: one acquire fence (cache invalidation) makes cached loads see it,
+  // cheaper than a coherent load per element.
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   if (MODE == 0) {
     if (kd == TB) fact::right_solve<true, false, true>(kd, Dg, lda, x);
     else fact::right_solve<true, false, false>(kd, Dg, lda, x);
