@@ -17,6 +17,9 @@
 #include <stdexcept>
 #include <string>
 #include <vector>
+#include <tuple>
+#include <thread>
+#include <memory>
 
 #include "exc_common.hpp"
 #include "excalibur_hip.h"
@@ -177,32 +180,75 @@ using exc::now_seconds;
 
 namespace {
 
-void upload_plan(long K, long passes, int mask, unsigned seed, DBuf& pairs_cs_dev, hipStream_t s) {
-  const long half = exc_givens_half(K);
-  std::vector<int32_t> pairs((size_t)passes * half * 2);
-  std::vector<double> cs((size_t)passes * half * 2);
-  ck(exc_givens_plan(K, passes, mask, seed, pairs.data(), cs.data()), "givens plan");
-  // one buffer: pairs (as raw int32) then c/s
-  const size_t pair_words = (pairs.size() * sizeof(int32_t) + 7) / 8;
-  pairs_cs_dev.alloc(pair_words + cs.size(), s);
-  ckh(hipMemcpyAsync(pairs_cs_dev.p, pairs.data(), pairs.size() * sizeof(int32_t), hipMemcpyHostToDevice, s), "plan");
-  ckh(hipMemcpyAsync(pairs_cs_dev.p + pair_words, cs.data(), cs.size() * sizeof(double), hipMemcpyHostToDevice, s),
-      "plan");
-  ckh(hipStreamSynchronize(s), "plan upload");
-}
+// The Givens plans of one run (bbtsvd.cpp:727-779). Each is a function of (K, mask, seed) alone,
+// since the plan reseeds the C library's generator itself, so all of them are drawn on a host
+// thread while the GPU runs the local QR, into pinned buffers that the sweeps upload without a
+// stream synchronization. The thread is the only user of rand() until it is joined: run() draws
+// both seeds before it starts, and nothing after.
+struct PlanJob {
+  long K = 0;
+  int mask = 0;
+  unsigned seed = 0;
+  size_t pair_words = 0, count = 0;  // doubles: pairs (as int32) then c/s
+  HBuf host;
+  DBuf dev;
+};
 
-// block_shift_mask(m, n, 32, x, mask, is_left, seed) for K <= 10000 (bbtsvd.cpp:533-546).
-void givens_mask(double* d_x, long m, long n, int mask, int is_left, unsigned seed, DBuf& plan, hipStream_t s) {
+class RunPlans {
+ public:
+  ~RunPlans() { join(); }
+  void start(const std::vector<std::tuple<long, int, unsigned>>& want) {
+    for (const auto& w : want) {
+      bool dup = false;
+      for (auto& j : jobs_) dup = dup || (j->K == std::get<0>(w) && j->mask == std::get<1>(w) && j->seed == std::get<2>(w));
+      if (dup || std::get<0>(w) <= 1 || std::get<0>(w) > 10000) continue;
+      auto j = std::make_unique<PlanJob>();
+      j->K = std::get<0>(w);
+      j->mask = std::get<1>(w);
+      j->seed = std::get<2>(w);
+      const long half = exc_givens_half(j->K);
+      j->pair_words = ((size_t)32 * half * 2 * sizeof(int32_t) + 7) / 8;
+      j->count = j->pair_words + (size_t)32 * half * 2;
+      j->host.alloc(j->count);
+      jobs_.push_back(std::move(j));
+    }
+    th_ = std::thread([this] {
+      for (auto& j : jobs_) {
+        if (exc_givens_plan(j->K, 32, j->mask, j->seed, (int32_t*)j->host.p, j->host.p + j->pair_words) != EXC_OK)
+          failed_ = true;
+      }
+    });
+  }
+  PlanJob& get(long K, int mask, unsigned seed) {
+    join();
+    if (failed_) throw std::runtime_error("givens plan: " + std::string(exc_last_error()));
+    for (auto& j : jobs_)
+      if (j->K == K && j->mask == mask && j->seed == seed) return *j;
+    throw std::runtime_error("givens plan not prepared");
+  }
+
+ private:
+  void join() {
+    if (th_.joinable()) th_.join();
+  }
+  std::vector<std::unique_ptr<PlanJob>> jobs_;
+  std::thread th_;
+  bool failed_ = false;
+};
+
+// block_shift_mask(m, n, 32, x, mask, is_left, seed) for K <= 10000 (bbtsvd.cpp:533-546). The
+// sweep is enqueued on s behind an asynchronous upload of the plan; the caller synchronizes.
+void givens_mask(double* d_x, long m, long n, int mask, int is_left, unsigned seed, RunPlans& plans, hipStream_t s) {
   const long K = is_left ? m : n;
   if (K <= 1) return;
   if (K > 10000)
     throw std::runtime_error("Givens masks with more than 10000 rotated lines use the virtual-matrix path, "
                              "which this build does not implement");
-  upload_plan(K, 32, mask, seed, plan, s);
-  const long half = exc_givens_half(K);
-  const size_t pair_words = ((size_t)32 * half * 2 * sizeof(int32_t) + 7) / 8;
-  ck(exc_givens_apply(d_x, m, n, (const int32_t*)plan.p, plan.p + pair_words, 32, mask, is_left, s), "givens apply");
-  ckh(hipStreamSynchronize(s), "givens");
+  PlanJob& p = plans.get(K, mask, seed);
+  p.dev.alloc(p.count, s);
+  ckh(hipMemcpyAsync(p.dev.p, p.host.p, p.count * sizeof(double), hipMemcpyHostToDevice, s), "plan");
+  ck(exc_givens_apply(d_x, m, n, (const int32_t*)p.dev.p, p.dev.p + p.pair_words, 32, mask, is_left, s),
+     "givens apply");
 }
 
 }  // namespace
@@ -447,6 +493,10 @@ void DecentralizedClient::run() {
   // 2) local QR (client.cpp:572-589)
   const size_t rl = reduced_local_ns[cid];
   const bool lqr = m < local_n && ((double)(local_n / m)) > thr;
+  // the four Givens plans of this run: masks (left K = m, right K = rl) and their removal
+  RunPlans plans;
+  plans.start({{(long)m, 1, public_seed}, {(long)rl, 1, private_seed}, {(long)m, 0, public_seed},
+               {(long)rl, 0, private_seed}});
   double* masked = D.X.p;
   if (lqr) {
     t0 = now_seconds();
@@ -464,8 +514,9 @@ void DecentralizedClient::run() {
 
   // 3) masks (client.cpp:592-605)
   t0 = now_seconds();
-  givens_mask(masked, (long)m, (long)rl, 1, 1, public_seed, D.plan, s);
-  givens_mask(masked, (long)m, (long)rl, 1, 0, private_seed, D.plan, s);
+  givens_mask(masked, (long)m, (long)rl, 1, 1, public_seed, plans, s);
+  givens_mask(masked, (long)m, (long)rl, 1, 0, private_seed, plans, s);
+  sync();
   time_apply_mask = now_seconds() - t0;
   log("Masking Costs(1) (short-and-wide) " + std::to_string(time_apply_mask));
 
@@ -797,7 +848,7 @@ void DecentralizedClient::run() {
       sync();
     }
     net.add_comm_seconds(now_seconds() - tc);
-    givens_mask(D.FinalU.p, (long)m, k, 0, 1, public_seed, D.plan, s);
+    givens_mask(D.FinalU.p, (long)m, k, 0, 1, public_seed, plans, s);
   }
   D.U.release();
   if (compute_v) {
@@ -813,7 +864,7 @@ void DecentralizedClient::run() {
        "combine V");
     sync();
     D.Qg.release();
-    givens_mask(D.tmp.p, k, (long)rl, 0, 0, private_seed, D.plan, s);
+    givens_mask(D.tmp.p, k, (long)rl, 0, 0, private_seed, plans, s);
     if (lqr) {
       D.FinalVT.alloc((size_t)k * local_n, s);
       exc_prof::label_next("gemm_f64_final_v");  // the largest single product of the run
