@@ -93,10 +93,19 @@ __device__ __forceinline__ void acc_to_lds(const d4 (&acc)[4][4], const Quad& q,
 
 // T (LDS, 64 x 65) <- the nr x nc tile of A at (r0, c0), zero outside (a finished tile of
 // another workgroup: coherent loads).
+// Lane c reads column c of 16 rows at a time, all 16 loads in flight before the first LDS store.
 __device__ __forceinline__ void tile_to_lds(double* T, const double* A, long lda, long r0, long c0, int nr, int nc) {
-  for (int e = threadIdx.x; e < TB * TB; e += TNT) {
-    const int r = e >> 6, c = e & 63;
-    T[r * LD + c] = (r < nr && c < nc) ? coh_ld(&A[(r0 + r) * lda + c0 + c]) : 0.0;
+  const int c = threadIdx.x;
+#pragma unroll
+  for (int rb = 0; rb < TB; rb += 16) {
+    double v[16];
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+      const int r = rb + q;
+      v[q] = (r < nr && c < nc) ? coh_ld(&A[(r0 + r) * lda + c0 + c]) : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < 16; q++) T[(rb + q) * LD + c] = v[q];
   }
 }
 
@@ -154,8 +163,20 @@ __device__ inline void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+__device__ __forceinline__ double readlane_f64(double v, int src) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, src);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
 // Cholesky of the diagonal tile in S (padded rows/cols identity), lane i = row i. Returns the
 // first non-positive pivot (1-based) or 0; such a pivot is replaced by 1 like the blocked path.
+// Panels of PW columns: inside a panel the pivot and the multipliers L[k][j] reach the other lanes
+// by v_readlane (no LDS, no barrier); after the panel its columns go through LDS once for the
+// trailing update. Every element receives the same fused multiply-adds in the same order as a
+// column-by-column right-looking factorization.
+constexpr int PW = 8;
 __device__ __forceinline__ int potf2_reg(double* S, double* xc, int kb) {
   const int lane = threadIdx.x;
   __asm__ volatile("" ::: "memory");  // keep this routine's loads inside its branch
@@ -164,24 +185,34 @@ __device__ __forceinline__ int potf2_reg(double* S, double* xc, int kb) {
   for (int k = 0; k < TB; k++) x[k] = k <= lane ? S[lane * LD + k] : 0.0;
   int bad = 0;
 #pragma unroll
-  for (int j = 0; j < TB; j++) {
-    double* c = xc + (j & 1) * TB;
-    c[lane] = x[j];
-    wave_sync();
-    double d = c[j];
-    if (!(d > 0.0)) {
-      if (!bad && j < kb) bad = j + 1;
-      d = 1.0;
-    }
-    double rp, piv;
-    rsqrt_pair(d, &rp, &piv);
-    const double l = lane > j ? x[j] * rp : 0.0;
-    x[j] = lane == j ? piv : (lane > j ? l : x[j]);
-    wave_sync();
-    c[lane] = l;
-    wave_sync();
+  for (int j0 = 0; j0 < TB; j0 += PW) {
+    double lown[PW];  // L[lane][j] for the panel (zero on and above the diagonal)
 #pragma unroll
-    for (int k = j + 1; k < TB; k++) x[k] = fma(-l, c[k], x[k]);
+    for (int j = j0; j < j0 + PW; j++) {
+      double d = readlane_f64(x[j], j);
+      if (!(d > 0.0)) {
+        if (!bad && j < kb) bad = j + 1;
+        d = 1.0;
+      }
+      double rp, piv;
+      rsqrt_pair(d, &rp, &piv);
+      const double l = lane > j ? x[j] * rp : 0.0;
+      x[j] = lane == j ? piv : (lane > j ? l : x[j]);
+      lown[j - j0] = l;
+#pragma unroll
+      for (int k = j + 1; k < j0 + PW; k++) x[k] = fma(-l, readlane_f64(l, k), x[k]);
+    }
+    if (j0 + PW < TB) {
+#pragma unroll
+      for (int jj = 0; jj < PW; jj++) xc[lane * (PW + 1) + jj] = lown[jj];
+      wave_sync();
+#pragma unroll
+      for (int k = j0 + PW; k < TB; k++) {
+#pragma unroll
+        for (int jj = 0; jj < PW; jj++) x[k] = fma(-lown[jj], xc[k * (PW + 1) + jj], x[k]);
+      }
+      wave_sync();
+    }
   }
 #pragma unroll
   for (int k = 0; k < TB; k++) S[lane * LD + k] = k <= lane ? x[k] : 0.0;
