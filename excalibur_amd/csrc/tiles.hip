@@ -256,25 +256,30 @@ __device__ __forceinline__ void getf2_reg(double* S, double* xr, int kb, double*
 // (LOWER_T: op(D) = L^T, Cholesky; else U, LU); else lane = column c, S[:, c] <- L^-1 S[:, c]
 // with L the unit lower part of D (LU).
 template <int MODE>  // 0: rows, L^T; 1: rows, U; 2: columns, unit L
-__device__ __forceinline__ void tile_solve(double* S, const double* Dg, long lda, int kd) {
+__device__ __forceinline__ void tile_solve(double* S, double* T, const double* Dg, long lda, int kd) {
+  // The diagonal tile was finished by another workgroup. The LU modes copy it into LDS (T) with
+  // one batched coherent copy and substitute from there; the Cholesky mode, whose substitution
+  // walks rows of L^T, is faster reading it through the caches after one acquire fence.
+  if (MODE != 0) {
+    tile_to_lds(T, Dg, lda, 0, 0, kd, kd);
+    __syncthreads();
+  } else {
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  }
   __asm__ volatile("" ::: "memory");
   const int lane = threadIdx.x;
   double x[TB];
 #pragma unroll
   for (int k = 0; k < TB; k++) x[k] = MODE == 2 ? S[k * LD + lane] : S[lane * LD + k];
-  // The diagonal tile was finished by another workgroup and is read many times at
-  // wavefront-uniform addresses: one acquire fence (cache invalidation) makes cached loads see it,
-  // cheaper than a coherent load per element.
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   if (MODE == 0) {
     if (kd == TB) fact::right_solve<true, false, true>(kd, Dg, lda, x);
     else fact::right_solve<true, false, false>(kd, Dg, lda, x);
   } else if (MODE == 1) {
-    if (kd == TB) fact::right_solve<false, false, true>(kd, Dg, lda, x);
-    else fact::right_solve<false, false, false>(kd, Dg, lda, x);
+    if (kd == TB) fact::right_solve<false, false, true>(kd, T, LD, x);
+    else fact::right_solve<false, false, false>(kd, T, LD, x);
   } else {
-    if (kd == TB) fact::left_solve<false, true, true>(kd, Dg, lda, x);
-    else fact::left_solve<false, true, false>(kd, Dg, lda, x);
+    if (kd == TB) fact::left_solve<false, true, true>(kd, T, LD, x);
+    else fact::left_solve<false, true, false>(kd, T, LD, x);
   }
 #pragma unroll
   for (int k = 0; k < TB; k++) {
@@ -338,7 +343,7 @@ __global__ __launch_bounds__(TNT) void k_potrf_tiles(int n, double* __restrict__
     lds_to_tile(S, A, lda, r0, c0, nr, nc, true);
   } else {
     if (!wait_flag(&done[chol_index(tj, tj, nb)], error, sh)) return;
-    tile_solve<0>(S, A + c0 * lda + c0, lda, nc);  // L(i, j) = A(i, j) L(j, j)^-T
+    tile_solve<0>(S, X, A + c0 * lda + c0, lda, nc);  // L(i, j) = A(i, j) L(j, j)^-T
     __syncthreads();
     lds_to_tile(S, A, lda, r0, c0, nr, nc, false);
   }
@@ -403,8 +408,8 @@ __global__ __launch_bounds__(TNT) void k_lu_tiles(int n, double* __restrict__ A,
     if (!wait_flag(&done[lu_index(d, d, nb)], error, sh)) return;
     const long dd = (long)d * TB;
     const int kd = min(TB, n - (int)dd);
-    if (ti > tj) tile_solve<1>(S, A + dd * lda + dd, lda, kd);  // L(i, j) = A(i, j) U(j, j)^-1
-    else tile_solve<2>(S, A + dd * lda + dd, lda, kd);          // U(i, j) = L(i, i)^-1 A(i, j)
+    if (ti > tj) tile_solve<1>(S, X, A + dd * lda + dd, lda, kd);  // L(i, j) = A(i, j) U(j, j)^-1
+    else tile_solve<2>(S, X, A + dd * lda + dd, lda, kd);          // U(i, j) = L(i, i)^-1 A(i, j)
     __syncthreads();
     lds_to_tile(S, A, lda, r0, c0, nr, nc, false);
   }
