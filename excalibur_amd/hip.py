@@ -465,6 +465,19 @@ def ralha_peers(F_list, generic=False, loop=False, cols_per_wave=0, timeout_s=30
                  U=np.ascontiguousarray(dU[p].to_host().T)) for p in range(P)]
 
 
+_LOOP_STREAMS = []
+
+
+def _loop_streams(P):
+    """P streams for concurrent loop launches, created once per process and reused, so their
+    hardware queues are assigned while few streams exist (loops on one queue would serialize)."""
+    while len(_LOOP_STREAMS) < P:
+        st = _vp()
+        call("exc_stream_create", ctypes.byref(st))
+        _LOOP_STREAMS.append(st.value)
+    return _LOOP_STREAMS[:P]
+
+
 def _ralha_loop_run(m, cs, dF, dU, dh, dt, da, db, cols_per_wave, timeout_s, boxes=False):
     """Steps -1 .. m-3 of every peer as one launch each, on P streams; messages all-reduced here
     (or, with boxes, exchanged by the launches themselves through device-memory boxes)."""
@@ -474,7 +487,7 @@ def _ralha_loop_run(m, cs, dF, dU, dh, dt, da, db, cols_per_wave, timeout_s, box
     for c in cs:
         if not _lib.exc_ralha_loop_fits(m, c, cols_per_wave):
             raise HipError(-1, "exc_ralha_loop_fits")
-    streams, links, ws = [], [], []
+    streams, links, ws = _loop_streams(P), [], []
     if boxes:
         box = [DeviceArray((_lib.exc_ralha_box_bytes(m),), np.uint8) for _ in range(P)]
         for b in box:
@@ -484,9 +497,6 @@ def _ralha_loop_run(m, cs, dF, dU, dh, dt, da, db, cols_per_wave, timeout_s, box
         nwg = (_I * P)(*[_lib.exc_ralha_loop_grid(m, c, cols_per_wave) for c in cs])
     try:
         for p in range(P):
-            st = _vp()
-            call("exc_stream_create", ctypes.byref(st))
-            streams.append(st.value)
             ln = _vp()
             call("exc_ralha_link_alloc", m, ctypes.byref(ln))
             links.append(ln.value)
@@ -533,8 +543,6 @@ def _ralha_loop_run(m, cs, dF, dU, dh, dt, da, db, cols_per_wave, timeout_s, box
     finally:
         for ln in links:
             _lib.exc_ralha_link_free(ln)
-        for st in streams:
-            _lib.exc_stream_destroy(st)
 
 
 def lower_to_upper(alpha, beta):

@@ -8,6 +8,13 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
+# The in-process bidiagonalization tests run one persistent loop kernel per simulated peer, each
+# on its own stream, and the loops wait on each other; two streams sharing one hardware queue
+# would serialize them. Give the test process more hardware queues than it has such streams
+# (set before the HIP runtime starts; the runtime's default is 4).
+if int(os.environ.get("GPU_MAX_HW_QUEUES", "0") or 0) < 8:
+    os.environ["GPU_MAX_HW_QUEUES"] = "8"
+
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) GPU and the built HIP library")
