@@ -218,7 +218,8 @@ int exc_ralha_step_generic(int i, int m, int c, double* d_fc, double* d_uc, cons
  * exc_ralha_step for i = -1 .. m-3, with the partial sums of each message added in another fixed
  * order. Applies when exc_ralha_loop_fits(m, c, cols_per_wave) returns 1: m <= 3000, and twice
  * the grid (one workgroup per 4 x cols_per_wave columns, at most 256) fits on the device at
- * once, so two peers' loops can share it. cols_per_wave = 0 picks 1, 2 or 4. No counterpart in
+ * once, so two peers' loops can share it. cols_per_wave = 0 picks 1 or 2; other values than
+ * 0, 1 and 2 do not fit (the loop declines them). No counterpart in
  * the reference, which runs each step as BLAS calls between blocking all-reduces. */
 size_t exc_ralha_loop_workspace(int m, int c);
 int exc_ralha_link_alloc(int m, void** h_link);

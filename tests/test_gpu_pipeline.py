@@ -110,6 +110,8 @@ def test_ralha_loop_declines_a_grid_that_cannot_be_resident(hip):
     assert hip.lib.exc_ralha_loop_fits(1100, 1100, 0) == 0
     assert hip.lib.exc_ralha_loop_fits(1000, 500, 0) == 1
     assert hip.lib.exc_ralha_loop_fits(5000, 10, 0) == 0  # m above the register form
+    assert hip.lib.exc_ralha_loop_fits(1000, 500, 4) == 0  # four columns per wavefront: not built
+    assert hip.lib.exc_ralha_loop_grid(1000, 1000, 0) == 125  # 2 columns per wavefront
 
 
 @pytest.mark.parametrize("P,m,cols", [(2, 60, [30, 30]), (3, 101, [34, 34, 33]), (2, 600, [301, 299]),
