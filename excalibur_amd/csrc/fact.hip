@@ -188,9 +188,11 @@ int launch_right_blk(int lower_t, int unit, int kb, const double* U, long ldu, d
 
 namespace fact {
 
+// Room for 16 split-K partials of an n x n product: enough for the GEMM's split rule to fill the
+// device with the lower tiles of an n x n Gram matrix over a long K (14 splits at n = 1000).
 size_t work_bytes(int n, long wide) {
   (void)wide;
-  return (size_t)8 * n * n * sizeof(double);
+  return (size_t)16 * n * n * sizeof(double);
 }
 
 // Split point of a recursive step: a multiple of NB near n / 2, 0 < n1 < n (for n > NB).

@@ -264,6 +264,20 @@ __global__ __launch_bounds__(256) void k_mirror_lower(long n, T* C, long ldc) {
   if (j > i) C[i * ldc + j] = C[j * ldc + i];
 }
 
+// Workgroups of k_gemm resident at once on the current device: per_cu per CU (two for the FP64
+// kernel's 256 VGPRs and 64 KB of LDS, three for FP32), times the CU count (cached per device).
+long resident_slots(int per_cu) {
+  static int cus[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256L * per_cu;
+  if (cus[dev] <= 0) {
+    int n = 0;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
+    cus[dev] = n;
+  }
+  return (long)cus[dev] * per_cu;
+}
+
 template <typename T>
 int gemm_impl(int ta, int tb, long M, long N, long K, double alpha, const T* A, long lda, const T* B, long ldb,
               double beta, T* C, long ldc, const ExcGemmOpts& o, hipStream_t s) {
@@ -277,13 +291,20 @@ int gemm_impl(int ta, int tb, long M, long N, long K, double alpha, const T* A, 
   long tiles = o.lower_only ? tm * (tm + 1) / 2 : tm * tn;
   int splits = 1;
   if (o.work && K > 4 * BK) {
-    // split K until about two workgroups per CU
-    long want = (512 + tiles - 1) / tiles;
-    long maxs = K / (4 * BK);
-    long sp = std::min(want, maxs);
-    size_t need = (size_t)sp * M * N * sizeof(T);
-    while (sp > 1 && need > o.work_bytes) { sp--; need = (size_t)sp * M * N * sizeof(T); }
-    splits = (int)std::max(1L, sp);
+    // Split K into the number of chunks that minimizes (rounds of resident workgroups) x (chunk
+    // depth): a grid just above the resident slots would run a second, nearly empty round that
+    // costs as much as the first. Ties go to fewer splits (less partial traffic).
+    const long slots = resident_slots(sizeof(T) == 8 ? 2 : 3);
+    const long maxs = std::min<long>(K / (4 * BK), 4 * ((slots + tiles - 1) / tiles));
+    long best = 1, best_cost = -1;
+    for (long sp = 1; sp <= std::max(1L, maxs); sp++) {
+      if (sp > 1 && (size_t)sp * M * N * sizeof(T) > o.work_bytes) break;
+      const long rounds = (tiles * sp + slots - 1) / slots;
+      const long depth = ((K + sp - 1) / sp + BK - 1) / BK;
+      const long cost = rounds * depth;
+      if (best_cost < 0 || cost < best_cost) { best = sp; best_cost = cost; }
+    }
+    splits = (int)best;
   }
   if (K == 0) splits = 1;
   if (splits > 1) {
