@@ -13,7 +13,7 @@
 namespace {
 
 using fact::NB;
-using fact::left_solve;
+using fact::readlane_f64;
 using fact::right_solve;
 
 // The 64 x 64 diagonal-block kernels are branch-free: a block smaller than 64 is padded with the
@@ -107,27 +107,73 @@ __global__ __launch_bounds__(64) void k_getf2_sign(int kb, double* __restrict__ 
     if (lane < kb && k < kb) Ab[(long)lane * lda + k] = x[k];
 }
 
-// X_b <- op(L_bb)^-1 X_b for one diagonal block (kb <= NB), one lane per right-hand side, the
-// column in registers. Left-looking: unknown i takes the dot product of row i of the triangle
-// (contiguous, wave-uniform, so it comes through the scalar cache) with the finished unknowns,
-// in four partial sums.
-template <bool UPPER, bool UNIT>
-__global__ __launch_bounds__(64) void k_trsm_left_blk(int kb, const double* __restrict__ L, long ldl,
-                                                      double* __restrict__ X, long ldx, long nrhs) {
-  const long col = (long)blockIdx.x * 64 + threadIdx.x;
-  const unsigned cl = (unsigned)(col < nrhs ? col : nrhs - 1);  // per-lane offset from a uniform row base
-  double x[NB];
+// X_b <- op(L_bb)^-1 X_b for one diagonal block (kb <= NB), one lane per ROW: lane i holds row i
+// of the triangle (strictly below the diagonal for L, strictly above for U, zeros elsewhere) and
+// row i of C right-hand sides. Step j (top down for L, bottom up for U) broadcasts lane j's
+// unknown, y_j times the pivot's reciprocal, with readlane, and every lane takes one fused
+// multiply-add per column; lanes the step does not touch hold a zero coefficient, so there is no
+// branch. Unknown j's chain is a multiply, a readlane and a fused multiply-add (about 90 cycles),
+// and the C columns interleave to hide it: a narrow solve (few waves) takes a few microseconds
+// instead of a 64-deep chain of dependent dot products and divisions per column. Rows at or
+// beyond kb are padding (zero coefficients and right-hand sides). The four wavefronts of a
+// workgroup share one staged copy of the triangle and take C columns each; the triangle and the
+// right-hand sides pass through LDS so that every global access is coalesced.
+template <bool UPPER, bool UNIT, int C>
+__global__ __launch_bounds__(256) void k_trsm_left_blk(int kb, const double* __restrict__ L, long ldl,
+                                                       double* __restrict__ X, long ldx, long nrhs) {
+  // LDS: first the triangle (rows padded to SL), then, once every wave holds its row, each wave's
+  // 64 x C block of right-hand sides (rows padded to S; both strides keep 16-byte alignment)
+  constexpr int SL = NB + 2, S = C + 2, RPI = 64 / C;
+  constexpr int kLds = NB * SL > 4 * NB * S ? NB * SL : 4 * NB * S;
+  __shared__ __attribute__((aligned(16))) double lds[kLds];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const long c0 = ((long)blockIdx.x * 4 + wave) * C;
+  const int row = lane < kb ? lane : kb - 1;
+  const bool live = lane < kb;
+  const int lc = lane < kb ? lane : kb - 1;
 #pragma unroll
-  for (int i = 0; i < NB; i++) {
-    x[i] = 0.0;
-    if (i < kb) x[i] = (X + (long)i * ldx)[cl];
+  for (int r = wave; r < NB; r += 4) lds[r * SL + lane] = L[(long)(r < kb ? r : kb - 1) * ldl + lc];
+  __syncthreads();
+  double lr[NB];
+#pragma unroll
+  for (int k = 0; k < NB; k++) {
+    const double v = lds[row * SL + k];
+    lr[k] = (live && k < kb && (UPPER ? k > lane : k < lane)) ? v : 0.0;
   }
-  if (kb == NB) left_solve<UPPER, UNIT, true>(kb, L, ldl, x);
-  else left_solve<UPPER, UNIT, false>(kb, L, ldl, x);
-  if (col < nrhs) {
+  const double rinv = UNIT ? 1.0 : 1.0 / lds[row * SL + row];
+  __syncthreads();
+  // coalesced: each load instruction covers 64 / C whole row segments of C columns
+  double* Xs = lds + wave * NB * S;
+  const int sr = lane / C, sc = lane % C;
+  const bool cok = c0 + sc < nrhs;
+  const long gcol = cok ? c0 + sc : nrhs - 1;
 #pragma unroll
-    for (int i = 0; i < NB; i++)
-      if (i < kb) (X + (long)i * ldx)[cl] = x[i];
+  for (int t = 0; t < C; t++) {
+    const int r = t * RPI + sr;
+    const double v = X[(long)(r < kb ? r : kb - 1) * ldx + gcol];
+    Xs[r * S + sc] = (r < kb && cok) ? v : 0.0;
+  }
+  __syncthreads();
+  double y[C];
+#pragma unroll
+  for (int c = 0; c < C; c++) y[c] = Xs[lane * S + c];
+#pragma unroll
+  for (int q = 0; q < NB; q++) {
+    const int j = UPPER ? NB - 1 - q : q;
+    if (j >= kb) continue;  // uniform
+#pragma unroll
+    for (int c = 0; c < C; c++) {
+      const double xj = readlane_f64(UNIT ? y[c] : y[c] * rinv, j);
+      y[c] = fma(-lr[j], xj, y[c]);
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < C; c++) Xs[lane * S + c] = UNIT ? y[c] : y[c] * rinv;
+  __syncthreads();
+#pragma unroll
+  for (int t = 0; t < C; t++) {
+    const int r = t * RPI + sr;
+    if (r < kb && cok) X[(long)r * ldx + c0 + sc] = Xs[r * S + sc];
   }
 }
 
@@ -164,11 +210,17 @@ __global__ __launch_bounds__(256) void k_zero_strict_upper(int n, double* A, lon
 
 int launch_left_blk(int upper, int unit, int kb, const double* L, long ldl, double* X, long ldx, long nrhs,
                     hipStream_t s) {
-  const dim3 grid(exc_detail::ceil_div(nrhs, 64)), block(64);
-  if (!upper && !unit) hipLaunchKernelGGL((k_trsm_left_blk<false, false>), grid, block, 0, s, kb, L, ldl, X, ldx, nrhs);
-  if (!upper && unit) hipLaunchKernelGGL((k_trsm_left_blk<false, true>), grid, block, 0, s, kb, L, ldl, X, ldx, nrhs);
-  if (upper && !unit) hipLaunchKernelGGL((k_trsm_left_blk<true, false>), grid, block, 0, s, kb, L, ldl, X, ldx, nrhs);
-  if (upper && unit) hipLaunchKernelGGL((k_trsm_left_blk<true, true>), grid, block, 0, s, kb, L, ldl, X, ldx, nrhs);
+  // 4 columns per wavefront up to 4096 right-hand sides (latency-bound: more waves), else 16
+  // (throughput-bound: fewer reads of the triangle); four wavefronts per workgroup
+  const bool wide = nrhs > 4L * 1024;
+  const dim3 grid(exc_detail::ceil_div(nrhs, 4L * (wide ? 16 : 4))), block(256);
+#define EXC_LEFT_BLK(U, N)                                                                                      \
+  if ((upper != 0) == U && (unit != 0) == N) {                                                                                \
+    if (wide) hipLaunchKernelGGL((k_trsm_left_blk<U, N, 16>), grid, block, 0, s, kb, L, ldl, X, ldx, nrhs);    \
+    else hipLaunchKernelGGL((k_trsm_left_blk<U, N, 4>), grid, block, 0, s, kb, L, ldl, X, ldx, nrhs);          \
+  }
+  EXC_LEFT_BLK(false, false) EXC_LEFT_BLK(false, true) EXC_LEFT_BLK(true, false) EXC_LEFT_BLK(true, true)
+#undef EXC_LEFT_BLK
   EXC_CHECK_HIP(hipGetLastError());
   return EXC_OK;
 }

@@ -8,6 +8,14 @@
 
 namespace fact {
 
+// Lane src's value of v, as a wave-uniform (scalar) value.
+__device__ __forceinline__ double readlane_f64(double v, int src) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, src);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
 // A triangle element: a plain load, or (COH) a coherent one for a block another workgroup of the
 // running grid has written.
 template <bool COH>

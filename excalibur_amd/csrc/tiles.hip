@@ -17,6 +17,8 @@
 
 namespace {
 
+using fact::readlane_f64;
+
 constexpr int TB = 64;    // tile edge
 constexpr int TNT = 64;   // threads per workgroup: one wavefront per tile
 constexpr int LD = TB + 1;
@@ -163,12 +165,6 @@ __device__ inline void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-__device__ __forceinline__ double readlane_f64(double v, int src) {
-  const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_readlane((int)b, src);
-  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
-  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
-}
 
 // Cholesky of the diagonal tile in S (padded rows/cols identity), lane i = row i. Returns the
 // first non-positive pivot (1-based) or 0; such a pivot is replaced by 1 like the blocked path.
