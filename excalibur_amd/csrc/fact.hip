@@ -14,7 +14,6 @@ namespace {
 
 using fact::NB;
 using fact::readlane_f64;
-using fact::right_solve;
 
 // The 64 x 64 diagonal-block kernels are branch-free: a block smaller than 64 is padded with the
 // identity (unit diagonal, zero off-diagonal), which leaves the padded unknowns untouched, so
@@ -109,10 +108,10 @@ __global__ __launch_bounds__(64) void k_getf2_sign(int kb, double* __restrict__ 
 
 // X_b <- op(L_bb)^-1 X_b for one diagonal block (kb <= NB), one lane per ROW: lane i holds row i
 // of the triangle (strictly below the diagonal for L, strictly above for U, zeros elsewhere) and
-// row i of C right-hand sides. Step j (top down for L, bottom up for U) broadcasts lane j's
-// unknown, y_j times the pivot's reciprocal, with readlane, and every lane takes one fused
+// row i of C right-hand sides, both scaled by the pivot's reciprocal. Step j (top down for L,
+// bottom up for U) broadcasts lane j's unknown with readlane, and every lane takes one fused
 // multiply-add per column; lanes the step does not touch hold a zero coefficient, so there is no
-// branch. Unknown j's chain is a multiply, a readlane and a fused multiply-add (about 90 cycles),
+// branch. Unknown j's chain is a readlane and a fused multiply-add (about 50 cycles),
 // and the C columns interleave to hide it: a narrow solve (few waves) takes a few microseconds
 // instead of a 64-deep chain of dependent dot products and divisions per column. Rows at or
 // beyond kb are padding (zero coefficients and right-hand sides). The four wavefronts of a
@@ -134,13 +133,14 @@ __global__ __launch_bounds__(256) void k_trsm_left_blk(int kb, const double* __r
 #pragma unroll
   for (int r = wave; r < NB; r += 4) lds[r * SL + lane] = L[(long)(r < kb ? r : kb - 1) * ldl + lc];
   __syncthreads();
+  // row i scaled by its pivot's reciprocal, so that each step's unknown is the broadcast value
+  const double rinv = UNIT ? 1.0 : 1.0 / lds[row * SL + row];
   double lr[NB];
 #pragma unroll
   for (int k = 0; k < NB; k++) {
     const double v = lds[row * SL + k];
-    lr[k] = (live && k < kb && (UPPER ? k > lane : k < lane)) ? v : 0.0;
+    lr[k] = (live && k < kb && (UPPER ? k > lane : k < lane)) ? (UNIT ? v : v * rinv) : 0.0;
   }
-  const double rinv = UNIT ? 1.0 : 1.0 / lds[row * SL + row];
   __syncthreads();
   // coalesced: each load instruction covers 64 / C whole row segments of C columns
   double* Xs = lds + wave * NB * S;
@@ -156,19 +156,16 @@ __global__ __launch_bounds__(256) void k_trsm_left_blk(int kb, const double* __r
   __syncthreads();
   double y[C];
 #pragma unroll
-  for (int c = 0; c < C; c++) y[c] = Xs[lane * S + c];
+  for (int c = 0; c < C; c++) y[c] = UNIT ? Xs[lane * S + c] : Xs[lane * S + c] * rinv;
 #pragma unroll
   for (int q = 0; q < NB; q++) {
     const int j = UPPER ? NB - 1 - q : q;
     if (j >= kb) continue;  // uniform
 #pragma unroll
-    for (int c = 0; c < C; c++) {
-      const double xj = readlane_f64(UNIT ? y[c] : y[c] * rinv, j);
-      y[c] = fma(-lr[j], xj, y[c]);
-    }
+    for (int c = 0; c < C; c++) y[c] = fma(-lr[j], readlane_f64(y[c], j), y[c]);
   }
 #pragma unroll
-  for (int c = 0; c < C; c++) Xs[lane * S + c] = UNIT ? y[c] : y[c] * rinv;
+  for (int c = 0; c < C; c++) Xs[lane * S + c] = y[c];
   __syncthreads();
 #pragma unroll
   for (int t = 0; t < C; t++) {
@@ -177,27 +174,51 @@ __global__ __launch_bounds__(256) void k_trsm_left_blk(int kb, const double* __r
   }
 }
 
-// X_b <- X_b U_bb^-1 (or X_b L_bb^-T) for one diagonal block, one lane per row in registers.
-// Unknown j takes the dot product of the finished unknowns with column j of op(U), which is the
-// contiguous row j of L for L^T; for U the update runs right-looking along the contiguous row j.
-template <bool LOWER_T, bool UNIT>
-__global__ __launch_bounds__(64) void k_trsm_right_blk(int kb, const double* __restrict__ U, long ldu,
-                                                       double* __restrict__ X, long ldx, long rows) {
-  const long row = (long)blockIdx.x * 64 + threadIdx.x;
-  const long rr = row < rows ? row : rows - 1;
-  double* xr = X + rr * ldx;  // this lane's row; element j at a constant offset
-  double x[NB];
+// X_b <- X_b U_bb^-1 (or X_b L_bb^-T) for one diagonal block (kb <= NB), one lane per COLUMN j
+// of the unknown rows: lane j holds column j of op(U) above the diagonal (U[k][j], or L[j][k]
+// for L^T) scaled by the pivot's reciprocal, and entry j of C rows of X (contiguous across the
+// lanes, so the accesses to X are coalesced as they are). Step k (ascending) broadcasts lane k's
+// unknown with readlane and every lane takes one fused multiply-add per row, the same scheme as
+// k_trsm_left_blk. The triangle is staged through LDS with coalesced row loads; the four
+// wavefronts of a workgroup share it and take C rows each.
+template <bool LOWER_T, bool UNIT, int C>
+__global__ __launch_bounds__(256) void k_trsm_right_blk(int kb, const double* __restrict__ U, long ldu,
+                                                        double* __restrict__ X, long ldx, long rows) {
+  constexpr int SL = NB + 2;
+  __shared__ __attribute__((aligned(16))) double lds[NB * SL];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const long r0 = ((long)blockIdx.x * 4 + wave) * C;
+  const bool live = lane < kb;
+  const int lc = lane < kb ? lane : kb - 1;
 #pragma unroll
-  for (int j = 0; j < NB; j++) {
-    x[j] = 0.0;
-    if (j < kb) x[j] = xr[j];
+  for (int r = wave; r < NB; r += 4) lds[r * SL + lane] = U[(long)(r < kb ? r : kb - 1) * ldu + lc];
+  __syncthreads();
+  // op(U)[k][j] = U[k][j], or L[j][k] for L^T
+  auto op = [&](int k, int j) { return LOWER_T ? lds[j * SL + k] : lds[k * SL + j]; };
+  const double rinv = UNIT ? 1.0 : 1.0 / op(lc, lc);
+  double co[NB];
+#pragma unroll
+  for (int k = 0; k < NB; k++) {
+    const double v = op(k, lc);
+    co[k] = (live && k < lane) ? (UNIT ? v : v * rinv) : 0.0;
   }
-  if (kb == NB) right_solve<LOWER_T, UNIT, true>(kb, U, ldu, x);
-  else right_solve<LOWER_T, UNIT, false>(kb, U, ldu, x);
-  if (row < rows) {
+  double y[C];
 #pragma unroll
-    for (int j = 0; j < NB; j++)
-      if (j < kb) xr[j] = x[j];
+  for (int c = 0; c < C; c++) {
+    const long rr = r0 + c < rows ? r0 + c : rows - 1;
+    const double v = X[rr * ldx + lc];
+    y[c] = (live && r0 + c < rows) ? (UNIT ? v : v * rinv) : 0.0;
+  }
+#pragma unroll
+  for (int k = 0; k < NB; k++) {
+    if (k >= kb) continue;  // uniform
+#pragma unroll
+    for (int c = 0; c < C; c++) y[c] = fma(-co[k], readlane_f64(y[c], k), y[c]);
+  }
+  if (live) {
+#pragma unroll
+    for (int c = 0; c < C; c++)
+      if (r0 + c < rows) X[(r0 + c) * ldx + lane] = y[c];
   }
 }
 
@@ -227,11 +248,16 @@ int launch_left_blk(int upper, int unit, int kb, const double* L, long ldl, doub
 
 int launch_right_blk(int lower_t, int unit, int kb, const double* U, long ldu, double* X, long ldx, long rows,
                      hipStream_t s) {
-  const dim3 grid(exc_detail::ceil_div(rows, 64)), block(64);
-  if (!lower_t && !unit) hipLaunchKernelGGL((k_trsm_right_blk<false, false>), grid, block, 0, s, kb, U, ldu, X, ldx, rows);
-  if (!lower_t && unit) hipLaunchKernelGGL((k_trsm_right_blk<false, true>), grid, block, 0, s, kb, U, ldu, X, ldx, rows);
-  if (lower_t && !unit) hipLaunchKernelGGL((k_trsm_right_blk<true, false>), grid, block, 0, s, kb, U, ldu, X, ldx, rows);
-  if (lower_t && unit) hipLaunchKernelGGL((k_trsm_right_blk<true, true>), grid, block, 0, s, kb, U, ldu, X, ldx, rows);
+  // as launch_left_blk: 4 rows per wavefront up to 4096 rows, else 16; four wavefronts per workgroup
+  const bool wide = rows > 4L * 1024;
+  const dim3 grid(exc_detail::ceil_div(rows, 4L * (wide ? 16 : 4))), block(256);
+#define EXC_RIGHT_BLK(T, N)                                                                                     \
+  if ((lower_t != 0) == T && (unit != 0) == N) {                                                                \
+    if (wide) hipLaunchKernelGGL((k_trsm_right_blk<T, N, 16>), grid, block, 0, s, kb, U, ldu, X, ldx, rows);    \
+    else hipLaunchKernelGGL((k_trsm_right_blk<T, N, 4>), grid, block, 0, s, kb, U, ldu, X, ldx, rows);          \
+  }
+  EXC_RIGHT_BLK(false, false) EXC_RIGHT_BLK(false, true) EXC_RIGHT_BLK(true, false) EXC_RIGHT_BLK(true, true)
+#undef EXC_RIGHT_BLK
   EXC_CHECK_HIP(hipGetLastError());
   return EXC_OK;
 }

@@ -69,7 +69,7 @@ def test_trsm_left(hip, upper, unit, n, nrhs):
 
 
 @pytest.mark.parametrize("lower_t", [False, True])
-@pytest.mark.parametrize("n,rows", [(3, 5), (128, 64), (300, 1000), (700, 130)])
+@pytest.mark.parametrize("n,rows", [(3, 5), (128, 64), (300, 1000), (700, 130), (200, 5000), (64, 4100)])
 def test_trsm_right(hip, lower_t, n, rows):
     banner(f"trsm_right n={n} rows={rows} lower_t={lower_t}")
     rng = np.random.default_rng(n * 5 + rows)
