@@ -45,8 +45,9 @@ struct DBuf {
   ~DBuf() { free_all(); }
   size_t cap = 0;
   // Reuses the allocation when it is large enough (repeated runs allocate nothing); the first
-  // `count` elements are zeroed.
-  void alloc(size_t count, hipStream_t s) {
+  // `count` elements are zeroed unless the caller overwrites all of them before any read (zero =
+  // false: a full output of a beta = 0 product, or a workspace whose users initialize it).
+  void alloc(size_t count, hipStream_t s, bool zero = true) {
     if (!p || cap < count) {
       free_all();
       void* q = nullptr;
@@ -55,7 +56,7 @@ struct DBuf {
       cap = std::max<size_t>(count, 1);
     }
     n = count;
-    ckh(hipMemsetAsync(p, 0, std::max<size_t>(count, 1) * sizeof(double), s), "memset");
+    if (zero) ckh(hipMemsetAsync(p, 0, std::max<size_t>(count, 1) * sizeof(double), s), "memset");
   }
   void release() { n = 0; }  // keeps the memory for the next run
   void free_all() {
@@ -502,7 +503,7 @@ void DecentralizedClient::run() {
     t0 = now_seconds();
     D.masked.alloc(m * m, s);
     const size_t wb = exc_qr_t_workspace((int)m, (long)local_n, 1);
-    D.work.alloc((wb + 7) / 8, s);
+    D.work.alloc((wb + 7) / 8, s, false);  // exc_qr_t writes every part of it before reading it
     ck(exc_qr_t((int)m, (long)local_n, D.X.p, (long)local_n, D.masked.p, compute_v ? D.X.p : nullptr, (long)local_n,
                 nullptr, 0, nullptr, nullptr, 0, D.work.p, D.work.n * 8, s),
        "local QR");
@@ -866,7 +867,7 @@ void DecentralizedClient::run() {
     D.Qg.release();
     givens_mask(D.tmp.p, k, (long)rl, 0, 0, private_seed, plans, s);
     if (lqr) {
-      D.FinalVT.alloc((size_t)k * local_n, s);
+      D.FinalVT.alloc((size_t)k * local_n, s, false);  // every element written by the beta = 0 product
       exc_prof::label_next("gemm_f64_final_v");  // the largest single product of the run
       ck(exc_dgemm(0, 0, k, (long)local_n, (long)rl, 1.0, D.tmp.p, (long)rl, D.X.p, (long)local_n, 0.0, D.FinalVT.p,
                    (long)local_n, nullptr, 0, s),
