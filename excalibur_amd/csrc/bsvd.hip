@@ -251,33 +251,34 @@ __global__ __launch_bounds__(256) void k_twisted(int n, int k, const double* __r
   const double w = omega[j];
   const double w2 = w * w;
 
-  // dstqds (bbtsvd.cpp:73-81)
+  // dstqds (bbtsvd.cpp:73-81, forward) and dpqds (bbtsvd.cpp:83-91, backward) are independent
+  // recurrences: one loop runs both, so the two chains of dependent divisions overlap. Each
+  // chain evaluates the reference's expressions in the reference's order.
   double sc = -w2;
   S[at(0)] = sc;
-  #pragma unroll 4
-  for (int i = 0; i < n - 1; i++) {
-    const double dp = d[i] + sc;
-    const double lp = d[i] * l[i] / dp;
-    DP[at(i)] = dp;
-    sc = lp * l[i] * sc - w2;
-    S[at(i + 1)] = sc;
-  }
-  DP[at(n - 1)] = d[n - 1] + sc;
-  // dpqds (bbtsvd.cpp:83-91)
   double pc = d[n - 1] - w2;
   Pp[at(n - 1)] = pc;
-  #pragma unroll 4
-  for (int i = n - 2; i >= 0; i--) {
-    const double rp = d[i] * (l[i] * l[i]) + pc;
-    RP[at(i + 1)] = rp;
-    pc = pc * d[i] / rp - w2;
-    Pp[at(i)] = pc;
+  #pragma unroll 2
+  for (int t = 0; t < n - 1; t++) {
+    const int i = t;          // dstqds step
+    const int q = n - 2 - t;  // dpqds step
+    const double dp = d[i] + sc;
+    const double rp = d[q] * (l[q] * l[q]) + pc;
+    const double lp = d[i] * l[i] / dp;
+    const double pq = pc * d[q] / rp;
+    DP[at(i)] = dp;
+    RP[at(q + 1)] = rp;
+    sc = lp * l[i] * sc - w2;
+    pc = pq - w2;
+    S[at(i + 1)] = sc;
+    Pp[at(q)] = pc;
   }
+  DP[at(n - 1)] = d[n - 1] + sc;
   RP[at(0)] = pc;
   // twist: first index of min |s + p + omega^2|
   int k0 = 0;
   double best = 0.0;
-  #pragma unroll 4
+  #pragma unroll 16
   for (int i = 0; i < n; i++) {
     const double g = S[at(i)] + Pp[at(i)] + w2;
     const double ag = fabs(g);
@@ -286,27 +287,27 @@ __global__ __launch_bounds__(256) void k_twisted(int n, int k, const double* __r
   // inverse iteration for v (bbtsvd.cpp:65-71), l+ = d l / d+, u+ = l d / r+
   vs[at(k0)] = 1.0;
   double z = 1.0;
-  #pragma unroll 4
+  #pragma unroll 16
   for (int i = k0 - 1; i >= 0; i--) {
     const double lp = d[i] * l[i] / DP[at(i)];
     z = -lp * z;
     vs[at(i)] = z;
   }
   z = 1.0;
-  #pragma unroll 4
+  #pragma unroll 16
   for (int i = k0 + 1; i < n; i++) {
     const double up = l[i - 1] * d[i - 1] / RP[at(i)];
     z = -up * z;
     vs[at(i)] = z;
   }
   double ss = 0.0;
-  #pragma unroll 4
+  #pragma unroll 16
   for (int i = 0; i < n; i++) {
     const double v = vs[at(i)];
     ss = ss + v * v;
   }
   double inv = 1.0 / sqrt(ss);
-  #pragma unroll 4
+  #pragma unroll 16
   for (int i = 0; i < n; i++) vs[at(i)] = inv * vs[at(i)];
 
   // coupling (bbtsvd.cpp:113-124): s[n] = -omega^2, p[n] = 1, r+[n] = 1
@@ -315,7 +316,7 @@ __global__ __launch_bounds__(256) void k_twisted(int n, int k, const double* __r
   auto Rv = [&](int i) -> double { return i == n ? 1.0 : RP[at(i)]; };
   int k1 = 0;
   best = 0.0;
-  #pragma unroll 4
+  #pragma unroll 16
   for (int i = 0; i < n; i++) {
     const double si = Sv(i), pi = Pv(i);
     const double g = si + pi + w2;
@@ -327,7 +328,7 @@ __global__ __launch_bounds__(256) void k_twisted(int n, int k, const double* __r
   // d+'_i = s_{i+1} / s_i d+_i, r+'_i = p_i / p_{i+1} r+_{i+1}
   u[at(k1)] = 1.0;
   z = 1.0;
-  #pragma unroll 4
+  #pragma unroll 16
   for (int i = k1 - 1; i >= 0; i--) {
     const double dpn = Sv(i + 1) / Sv(i) * DP[at(i)];
     const double lpn = a[i + 1] / dpn * b[i];
@@ -335,7 +336,7 @@ __global__ __launch_bounds__(256) void k_twisted(int n, int k, const double* __r
     u[at(i)] = z;
   }
   z = 1.0;
-  #pragma unroll 4
+  #pragma unroll 16
   for (int i = k1 + 1; i < n; i++) {
     const int q = i - 1;
     const double rpn = Pv(q + 1) / Pv(q + 2) * Rv(q + 2);
@@ -344,13 +345,13 @@ __global__ __launch_bounds__(256) void k_twisted(int n, int k, const double* __r
     u[at(i)] = z;
   }
   ss = 0.0;
-  #pragma unroll 4
+  #pragma unroll 16
   for (int i = 0; i < n; i++) {
     const double v = u[at(i)];
     ss = ss + v * v;
   }
   inv = 1.0 / sqrt(ss);
-  #pragma unroll 4
+  #pragma unroll 16
   for (int i = 0; i < n; i++) u[at(i)] = inv * u[at(i)];
   if (twists) {
     twists[2 * j] = k0;
