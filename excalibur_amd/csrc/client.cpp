@@ -807,8 +807,10 @@ void DecentralizedClient::run() {
   if (compute_u) {
     // this peer's rows of FinalU: (bv U)^T = U^T bv^T, with U^T stored row-major (c x m)
     D.FinalU.alloc(m * (size_t)k, s);
+    // split-K partials: a c x k output over K = m is too few tiles to fill the device
+    D.work.alloc((exc_gemm_workspace((long)c, k, (long)m) + 7) / 8, s, false);
     ck(exc_dgemm(0, 1, (long)c, k, (long)m, 1.0, D.U.p, (long)m, D.bv.p, (long)m, 0.0,
-                 D.FinalU.p + m_part_pos[cid] * (size_t)k, k, nullptr, 0, s),
+                 D.FinalU.p + m_part_pos[cid] * (size_t)k, k, D.work.p, D.work.n * 8, s),
        "part final U");
     // ring all-gather of the row blocks of FinalU (client.cpp:1062-1096)
     const double tc = now_seconds();
@@ -860,8 +862,9 @@ void DecentralizedClient::run() {
       ck(exc_ormqr_house_ln((int)m, k, D.house.p, D.tau_h.p, D.bu.p, k, D.work.p, D.work.n * 8, s), "ormqr house");
     }
     D.tmp.alloc((size_t)k * rl, s);
+    D.work.alloc((exc_gemm_workspace(k, (long)rl, (long)m) + 7) / 8, s, false);  // split-K partials
     ck(exc_dgemm(1, 0, k, (long)rl, (long)m, 1.0, D.bu.p, k, D.Qg.p + reduced_n_pos[cid], (long)R, 0.0, D.tmp.p,
-                 (long)rl, nullptr, 0, s),
+                 (long)rl, D.work.p, D.work.n * 8, s),
        "combine V");
     sync();
     D.Qg.release();
