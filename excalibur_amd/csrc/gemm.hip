@@ -290,11 +290,13 @@ int gemm_impl(int ta, int tb, long M, long N, long K, double alpha, const T* A, 
   const long tm = (M + BM - 1) / BM, tn = (N + BN - 1) / BN;
   long tiles = o.lower_only ? tm * (tm + 1) / 2 : tm * tn;
   int splits = 1;
-  if (o.work && K > 4 * BK) {
-    // Split K into the number of chunks that minimizes (rounds of resident workgroups) x (chunk
-    // depth): a grid just above the resident slots would run a second, nearly empty round that
-    // costs as much as the first. Ties go to fewer splits (less partial traffic).
-    const long slots = resident_slots(sizeof(T) == 8 ? 2 : 3);
+  const long slots = resident_slots(sizeof(T) == 8 ? 2 : 3);
+  if (o.work && K > 4 * BK && tiles < slots) {
+    // Split K (only while the tiles alone leave resident slots idle: the partials of a large
+    // output cost more to sum than the split saves) into the number of chunks that minimizes
+    // (rounds of resident workgroups) x (chunk depth): a grid just above the resident slots
+    // would run a second, nearly empty round that costs as much as the first. Ties go to fewer
+    // splits (less partial traffic).
     const long maxs = std::min<long>(K / (4 * BK), 4 * ((slots + tiles - 1) / tiles));
     long best = 1, best_cost = -1;
     for (long sp = 1; sp <= std::max(1L, maxs); sp++) {
