@@ -140,9 +140,21 @@ __global__ __launch_bounds__(NT, 2) void k_gemm(GemmArgs g) {
     k1 = min(k1, ks + g.kchunk);
   }
   const Stager<T, TA, TB> st(g, i0, j0, tid);
-  // interior tile: no row or column bounds, no triangle, chunks aligned to BK; every whole chunk
-  // then loads without checks and only a last partial chunk checks its k range
-  const bool interior = i0 + BM <= g.M && j0 + BN <= g.N && g.tri == EXC_TRI_NONE && (k0 % BK) == 0;
+  // interior tile: no row or column bounds, chunks aligned to BK. A whole chunk of it then loads
+  // without checks unless it crosses the diagonal of a triangular operand (the k range already
+  // excludes the chunks wholly on the zero side), so only such chunks and a last partial chunk
+  // check their elements.
+  const bool interior = i0 + BM <= g.M && j0 + BN <= g.N && (k0 % BK) == 0;
+  auto fast = [&](long kb) {
+    if (!interior || kb + BK > k1) return false;
+    switch (g.tri) {
+      case EXC_TRI_A_LOWER: return kb + BK <= i0 + 1;       // every k <= i0 <= row
+      case EXC_TRI_A_UPPER: return kb >= i0 + BM - 1;       // every k >= last row
+      case EXC_TRI_B_LOWER: return kb >= j0 + BN - 1;       // every k >= last column
+      case EXC_TRI_B_UPPER: return kb + BK <= j0 + 1;       // every k <= j0 <= column
+      default: return true;
+    }
+  };
   const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
   acc_t acc[4][4];
 #pragma unroll
@@ -182,7 +194,7 @@ __global__ __launch_bounds__(NT, 2) void k_gemm(GemmArgs g) {
   const long kstart = (k0 / BK) * BK;
   int buf = 0;
   if (kstart < k1) {
-    if (interior && kstart + BK <= k1) load_fast(kstart);
+    if (fast(kstart)) load_fast(kstart);
     else load_checked(kstart);
     store_chunk(0);
   }
@@ -191,7 +203,7 @@ __global__ __launch_bounds__(NT, 2) void k_gemm(GemmArgs g) {
   for (long kb = kstart; kb < k1; kb += BK) {
     const bool more = kb + BK < k1;
     if (more) {
-      if (interior && kb + 2 * BK <= k1) load_fast(kb + BK);
+      if (fast(kb + BK)) load_fast(kb + BK);
       else load_checked(kb + BK);
     }
 #pragma unroll

@@ -55,6 +55,16 @@ __global__ __launch_bounds__(256) void k_scale_rows(long rows, long cols, double
   A[i * lda + j] *= s[i];
 }
 
+// B[i][j] = s[i] A[i][j] (the row scaling of k_scale_rows into another buffer)
+__global__ __launch_bounds__(256) void k_scale_rows_copy(long rows, long cols, const double* __restrict__ A, long lda,
+                                                         double* __restrict__ B, long ldb,
+                                                         const double* __restrict__ s) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= rows * cols) return;
+  const long i = e / cols, j = e - i * cols;
+  B[i * ldb + j] = A[i * lda + j] * s[i];
+}
+
 __global__ __launch_bounds__(256) void k_scale_cols(long rows, long cols, double* __restrict__ A, long lda,
                                                     const double* __restrict__ s) {
   const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -187,6 +197,8 @@ size_t qr_t_workspace(int m, long r, bool want_q) {
   w.d(m);                           // signs
   w.d(4);                           // shift
   w.i(4);                           // info
+  w.d((size_t)m * r);               // the last pass's output (L^-1 Y)
+  w.d((size_t)m * m);               // L^-1 of the last pass
   return w.used + fact::work_bytes(m, r) + 4096;
 }
 
@@ -204,6 +216,8 @@ int qr_t(int m, long r, const double* X, long ldx, const QrOut& o, int square_la
   double* sgn = w.d(m);
   double* shift = w.d(4);
   int* info = w.i(4);
+  double* Yn = w.d((size_t)m * r);
+  double* Linv = w.d((size_t)m * m);
   size_t gw_bytes;
   void* gw = w.rest(&gw_bytes);
   EXC_REQUIRE(w.used <= wb, EXC_ERR_SPACE, "qr_t: workspace too small");
@@ -240,8 +254,25 @@ int qr_t(int m, long r, const double* X, long ldx, const QrOut& o, int square_la
       shifted = true;  // redo this pass with the shift
       continue;
     }
-    rc = fact::trsm_left(0, 0, m, r, G, m, Y, ldy, gw, gw_bytes, s);
-    if (rc) return rc;
+    if (clean == 1 && !shifted) {
+      // The last pass: its input is orthonormal to about u cond(X)^2, so L is within that of
+      // the identity and multiplying by its explicit inverse (formed by substitution) is as
+      // accurate as substitution itself. One triangular-operand product then replaces the wide
+      // solve: Yn = L^-1 Y.
+      hipLaunchKernelGGL(k_identity_prefix, dim3(g1((long)m * m)), dim3(256), 0, s, m, (long)m, Linv, (long)m);
+      EXC_CHECK_HIP(hipGetLastError());
+      rc = fact::trsm_left(0, 0, m, m, G, m, Linv, m, gw, gw_bytes, s);
+      if (rc) return rc;
+      ExcGemmOpts lo = plain;
+      lo.tri = EXC_TRI_A_LOWER;
+      rc = exc_gemm_opts(0, 0, m, r, m, 1.0, Linv, m, Y, ldy, 0.0, Yn, r, lo, s);
+      if (rc) return rc;
+      Y = Yn;
+      ldy = r;
+    } else {
+      rc = fact::trsm_left(0, 0, m, r, G, m, Y, ldy, gw, gw_bytes, s);
+      if (rc) return rc;
+    }
     if (!have_rt) {
       hipLaunchKernelGGL(k_copy2d, dim3(g1((long)m * m)), dim3(256), 0, s, (long)m, (long)m, G, (long)m, o.rt,
                          (long)m);
@@ -285,7 +316,10 @@ int qr_t(int m, long r, const double* X, long ldx, const QrOut& o, int square_la
   }
   // LAPACK's signs: R <- S R (columns of R^T), Q <- Q S (rows of Q^T)
   hipLaunchKernelGGL(k_scale_cols, dim3(g1((long)m * m)), dim3(256), 0, s, (long)m, (long)m, o.rt, (long)m, sgn);
-  if (o.qt) hipLaunchKernelGGL(k_scale_rows, dim3(g1((long)m * r)), dim3(256), 0, s, (long)m, r, o.qt, o.ldq, sgn);
+  if (o.qt) {
+    if (Y == o.qt) hipLaunchKernelGGL(k_scale_rows, dim3(g1((long)m * r)), dim3(256), 0, s, (long)m, r, o.qt, o.ldq, sgn);
+    else hipLaunchKernelGGL(k_scale_rows_copy, dim3(g1((long)m * r)), dim3(256), 0, s, (long)m, r, Y, ldy, o.qt, o.ldq, sgn);
+  }
   EXC_CHECK_HIP(hipGetLastError());
   if (o.sgn) EXC_CHECK_HIP(hipMemcpyAsync(o.sgn, sgn, m * sizeof(double), hipMemcpyDeviceToDevice, s));
   return EXC_OK;
