@@ -863,13 +863,32 @@ __global__ __launch_bounds__(256) void k_apply_flags(int m, int k, double* __res
 
 // Back-rotation of client.cpp:1127-1128: for i = m-2..0, rows (i, i+1) of bu (m x k) rotated by
 // drot with (c_i, -s_i). One lane per column carries row i+1 down the recurrence.
-__global__ __launch_bounds__(256) void k_rot_back(int m, int k, double* __restrict__ bu,
-                                                  const double* __restrict__ cs) {
+__global__ __launch_bounds__(64) void k_rot_back(int m, int k, double* __restrict__ bu,
+                                                 const double* __restrict__ cs) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= k || m < 2) return;
+  // Rows are read 16 at a time into registers before the 16 rotations that use them: the
+  // stores of a block go to rows at or above its first row, which were all read already, so
+  // the loads of a block are issued together instead of each waiting behind the previous
+  // step's store. Every rotation evaluates the same expressions in the same order.
+  constexpr int RB = 16;
   double y = bu[(long)(m - 1) * k + j];
-#pragma unroll 8
-  for (int i = m - 2; i >= 0; i--) {
+  int i = m - 2;
+  for (; i >= RB - 1; i -= RB) {
+    double x[RB];
+#pragma unroll
+    for (int q = 0; q < RB; q++) x[q] = bu[(long)(i - q) * k + j];
+#pragma unroll
+    for (int q = 0; q < RB; q++) {
+      const int r = i - q;
+      const double c = cs[2 * r], s = -cs[2 * r + 1];
+      const double xn = c * x[q] + s * y;
+      const double yn = c * y - s * x[q];
+      bu[(long)(r + 1) * k + j] = yn;
+      y = xn;
+    }
+  }
+  for (; i >= 0; i--) {
     const double c = cs[2 * i], s = -cs[2 * i + 1];
     const double x = bu[(long)i * k + j];
     const double xn = c * x + s * y;
@@ -1279,7 +1298,7 @@ int exc_sign_normalize(int m, int k, double* d_bu, double* d_bv, double* d_flip,
 int exc_rot_back(int m, int k, double* d_bu, const double* d_cs, exc_stream_t stream) {
   EXC_REQUIRE(m >= 1 && k >= 0 && d_bu && d_cs, EXC_ERR_ARG, "exc_rot_back: bad arguments");
   if (k == 0 || m < 2) return EXC_OK;
-  hipLaunchKernelGGL(k_rot_back, dim3(exc_detail::ceil_div(k, 256)), dim3(256), 0, exc_detail::stream(stream), m, k,
+  hipLaunchKernelGGL(k_rot_back, dim3(exc_detail::ceil_div(k, 64)), dim3(64), 0, exc_detail::stream(stream), m, k,
                      d_bu, d_cs);
   EXC_CHECK_HIP(hipGetLastError());
   return EXC_OK;
