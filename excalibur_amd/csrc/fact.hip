@@ -174,6 +174,95 @@ __global__ __launch_bounds__(256) void k_trsm_left_blk(int kb, const double* __r
   }
 }
 
+// The same solve for a diagonal block of 64 < kb <= 128 rows, as two 64-row blocks in one launch
+// (narrow right-hand sides, where the update between two leaves would otherwise be a separate
+// small GEMM on the critical path): solve the first block (rows [0, 64) of L, or [64, kb) of U),
+// subtract its contribution from the second block's right-hand sides (one fused multiply-add per
+// coefficient, the solved unknowns broadcast from LDS), then solve the second block. Each wave
+// keeps its 128 x C right-hand sides in LDS for the whole launch.
+template <bool UPPER, bool UNIT, int C>
+__global__ __launch_bounds__(256) void k_trsm_left_blk2(int kb, const double* __restrict__ L, long ldl,
+                                                        double* __restrict__ X, long ldx, long nrhs) {
+  constexpr int SL = NB + 2, S = C + 2, RPI = 64 / C;
+  __shared__ __attribute__((aligned(16))) double Ls[NB * SL];
+  __shared__ __attribute__((aligned(16))) double Xall[4 * 2 * NB * S];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const long c0 = ((long)blockIdx.x * 4 + wave) * C;
+  double* Xs = Xall + wave * 2 * NB * S;
+  const int sr = lane / C, sc = lane % C;
+  const bool cok = c0 + sc < nrhs;
+  const long gcol = cok ? c0 + sc : nrhs - 1;
+#pragma unroll
+  for (int t = 0; t < 2 * C; t++) {
+    const int r = t * RPI + sr;
+    const double v = X[(long)(r < kb ? r : kb - 1) * ldx + gcol];
+    Xs[r * S + sc] = (r < kb && cok) ? v : 0.0;
+  }
+  // rows [b0, b0 + bn) of the first and second blocks
+  const int f0 = UPPER ? NB : 0, fn = UPPER ? kb - NB : NB;
+  const int s0 = UPPER ? 0 : NB, sn = UPPER ? NB : kb - NB;
+  // Ls <- the bn x bn diagonal triangle at (b0, b0); lanes read their scaled row into lr
+  auto stage = [&](int r0, int rn, int q0, int qn) {
+    const int lc = lane < qn ? lane : qn - 1;
+#pragma unroll
+    for (int r = wave; r < NB; r += 4)
+      Ls[r * SL + lane] = (r < rn && lane < qn) ? L[(long)(r0 + (r < rn ? r : rn - 1)) * ldl + q0 + lc] : 0.0;
+  };
+  auto solve = [&](int b0, int bn, double (&y)[C]) {
+    __syncthreads();  // earlier readers of Ls are done
+    stage(b0, bn, b0, bn);
+    __syncthreads();
+    const bool live = lane < bn;
+    const int row = live ? lane : bn - 1;
+    const double rinv = UNIT ? 1.0 : 1.0 / Ls[row * SL + row];
+    double lr[NB];
+#pragma unroll
+    for (int k = 0; k < NB; k++) {
+      const double v = Ls[row * SL + k];
+      lr[k] = (live && k < bn && (UPPER ? k > lane : k < lane)) ? (UNIT ? v : v * rinv) : 0.0;
+    }
+#pragma unroll
+    for (int c = 0; c < C; c++) y[c] = UNIT ? y[c] : y[c] * rinv;
+#pragma unroll
+    for (int q = 0; q < NB; q++) {
+      const int j = UPPER ? NB - 1 - q : q;
+      if (j >= bn) continue;  // uniform
+#pragma unroll
+      for (int c = 0; c < C; c++) y[c] = fma(-lr[j], readlane_f64(y[c], j), y[c]);
+    }
+#pragma unroll
+    for (int c = 0; c < C; c++) Xs[(b0 + lane) * S + c] = y[c];
+  };
+  __syncthreads();
+  double y[C];
+#pragma unroll
+  for (int c = 0; c < C; c++) y[c] = Xs[(f0 + lane) * S + c];
+  solve(f0, fn, y);
+  // second block: y -= O x_first, O = rows [s0, s0 + sn) x columns [f0, f0 + fn) of the triangle
+  __syncthreads();
+  stage(s0, sn, f0, fn);
+  __syncthreads();
+  {
+    double o[NB];
+#pragma unroll
+    for (int k = 0; k < NB; k++) o[k] = Ls[lane * SL + k];
+#pragma unroll
+    for (int c = 0; c < C; c++) y[c] = Xs[(s0 + lane) * S + c];
+#pragma unroll
+    for (int k = 0; k < NB; k++) {
+#pragma unroll
+      for (int c = 0; c < C; c++) y[c] = fma(-o[k], Xs[(f0 + k) * S + c], y[c]);
+    }
+  }
+  solve(s0, sn, y);
+  __syncthreads();
+#pragma unroll
+  for (int t = 0; t < 2 * C; t++) {
+    const int r = t * RPI + sr;
+    if (r < kb && cok) X[(long)r * ldx + c0 + sc] = Xs[r * S + sc];
+  }
+}
+
 // X_b <- X_b U_bb^-1 (or X_b L_bb^-T) for one diagonal block (kb <= NB), one lane per COLUMN j
 // of the unknown rows: lane j holds column j of op(U) above the diagonal (U[k][j], or L[j][k]
 // for L^T) scaled by the pivot's reciprocal, and entry j of C rows of X (contiguous across the
@@ -246,6 +335,18 @@ int launch_left_blk(int upper, int unit, int kb, const double* L, long ldl, doub
   return EXC_OK;
 }
 
+// Two-block leaves (64 < kb <= 128) for narrow right-hand sides: 4 columns per wavefront.
+int launch_left_blk2(int upper, int unit, int kb, const double* L, long ldl, double* X, long ldx, long nrhs,
+                     hipStream_t s) {
+  const dim3 grid(exc_detail::ceil_div(nrhs, 16L)), block(256);
+  if (!upper && !unit) hipLaunchKernelGGL((k_trsm_left_blk2<false, false, 4>), grid, block, 0, s, kb, L, ldl, X, ldx, nrhs);
+  if (!upper && unit) hipLaunchKernelGGL((k_trsm_left_blk2<false, true, 4>), grid, block, 0, s, kb, L, ldl, X, ldx, nrhs);
+  if (upper && !unit) hipLaunchKernelGGL((k_trsm_left_blk2<true, false, 4>), grid, block, 0, s, kb, L, ldl, X, ldx, nrhs);
+  if (upper && unit) hipLaunchKernelGGL((k_trsm_left_blk2<true, true, 4>), grid, block, 0, s, kb, L, ldl, X, ldx, nrhs);
+  EXC_CHECK_HIP(hipGetLastError());
+  return EXC_OK;
+}
+
 int launch_right_blk(int lower_t, int unit, int kb, const double* U, long ldu, double* X, long ldx, long rows,
                      hipStream_t s) {
   // as launch_left_blk: 4 rows per wavefront up to 4096 rows, else 16; four wavefronts per workgroup
@@ -286,6 +387,7 @@ int trsm_left(int upper, int unit, int n, long nrhs, const double* L, long ldl, 
   EXC_REQUIRE(n >= 0 && nrhs >= 0 && L && X, EXC_ERR_ARG, "trsm_left: bad arguments");
   if (n == 0 || nrhs == 0) return EXC_OK;
   if (n <= NB) return launch_left_blk(upper, unit, n, L, ldl, X, ldx, nrhs, s);
+  if (n <= 2 * NB && nrhs <= 4L * 1024) return launch_left_blk2(upper, unit, n, L, ldl, X, ldx, nrhs, s);
   const int n1 = split(n), n2 = n - n1;
   ExcGemmOpts o;
   o.work = work;

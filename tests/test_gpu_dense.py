@@ -53,7 +53,7 @@ def test_potrf(hip, n):
 
 @pytest.mark.parametrize("upper", [False, True])
 @pytest.mark.parametrize("unit", [False, True])
-@pytest.mark.parametrize("n,nrhs", [(3, 5), (128, 64), (300, 1000), (700, 130), (200, 5000), (64, 4100)])
+@pytest.mark.parametrize("n,nrhs", [(3, 5), (100, 37), (128, 64), (300, 1000), (700, 130), (200, 5000), (64, 4100)])
 def test_trsm_left(hip, upper, unit, n, nrhs):
     banner(f"trsm_left n={n} nrhs={nrhs} upper={upper} unit={unit}")
     rng = np.random.default_rng(n * 3 + nrhs)
