@@ -175,8 +175,8 @@ __global__ __launch_bounds__(256) void k_trsm_left_blk(int kb, const double* __r
 }
 
 // The same solve for a diagonal block of 64 < kb <= 128 rows, as two 64-row blocks in one launch
-// (narrow right-hand sides, where the update between two leaves would otherwise be a separate
-// small GEMM on the critical path): solve the first block (rows [0, 64) of L, or [64, kb) of U),
+// (the update between two leaves would otherwise be a separate small GEMM, latency-bound for
+// narrow right-hand sides and memory-bound for wide ones): solve the first block (rows [0, 64) of L, or [64, kb) of U),
 // subtract its contribution from the second block's right-hand sides (one fused multiply-add per
 // coefficient, the solved unknowns broadcast from LDS), then solve the second block. Each wave
 // keeps its 128 x C right-hand sides in LDS for the whole launch.
@@ -335,14 +335,18 @@ int launch_left_blk(int upper, int unit, int kb, const double* L, long ldl, doub
   return EXC_OK;
 }
 
-// Two-block leaves (64 < kb <= 128) for narrow right-hand sides: 4 columns per wavefront.
+// Two-block leaves (64 < kb <= 128): 4 columns per wavefront up to 4096 right-hand sides, else 8.
 int launch_left_blk2(int upper, int unit, int kb, const double* L, long ldl, double* X, long ldx, long nrhs,
                      hipStream_t s) {
-  const dim3 grid(exc_detail::ceil_div(nrhs, 16L)), block(256);
-  if (!upper && !unit) hipLaunchKernelGGL((k_trsm_left_blk2<false, false, 4>), grid, block, 0, s, kb, L, ldl, X, ldx, nrhs);
-  if (!upper && unit) hipLaunchKernelGGL((k_trsm_left_blk2<false, true, 4>), grid, block, 0, s, kb, L, ldl, X, ldx, nrhs);
-  if (upper && !unit) hipLaunchKernelGGL((k_trsm_left_blk2<true, false, 4>), grid, block, 0, s, kb, L, ldl, X, ldx, nrhs);
-  if (upper && unit) hipLaunchKernelGGL((k_trsm_left_blk2<true, true, 4>), grid, block, 0, s, kb, L, ldl, X, ldx, nrhs);
+  const bool wide = nrhs > 4L * 1024;
+  const dim3 grid(exc_detail::ceil_div(nrhs, 4L * (wide ? 8 : 4))), block(256);
+#define EXC_LEFT_BLK2(U, N)                                                                                     \
+  if ((upper != 0) == U && (unit != 0) == N) {                                                                  \
+    if (wide) hipLaunchKernelGGL((k_trsm_left_blk2<U, N, 8>), grid, block, 0, s, kb, L, ldl, X, ldx, nrhs);     \
+    else hipLaunchKernelGGL((k_trsm_left_blk2<U, N, 4>), grid, block, 0, s, kb, L, ldl, X, ldx, nrhs);          \
+  }
+  EXC_LEFT_BLK2(false, false) EXC_LEFT_BLK2(false, true) EXC_LEFT_BLK2(true, false) EXC_LEFT_BLK2(true, true)
+#undef EXC_LEFT_BLK2
   EXC_CHECK_HIP(hipGetLastError());
   return EXC_OK;
 }
@@ -387,7 +391,7 @@ int trsm_left(int upper, int unit, int n, long nrhs, const double* L, long ldl, 
   EXC_REQUIRE(n >= 0 && nrhs >= 0 && L && X, EXC_ERR_ARG, "trsm_left: bad arguments");
   if (n == 0 || nrhs == 0) return EXC_OK;
   if (n <= NB) return launch_left_blk(upper, unit, n, L, ldl, X, ldx, nrhs, s);
-  if (n <= 2 * NB && nrhs <= 4L * 1024) return launch_left_blk2(upper, unit, n, L, ldl, X, ldx, nrhs, s);
+  if (n <= 2 * NB) return launch_left_blk2(upper, unit, n, L, ldl, X, ldx, nrhs, s);
   const int n1 = split(n), n2 = n - n1;
   ExcGemmOpts o;
   o.work = work;
