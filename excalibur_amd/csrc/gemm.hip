@@ -304,19 +304,22 @@ int gemm_impl(int ta, int tb, long M, long N, long K, double alpha, const T* A, 
   int splits = 1;
   const long slots = resident_slots(sizeof(T) == 8 ? 2 : 3);
   if (o.work && K > 4 * BK && tiles < slots) {
-    // Split K (only while the tiles alone leave resident slots idle: the partials of a large
-    // output cost more to sum than the split saves) into the number of chunks that minimizes
-    // (rounds of resident workgroups) x (chunk depth): a grid just above the resident slots
-    // would run a second, nearly empty round that costs as much as the first. Ties go to fewer
-    // splits (less partial traffic).
-    const long maxs = std::min<long>(K / (4 * BK), 4 * ((slots + tiles - 1) / tiles));
-    long best = 1, best_cost = -1;
+    // Split K (only while the tiles alone leave resident slots idle) into the number of
+    // chunks that minimizes an estimate of the time: (rounds of resident workgroups) x (chunk
+    // depth) x ~3 us per 16-deep chunk (a workgroup alone on its CU waits on its loads), plus,
+    // when split, the fixed-order reduction: ~5 us and a read of every partial at ~3 TB/s. A grid
+    // just above the resident slots would run a second, nearly empty round; a large output
+    // would spend more summing partials than the split saves. Ties go to fewer splits.
+    const long maxs = std::min<long>(K / BK, 4 * ((slots + tiles - 1) / tiles));
+    long best = 1;
+    double best_cost = -1.0;
     for (long sp = 1; sp <= std::max(1L, maxs); sp++) {
       if (sp > 1 && (size_t)sp * M * N * sizeof(T) > o.work_bytes) break;
       const long rounds = (tiles * sp + slots - 1) / slots;
       const long depth = ((K + sp - 1) / sp + BK - 1) / BK;
-      const long cost = rounds * depth;
-      if (best_cost < 0 || cost < best_cost) { best = sp; best_cost = cost; }
+      double cost = 3.0 * (double)(rounds * depth);
+      if (sp > 1) cost += 5.0 + (double)sp * M * N * sizeof(T) / 3.0e6;
+      if (best_cost < 0.0 || cost < best_cost) { best = sp; best_cost = cost; }
     }
     splits = (int)best;
   }
