@@ -82,11 +82,31 @@ __global__ void k_set_interval(BisectHdr* hdr, double lo, double hi) {
 }
 
 // Counts at the root's ends and whether the root itself is a leaf (bbtsvd.cpp:21-36).
+// ldl_decomposition (bbtsvd.cpp:54-63): d = diagonal, l = subdiagonal of B^T B = L D L^T.
+__device__ __forceinline__ void ldl_seq(int n, const double* __restrict__ a, const double* __restrict__ b,
+                                        double* __restrict__ l, double* __restrict__ d) {
+  double dp = a[0] * a[0];
+  d[0] = dp;
+  for (int i = 1; i < n; i++) {
+    const double ab = a[i - 1] * b[i - 1];
+    l[i - 1] = ab / dp;
+    dp = a[i] * a[i] + b[i - 1] * b[i - 1] - (ab * ab) / dp;
+    d[i] = dp;
+  }
+}
+
+// The root's Sturm counts (lanes 0 and 1 of wavefront 0, side by side: one call with the lane's
+// end point) and, when l and d are given, the LDL factorization the twisted factorization needs
+// later, in wavefront 1 at the same time. 128 threads.
 __global__ void k_bisect_init(int n, const double* __restrict__ a, const double* __restrict__ b, double tol,
-                              BisectHdr* hdr) {
+                              BisectHdr* hdr, double* __restrict__ l, double* __restrict__ d) {
   const double lo = hdr->lo, hi = hdr->hi;
-  if (threadIdx.x == 0) hdr->nclo = sturm_count(n, a, b, lo);
-  if (threadIdx.x == 1) hdr->nchi = sturm_count(n, a, b, hi);
+  if (threadIdx.x < 2) {
+    const int c = sturm_count(n, a, b, threadIdx.x == 0 ? lo : hi);
+    if (threadIdx.x == 0) hdr->nclo = c;
+    else hdr->nchi = c;
+  }
+  if (threadIdx.x == 64 && l) ldl_seq(n, a, b, l, d);
   if (threadIdx.x == 2) {
     const double mid = (lo + hi) / 2;
     hdr->root_mid = mid;
@@ -221,14 +241,7 @@ __global__ __launch_bounds__(1024) void k_bisect_compact(BisectHdr* hdr, int top
 __global__ void k_ldl(int n, const double* __restrict__ a, const double* __restrict__ b, double* __restrict__ l,
                       double* __restrict__ d) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  double dp = a[0] * a[0];
-  d[0] = dp;
-  for (int i = 1; i < n; i++) {
-    const double ab = a[i - 1] * b[i - 1];
-    l[i - 1] = ab / dp;
-    dp = a[i] * a[i] + b[i - 1] * b[i - 1] - (ab * ab) / dp;
-    d[i] = dp;
-  }
+  ldl_seq(n, a, b, l, d);
 }
 
 // One lane per singular value: dstqds, dpqds, twist, inverse iteration for v, coupling, twist
@@ -585,8 +598,9 @@ int run_bfs(int n, const double* a, const double* b, double tol, int top_k, doub
 
 // Shared by exc_bisection and exc_bidiag_svd: the interval is already in hdr.
 int run_bisection(int n, const double* a, const double* b, double tol, int top_k, double* sigma, int* count,
-                  double* olo, double* ohi, const BisectWs& w, hipStream_t s) {
-  hipLaunchKernelGGL(k_bisect_init, dim3(1), dim3(64), 0, s, n, a, b, tol, w.hdr);
+                  double* olo, double* ohi, const BisectWs& w, hipStream_t s, double* ldl_l = nullptr,
+                  double* ldl_d = nullptr) {
+  hipLaunchKernelGGL(k_bisect_init, dim3(1), dim3(128), 0, s, n, a, b, tol, w.hdr, ldl_l, ldl_d);
   EXC_CHECK_HIP(hipGetLastError());
   int cap = min(n, top_k + 64);
   for (int attempt = 0; attempt < 2; attempt++) {
@@ -669,9 +683,12 @@ void twist_ws(int n, int k, Bump& B, TwistWs* w) {
 }
 
 int run_twisted(int n, int k, const double* omega, const double* a, const double* b, double* u, double* v,
-                int* twists, const TwistWs& w, bool apply_vsign, const double* sigma_for_sign, hipStream_t s) {
-  hipLaunchKernelGGL(k_ldl, dim3(1), dim3(64), 0, s, n, a, b, w.l, w.d);
-  EXC_CHECK_HIP(hipGetLastError());
+                int* twists, const TwistWs& w, bool apply_vsign, const double* sigma_for_sign, hipStream_t s,
+                bool ldl_done = false) {
+  if (!ldl_done) {
+    hipLaunchKernelGGL(k_ldl, dim3(1), dim3(64), 0, s, n, a, b, w.l, w.d);
+    EXC_CHECK_HIP(hipGetLastError());
+  }
   hipLaunchKernelGGL(k_twisted, dim3(exc_detail::ceil_div(k, 64)), dim3(64), 0, s, n, k, a, b, w.l, w.d, omega, u,
                      w.vs, w.S, w.P, w.DP, w.RP, twists);
   EXC_CHECK_HIP(hipGetLastError());
@@ -732,14 +749,15 @@ int exc_bidiag_svd(int k, const double* d_alpha, const double* d_beta, double* d
   hipStream_t s = exc_detail::stream(stream);
   hipLaunchKernelGGL(k_frob_bound, dim3(1), dim3(256), 0, s, k, d_alpha, d_beta, bw.hdr);
   EXC_CHECK_HIP(hipGetLastError());
-  int rc = run_bisection(k, d_alpha, d_beta, 1e-20, top_k, d_sigma, dcount, nullptr, nullptr, bw, s);
+  int rc = run_bisection(k, d_alpha, d_beta, 1e-20, top_k, d_sigma, dcount, nullptr, nullptr, bw, s,
+                         top_k > 0 ? tw.l : nullptr, top_k > 0 ? tw.d : nullptr);
   if (rc) return rc;
   int cnt = 0;
   EXC_CHECK_HIP(hipMemcpyAsync(&cnt, dcount, sizeof(int), hipMemcpyDeviceToHost, s));
   EXC_CHECK_HIP(hipStreamSynchronize(s));
   EXC_REQUIRE(cnt >= top_k, EXC_ERR_COUNT, "exc_bidiag_svd: bisection found fewer singular values than top_k");
   if (top_k == 0) return EXC_OK;
-  rc = run_twisted(k, top_k, d_sigma, d_alpha, d_beta, d_u, d_v, d_twists, tw, true, d_sigma, s);
+  rc = run_twisted(k, top_k, d_sigma, d_alpha, d_beta, d_u, d_v, d_twists, tw, true, d_sigma, s, true);
   if (rc) return rc;
   hipLaunchKernelGGL(k_check_none, dim3(exc_detail::ceil_div(top_k, 256)), dim3(256), 0, s, k, top_k, d_alpha,
                      d_sigma, d_u, d_v);
