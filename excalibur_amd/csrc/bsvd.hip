@@ -257,8 +257,12 @@ __global__ __launch_bounds__(256) void k_twisted(int n, int k, const double* __r
                                                  double* __restrict__ S, double* __restrict__ Pp,
                                                  double* __restrict__ DP, double* __restrict__ RP,
                                                  int* __restrict__ twists) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= k) return;
+  // two wavefronts per 64 values: wavefront 0 runs the recurrences; after the barrier it takes
+  // the v side (twist, inverse iteration, normalization) and wavefront 1 the u side (coupling,
+  // inverse iteration, normalization), which depend only on the recurrences' sequences
+  const int wave = threadIdx.x >> 6;
+  const int j = blockIdx.x * 64 + (threadIdx.x & 63);
+  const bool active = j < k;
   const size_t K = (size_t)k;
   auto at = [&](int i) -> size_t { return (size_t)i * K + j; };
   const double w = omega[j];
@@ -267,6 +271,7 @@ __global__ __launch_bounds__(256) void k_twisted(int n, int k, const double* __r
   // dstqds (bbtsvd.cpp:73-81, forward) and dpqds (bbtsvd.cpp:83-91, backward) are independent
   // recurrences: one loop runs both, so the two chains of dependent divisions overlap. Each
   // chain evaluates the reference's expressions in the reference's order.
+  if (wave == 0 && active) {
   double sc = -w2;
   S[at(0)] = sc;
   double pc = d[n - 1] - w2;
@@ -288,6 +293,10 @@ __global__ __launch_bounds__(256) void k_twisted(int n, int k, const double* __r
   }
   DP[at(n - 1)] = d[n - 1] + sc;
   RP[at(0)] = pc;
+  }
+  __syncthreads();  // the sequences are visible to both wavefronts of the workgroup
+  if (!active) return;
+  if (wave == 0) {
   // twist: first index of min |s + p + omega^2|
   int k0 = 0;
   double best = 0.0;
@@ -323,12 +332,15 @@ __global__ __launch_bounds__(256) void k_twisted(int n, int k, const double* __r
   #pragma unroll 16
   for (int i = 0; i < n; i++) vs[at(i)] = inv * vs[at(i)];
 
+  if (twists) twists[2 * j] = k0;
+  return;
+  }
   // coupling (bbtsvd.cpp:113-124): s[n] = -omega^2, p[n] = 1, r+[n] = 1
   auto Sv = [&](int i) -> double { return i == n ? -w2 : S[at(i)]; };
   auto Pv = [&](int i) -> double { return i == n ? 1.0 : Pp[at(i)]; };
   auto Rv = [&](int i) -> double { return i == n ? 1.0 : RP[at(i)]; };
   int k1 = 0;
-  best = 0.0;
+  double best = 0.0;
   #pragma unroll 16
   for (int i = 0; i < n; i++) {
     const double si = Sv(i), pi = Pv(i);
@@ -340,7 +352,7 @@ __global__ __launch_bounds__(256) void k_twisted(int n, int k, const double* __r
   // inverse iteration for u with l+_i = a_{i+1} / d+'_i b_i, u+_i = a_{i+1} / r+'_{i+1} b_i,
   // d+'_i = s_{i+1} / s_i d+_i, r+'_i = p_i / p_{i+1} r+_{i+1}
   u[at(k1)] = 1.0;
-  z = 1.0;
+  double z = 1.0;
   #pragma unroll 16
   for (int i = k1 - 1; i >= 0; i--) {
     const double dpn = Sv(i + 1) / Sv(i) * DP[at(i)];
@@ -357,19 +369,16 @@ __global__ __launch_bounds__(256) void k_twisted(int n, int k, const double* __r
     z = -upn * z;
     u[at(i)] = z;
   }
-  ss = 0.0;
+  double ss = 0.0;
   #pragma unroll 16
   for (int i = 0; i < n; i++) {
     const double v = u[at(i)];
     ss = ss + v * v;
   }
-  inv = 1.0 / sqrt(ss);
+  const double inv = 1.0 / sqrt(ss);
   #pragma unroll 16
   for (int i = 0; i < n; i++) u[at(i)] = inv * u[at(i)];
-  if (twists) {
-    twists[2 * j] = k0;
-    twists[2 * j + 1] = k1;
-  }
+  if (twists) twists[2 * j + 1] = k1;
 }
 
 // Sign of v (bbtsvd.cpp:943-955): p = first index of max |v_j|, signer = (B^T u_j)_p / sigma_j.
@@ -689,7 +698,7 @@ int run_twisted(int n, int k, const double* omega, const double* a, const double
     hipLaunchKernelGGL(k_ldl, dim3(1), dim3(64), 0, s, n, a, b, w.l, w.d);
     EXC_CHECK_HIP(hipGetLastError());
   }
-  hipLaunchKernelGGL(k_twisted, dim3(exc_detail::ceil_div(k, 64)), dim3(64), 0, s, n, k, a, b, w.l, w.d, omega, u,
+  hipLaunchKernelGGL(k_twisted, dim3(exc_detail::ceil_div(k, 64)), dim3(128), 0, s, n, k, a, b, w.l, w.d, omega, u,
                      w.vs, w.S, w.P, w.DP, w.RP, twists);
   EXC_CHECK_HIP(hipGetLastError());
   if (apply_vsign) {
