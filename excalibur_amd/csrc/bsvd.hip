@@ -257,15 +257,15 @@ __global__ __launch_bounds__(256) void k_twisted(int n, int k, const double* __r
                                                  double* __restrict__ S, double* __restrict__ Pp,
                                                  double* __restrict__ DP, double* __restrict__ RP,
                                                  int* __restrict__ twists) {
-  // two wavefronts per 64 values: wavefront 0 runs the recurrences; after the barrier it takes
-  // the v side (twist, inverse iteration, normalization) and wavefront 1 the u side (coupling,
-  // inverse iteration, normalization), which depend only on the recurrences' sequences
+  // four wavefronts per 64 values: wavefront 0 runs the recurrences; then wavefronts 0 and 1
+  // find the two twist indices, the four inverse-iteration sweeps (v and u, each backward and
+  // forward from its twist) run one per wavefront, and wavefronts 0 and 1 normalize v and u
   const int wave = threadIdx.x >> 6;
   const int j = blockIdx.x * 64 + (threadIdx.x & 63);
   const bool active = j < k;
   const size_t K = (size_t)k;
   auto at = [&](int i) -> size_t { return (size_t)i * K + j; };
-  const double w = omega[j];
+  const double w = active ? omega[j] : 0.0;  // lanes past k read nothing (they only join the barriers)
   const double w2 = w * w;
 
   // dstqds (bbtsvd.cpp:73-81, forward) and dpqds (bbtsvd.cpp:83-91, backward) are independent
@@ -294,91 +294,99 @@ __global__ __launch_bounds__(256) void k_twisted(int n, int k, const double* __r
   DP[at(n - 1)] = d[n - 1] + sc;
   RP[at(0)] = pc;
   }
-  __syncthreads();  // the sequences are visible to both wavefronts of the workgroup
-  if (!active) return;
-  if (wave == 0) {
-  // twist: first index of min |s + p + omega^2|
-  int k0 = 0;
-  double best = 0.0;
-  #pragma unroll 16
-  for (int i = 0; i < n; i++) {
-    const double g = S[at(i)] + Pp[at(i)] + w2;
-    const double ag = fabs(g);
-    if (i == 0 || ag < best) { best = ag; k0 = i; }
-  }
-  // inverse iteration for v (bbtsvd.cpp:65-71), l+ = d l / d+, u+ = l d / r+
-  vs[at(k0)] = 1.0;
-  double z = 1.0;
-  #pragma unroll 16
-  for (int i = k0 - 1; i >= 0; i--) {
-    const double lp = d[i] * l[i] / DP[at(i)];
-    z = -lp * z;
-    vs[at(i)] = z;
-  }
-  z = 1.0;
-  #pragma unroll 16
-  for (int i = k0 + 1; i < n; i++) {
-    const double up = l[i - 1] * d[i - 1] / RP[at(i)];
-    z = -up * z;
-    vs[at(i)] = z;
-  }
-  double ss = 0.0;
-  #pragma unroll 16
-  for (int i = 0; i < n; i++) {
-    const double v = vs[at(i)];
-    ss = ss + v * v;
-  }
-  double inv = 1.0 / sqrt(ss);
-  #pragma unroll 16
-  for (int i = 0; i < n; i++) vs[at(i)] = inv * vs[at(i)];
-
-  if (twists) twists[2 * j] = k0;
-  return;
-  }
+  __syncthreads();  // the sequences are visible to every wavefront of the workgroup
+  const int lane = threadIdx.x & 63;
+  __shared__ int twist_at[2][64];  // k0 (v side) and k1 (u side) of each value
   // coupling (bbtsvd.cpp:113-124): s[n] = -omega^2, p[n] = 1, r+[n] = 1
   auto Sv = [&](int i) -> double { return i == n ? -w2 : S[at(i)]; };
   auto Pv = [&](int i) -> double { return i == n ? 1.0 : Pp[at(i)]; };
   auto Rv = [&](int i) -> double { return i == n ? 1.0 : RP[at(i)]; };
-  int k1 = 0;
-  double best = 0.0;
-  #pragma unroll 16
-  for (int i = 0; i < n; i++) {
-    const double si = Sv(i), pi = Pv(i);
-    const double g = si + pi + w2;
-    const double gn = -w2 * g * Rv(i + 1) / (si * Pv(i + 1));
-    const double ag = fabs(gn);
-    if (i == 0 || ag < best) { best = ag; k1 = i; }
+  if (active && wave == 0) {
+    // twist: first index of min |s + p + omega^2|
+    int k0 = 0;
+    double best = 0.0;
+    #pragma unroll 16
+    for (int i = 0; i < n; i++) {
+      const double g = S[at(i)] + Pp[at(i)] + w2;
+      const double ag = fabs(g);
+      if (i == 0 || ag < best) { best = ag; k0 = i; }
+    }
+    twist_at[0][lane] = k0;
+    vs[at(k0)] = 1.0;
+    if (twists) twists[2 * j] = k0;
   }
-  // inverse iteration for u with l+_i = a_{i+1} / d+'_i b_i, u+_i = a_{i+1} / r+'_{i+1} b_i,
+  if (active && wave == 1) {
+    int k1 = 0;
+    double best = 0.0;
+    #pragma unroll 16
+    for (int i = 0; i < n; i++) {
+      const double si = Sv(i), pi = Pv(i);
+      const double g = si + pi + w2;
+      const double gn = -w2 * g * Rv(i + 1) / (si * Pv(i + 1));
+      const double ag = fabs(gn);
+      if (i == 0 || ag < best) { best = ag; k1 = i; }
+    }
+    twist_at[1][lane] = k1;
+    u[at(k1)] = 1.0;
+    if (twists) twists[2 * j + 1] = k1;
+  }
+  __syncthreads();
+  // the four inverse-iteration sweeps, one per wavefront: v (bbtsvd.cpp:65-71) with
+  // l+ = d l / d+, u+ = l d / r+; u with l+_i = a_{i+1} / d+'_i b_i, u+_i = a_{i+1} / r+'_{i+1} b_i,
   // d+'_i = s_{i+1} / s_i d+_i, r+'_i = p_i / p_{i+1} r+_{i+1}
-  u[at(k1)] = 1.0;
-  double z = 1.0;
-  #pragma unroll 16
-  for (int i = k1 - 1; i >= 0; i--) {
-    const double dpn = Sv(i + 1) / Sv(i) * DP[at(i)];
-    const double lpn = a[i + 1] / dpn * b[i];
-    z = -lpn * z;
-    u[at(i)] = z;
+  if (active) {
+    double z = 1.0;
+    if (wave == 0) {
+      const int k0 = twist_at[0][lane];
+      #pragma unroll 16
+      for (int i = k0 - 1; i >= 0; i--) {
+        const double lp = d[i] * l[i] / DP[at(i)];
+        z = -lp * z;
+        vs[at(i)] = z;
+      }
+    } else if (wave == 1) {
+      const int k0 = twist_at[0][lane];
+      #pragma unroll 16
+      for (int i = k0 + 1; i < n; i++) {
+        const double up = l[i - 1] * d[i - 1] / RP[at(i)];
+        z = -up * z;
+        vs[at(i)] = z;
+      }
+    } else if (wave == 2) {
+      const int k1 = twist_at[1][lane];
+      #pragma unroll 16
+      for (int i = k1 - 1; i >= 0; i--) {
+        const double dpn = Sv(i + 1) / Sv(i) * DP[at(i)];
+        const double lpn = a[i + 1] / dpn * b[i];
+        z = -lpn * z;
+        u[at(i)] = z;
+      }
+    } else {
+      const int k1 = twist_at[1][lane];
+      #pragma unroll 16
+      for (int i = k1 + 1; i < n; i++) {
+        const int q = i - 1;
+        const double rpn = Pv(q + 1) / Pv(q + 2) * Rv(q + 2);
+        const double upn = a[q + 1] / rpn * b[q];
+        z = -upn * z;
+        u[at(i)] = z;
+      }
+    }
   }
-  z = 1.0;
-  #pragma unroll 16
-  for (int i = k1 + 1; i < n; i++) {
-    const int q = i - 1;
-    const double rpn = Pv(q + 1) / Pv(q + 2) * Rv(q + 2);
-    const double upn = a[q + 1] / rpn * b[q];
-    z = -upn * z;
-    u[at(i)] = z;
+  __syncthreads();
+  // normalization: the sum of squares in index order, then the scaling (wave 0: v, wave 1: u)
+  if (active && wave < 2) {
+    double* vec = wave == 0 ? vs : u;
+    double ss = 0.0;
+    #pragma unroll 16
+    for (int i = 0; i < n; i++) {
+      const double v = vec[at(i)];
+      ss = ss + v * v;
+    }
+    const double inv = 1.0 / sqrt(ss);
+    #pragma unroll 16
+    for (int i = 0; i < n; i++) vec[at(i)] = inv * vec[at(i)];
   }
-  double ss = 0.0;
-  #pragma unroll 16
-  for (int i = 0; i < n; i++) {
-    const double v = u[at(i)];
-    ss = ss + v * v;
-  }
-  const double inv = 1.0 / sqrt(ss);
-  #pragma unroll 16
-  for (int i = 0; i < n; i++) u[at(i)] = inv * u[at(i)];
-  if (twists) twists[2 * j + 1] = k1;
 }
 
 // Sign of v (bbtsvd.cpp:943-955): p = first index of max |v_j|, signer = (B^T u_j)_p / sigma_j.
@@ -698,7 +706,7 @@ int run_twisted(int n, int k, const double* omega, const double* a, const double
     hipLaunchKernelGGL(k_ldl, dim3(1), dim3(64), 0, s, n, a, b, w.l, w.d);
     EXC_CHECK_HIP(hipGetLastError());
   }
-  hipLaunchKernelGGL(k_twisted, dim3(exc_detail::ceil_div(k, 64)), dim3(128), 0, s, n, k, a, b, w.l, w.d, omega, u,
+  hipLaunchKernelGGL(k_twisted, dim3(exc_detail::ceil_div(k, 64)), dim3(256), 0, s, n, k, a, b, w.l, w.d, omega, u,
                      w.vs, w.S, w.P, w.DP, w.RP, twists);
   EXC_CHECK_HIP(hipGetLastError());
   if (apply_vsign) {
