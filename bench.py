@@ -33,7 +33,7 @@ CONFIGS = {
     "c2": (2, 50_000, 1_000, "geometric6"),
     "c1": (2, 500, 100, "gaussian"),
     "small": (2, 6_000, 200, "geometric6"),
-    "cpu_sample": (2, 15_000, 400, "geometric6"),
+    "cpu_sample": (2, 10_000, 1_000, "geometric6"),
 }
 
 
@@ -132,27 +132,74 @@ def isolated_gemms(n, rows, reps=5):
     return out
 
 
+def cpu_info():
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return model, os.cpu_count() or 1
+
+
 def cpu_baseline(sample="cpu_sample"):
-    """The oracle (CPU FP64 restatement, one core) on a bounded sample of the workload, scaled to the
-    full configuration by the ratio of the pipeline's dominant work (m^2 r terms)."""
+    """The oracle (the CPU FP64 restatement of the path) timed on the host cores on a bounded
+    sample of the workload: the whole simulated run() of both peers in one process, its
+    independent columns and rows split over OMP_NUM_THREADS threads. Measured, not scaled."""
     from oracle import pyoracle as O
     xs, _, _ = make_blocks(sample, seed=1)
+    model, nproc = cpu_info()
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(nproc, 16)
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    O.lib()
     t0 = time.time()
     run = O.Run(xs, [0] * len(xs))
     dt = time.time() - t0
     P, rows, n, _ = CONFIGS[sample]
-    Pf, rowsf, nf, _ = CONFIGS["c2"]
+    stages = {k: round(run.timer(k), 4) for k in ("time_local_qr", "time_global_qr", "time_bi_diag", "time_bsvd",
+                                                   "time_combine_svd_remove_mask")}
+    return {"value": round(dt, 4), "unit": "s", "cores": threads, "kind": "port",
+            "sample": f"oracle (C++ FP64 restatement of the path, one process simulating both peers, "
+                      f"{threads} OpenMP threads) on {P} peers x {rows} x {n} rows of A with the same "
+                      f"spectrum as the workload, full SVD; measured, not scaled",
+            "nproc": nproc, "cpu_model": model, "stages_s": stages}
 
-    def work(P_, r, m):
-        # local QR + orgqr (2 x 2 r m^2), final V GEMM (2 r m^2), global QR / Ralha / combine (O(P m^3))
-        return P_ * (6.0 * r * m * m) + P_ * 10.0 * m ** 3
 
-    scale = work(Pf, rowsf, nf) / work(P, rows, n)
-    return {"value": dt * scale, "unit": "s", "cores": 1, "kind": "port",
-            "sample": f"oracle (C++ FP64 restatement, single thread) on {P} peers x {rows} x {n} "
-                      f"(geometric spectrum) took {dt:.3f} s; scaled x{scale:.1f} by the m^2 r + m^3 work "
-                      f"ratio to {Pf} x {rowsf} x {nf}",
-            "sample_seconds": dt}
+def run_variant(xs, form, device_exchange, steps, warmup, dump, workdir):
+    """One run_peers call: warmup + steps repeats of run() on the resident input. Returns the step
+    times (max over peers), the stage breakdown, the HIP-event kernel records and the peers."""
+    from excalibur_amd.launch import run_peers
+    repeat = warmup + steps
+    res = run_peers(xs, seed=0, mode=1, topk=-1, opt=3, form=form, dump=dump, workdir=workdir, timeout=1200,
+                    extra=["--repeat", str(repeat), "--profile", "1", "--device-exchange", str(device_exchange)])
+    per_peer = []
+    for p in res["peers"]:
+        runs, profs = [], []
+        for line in p["log"].splitlines():
+            if line.startswith("EXC_RESULT "):
+                runs.append(json.loads(line[len("EXC_RESULT "):]))
+            elif line.startswith("EXC_PROF "):
+                profs.append(json.loads(line[len("EXC_PROF "):]))
+        per_peer.append((runs, profs))
+    timed = range(warmup, repeat)
+    step_times = [max(pp[0][r]["total"] for pp in per_peer) for r in timed]
+    keys = ["time_local_qr", "time_apply_mask", "time_exchange_low_dim", "time_global_qr", "time_bi_diag",
+            "time_ralha_allreduce", "time_bsvd", "time_combine_svd_remove_mask", "communication"]
+    keys += [k for k in per_peer[0][0][warmup] if k.startswith("time_refine")]
+    stages = {k: float(np.mean([max(pp[0][r].get(k, 0.0) for pp in per_peer) for r in timed])) for k in keys}
+    # the copy of the results into the host members happens after "Finish", on the last run only
+    d2h = max(pp[0][repeat - 1].get("time_results_d2h", 0.0) for pp in per_peer)
+    agg = {}
+    for pp in per_peer:
+        for r in timed:
+            for name, v in pp[1][r].items():
+                a = agg.setdefault(name, {"launches": 0, "ms": 0.0, "flops": 0.0, "bytes": 0.0})
+                for f in ("launches", "ms", "flops", "bytes"):
+                    a[f] += v[f]
+    return {"step_times": step_times, "ms_per_step": 1000.0 * float(np.mean(step_times)), "stages": stages,
+            "agg": agg, "steps": len(timed), "results_d2h_s": d2h, "res": res}
 
 
 def main():
@@ -162,6 +209,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
     ap.add_argument("--form", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--device-exchange", type=int, default=0, choices=[0, 1],
+                    help="1: peers on one device exchange data in device memory instead of host TCP")
+    ap.add_argument("--no-variants", action="store_true",
+                    help="skip the extra runs (device-memory exchange, the other form, the CPU sample on the GPU)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--workdir", default=None)
@@ -190,7 +241,7 @@ def main():
         P, rows, n, _ = CONFIGS[args.config]
         print(json.dumps(isolated_gemms(n, rows)))
         return
-    from excalibur_amd.launch import run_peers, write_peer_files
+    from excalibur_amd.launch import write_peer_files
 
     if args.prepare:
         xs, _, _ = make_blocks(args.config)
@@ -202,72 +253,64 @@ def main():
     xs, sigma_true, A = make_blocks(args.config)
     log(f"generated {args.config} in {time.time() - t0:.1f} s: {len(xs)} peers x {xs[0].shape[1]} x {xs[0].shape[0]}")
     workdir = args.workdir or tempfile.mkdtemp(prefix="exc_bench_")
-    repeat = args.warmup + args.steps
     t0 = time.time()
-    res = run_peers(xs, seed=0, mode=1, topk=-1, opt=3, form=args.form, dump=not args.no_accuracy,
-                    workdir=workdir, timeout=1200, extra=["--repeat", str(repeat), "--profile", "1"])
-    log(f"peers finished in {time.time() - t0:.1f} s")
-    # per-run results: EXC_RESULT / EXC_PROF lines in order, one per repeat
-    per_peer = []
-    for p in res["peers"]:
-        runs, profs = [], []
-        for line in p["log"].splitlines():
-            if line.startswith("EXC_RESULT "):
-                runs.append(json.loads(line[len("EXC_RESULT "):]))
-            elif line.startswith("EXC_PROF "):
-                profs.append(json.loads(line[len("EXC_PROF "):]))
-        per_peer.append((runs, profs))
-    timed = range(args.warmup, repeat)
-    step_times = [max(pp[0][r]["total"] for pp in per_peer) for r in timed]
-    ms_per_step = 1000.0 * float(np.mean(step_times))
-    # stage breakdown (max over peers, mean over timed steps)
-    keys = ["time_local_qr", "time_apply_mask", "time_exchange_low_dim", "time_global_qr", "time_bi_diag",
-            "time_ralha_allreduce", "time_bsvd", "time_combine_svd_remove_mask", "communication"]
-    stages = {k: float(np.mean([max(pp[0][r][k] for pp in per_peer) for r in timed])) for k in keys}
-    # dominant kernel: the FP64 MFMA GEMM (HIP events on its stream, all launches of both peers)
-    agg = {}
-    for pp in per_peer:
-        for r in timed:
-            for name, v in pp[1][r].items():
-                a = agg.setdefault(name, {"launches": 0, "ms": 0.0, "flops": 0.0, "bytes": 0.0})
-                for f in ("launches", "ms", "flops", "bytes"):
-                    a[f] += v[f]
+    main_run = run_variant(xs, args.form, args.device_exchange, args.steps, args.warmup, not args.no_accuracy,
+                           workdir)
+    log(f"peers finished in {time.time() - t0:.1f} s: {main_run['ms_per_step']:.2f} ms per SVD")
+    step_times, ms_per_step, stages, agg = (main_run["step_times"], main_run["ms_per_step"], main_run["stages"],
+                                            main_run["agg"])
+    nsteps = main_run["steps"]
+
     def rate(rec):
         return rec["flops"] / (rec["ms"] * 1e-3) / 1e12 if rec["ms"] > 0 else 0.0
-    g = agg.get("gemm_f64", {"launches": 0, "ms": 0.0, "flops": 0.0, "bytes": 0.0})
+    gname = "gemm_f64"
+    g = agg.get(gname, {"launches": 0, "ms": 0.0, "flops": 0.0, "bytes": 0.0})
     fv = agg.get("gemm_f64_final_v", {"launches": 0, "ms": 0.0, "flops": 0.0, "bytes": 0.0})
     achieved = rate(g)
     roofline = {"bound": "mfma", "kernel": "k_gemm<double> (all FP64 MFMA GEMM launches of both peers)",
                 "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": None,
                 "traffic_final_v": pmc_traffic(),
-                "launches_per_step": g["launches"] / max(1, len(timed)),
+                "launches_per_step": g["launches"] / max(1, nsteps),
                 "avg_launch_us": round(1000.0 * g["ms"] / max(1, g["launches"]), 2),
-                "flops_per_step": g["flops"] / max(1, len(timed)),
-                "final_v": {"launches_per_step": fv["launches"] / max(1, len(timed)),
+                "flops_per_step": g["flops"] / max(1, nsteps),
+                "final_v": {"launches_per_step": fv["launches"] / max(1, nsteps),
                             "avg_launch_us": round(1000.0 * fv["ms"] / max(1, fv["launches"]), 2),
                             "achieved": round(rate(fv), 3), "frac": round(rate(fv) / PEAK_FP64_TFLOPS, 4)},
                 "note": "in-pipeline: both peer processes share the GPU, so a launch's duration includes "
                         "the other peer's concurrent work; 'isolated' is the same kernel alone"}
-    kernels = {k: {"launches": v["launches"] / len(timed), "ms": v["ms"] / len(timed),
+    kernels = {k: {"launches": v["launches"] / nsteps, "ms": v["ms"] / nsteps,
                    "achieved": (v["flops"] / (v["ms"] * 1e-3) / 1e12) if v["ms"] > 0 else 0.0,
                    "gbs": (v["bytes"] / (v["ms"] * 1e-3) / 1e9) if v["ms"] > 0 else 0.0}
                for k, v in agg.items()}
     acc = {}
     if not args.no_accuracy:
         t0 = time.time()
-        acc = accuracy(res["peers"], xs, sigma_true, A)
+        acc = accuracy(main_run["res"]["peers"], xs, sigma_true, A)
         log(f"accuracy checked in {time.time() - t0:.1f} s: {acc}")
     P, rows, n, _ = CONFIGS[args.config]
     try:
         roofline["isolated"] = isolated_gemms(n, rows)
     except Exception as e:  # the measurement is extra information; the run itself already finished
         log(f"isolated GEMM timing failed: {e}")
+    variants = {}
+    if not args.no_variants:
+        # the same workload with the exchanges in device memory (IPC), a deviation from host TCP
+        t0 = time.time()
+        dx = run_variant(xs, args.form, 1 - args.device_exchange, args.steps, args.warmup, False, None)
+        variants["device_exchange" if args.device_exchange == 0 else "host_tcp"] = {
+            "ms_per_step": round(dx["ms_per_step"], 3), "step_times_s": [round(s, 6) for s in dx["step_times"]],
+            "stages_s": dx["stages"]}
+        log(f"device-exchange variant in {time.time() - t0:.1f} s: {dx['ms_per_step']:.2f} ms")
     cpu = None
     if not args.no_cpu:
         t0 = time.time()
         cpu = cpu_baseline()
-        log(f"cpu baseline in {time.time() - t0:.1f} s")
+        log(f"cpu baseline in {time.time() - t0:.1f} s: {cpu['value']} s")
+        if not args.no_variants:
+            sx, _, _ = make_blocks("cpu_sample", seed=1)
+            gs = run_variant(sx, args.form, args.device_exchange, 2, 1, False, None)
+            cpu["gpu_same_sample_s"] = round(gs["ms_per_step"] / 1000.0, 6)
     line = {
         "metric": "SVD wall-time (s) + % roofline, mixed vs plain FP64, at <=1e-12 rel sigma err",
         "value": round(ms_per_step / 1000.0, 6),
@@ -282,16 +325,19 @@ def main():
         "dtype": "f64" if args.form == "fp64" else "f32+f64",
         "data": "synthetic",
         "form": args.form,
+        "transport": "host TCP (loopback)" if args.device_exchange == 0 else "device memory (IPC)",
         "config": {"workload": f"{P} peers x {rows} x {n} float64 row blocks of A = Q1 diag(sigma) Q2^T, "
                                f"sigma_i = 10^(-6i/{n - 1}); full SVD, Givens masks, bisection + twisted "
                                f"(mode 1, opt 3)" if args.config != "c1" else
                                f"{P} peers x {rows} x {n} N(0,1) row blocks, full SVD",
                    "peers": P, "rows_per_peer": rows, "columns": n},
-        "step_times_s": [round(t, 6) for t in step_times],
+        "step_times_s": [round(s, 6) for s in step_times],
         "stages_s": stages,
+        "results_d2h_s": round(main_run["results_d2h_s"], 6),
         "accuracy": acc,
         "roofline": roofline,
         "kernels": kernels,
+        "variants": variants,
         "cpu_baseline": cpu,
         "reference_cpu_measured_s": 4.99,
     }

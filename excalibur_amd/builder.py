@@ -64,6 +64,19 @@ def build(verbose=False, jobs=None):
         if r.returncode != 0:
             raise RuntimeError(f"link failed\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
         os.replace(tmp, PEER)
+    # the C++ test driver that constructs the client as reference code does (host code only)
+    drv = os.path.join(ROOT, "tests", "cpp", "ref_ctor_main.cpp")
+    if os.path.exists(drv):
+        out_dir = os.path.join(ROOT, "tests", "cpp", "_build")
+        os.makedirs(out_dir, exist_ok=True)
+        out = os.path.join(out_dir, "ref_ctor_main")
+        cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-I", os.path.join(ROOT, "include"), drv, "-o", out + ".tmp",
+               "-L", HERE, "-lexcalibur_amd", "-Wl,-rpath," + HERE, "-Wl,-rpath,$ORIGIN/../../../excalibur_amd",
+               "-lpthread"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"test driver build failed\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+        os.replace(out + ".tmp", out)
     if verbose:
         print("built", LIB)
 

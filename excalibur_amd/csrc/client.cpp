@@ -76,7 +76,7 @@ struct HBuf {
   ~HBuf() {
     if (p) exc_host_free(p);
   }
-  void alloc(size_t count) {
+  void alloc(size_t count, bool zero = true) {
     if (!p || cap < count) {
       if (p) exc_host_free(p);
       void* q = nullptr;
@@ -85,7 +85,7 @@ struct HBuf {
       cap = std::max<size_t>(count, 1);
     }
     n = count;
-    memset(p, 0, std::max<size_t>(count, 1) * sizeof(double));
+    if (zero) memset(p, 0, std::max<size_t>(count, 1) * sizeof(double));
   }
 };
 
@@ -96,10 +96,14 @@ struct ClientDevice {
   DBuf masked, P, Vt, tau_g, Qg, F, U, alpha, beta, house, tau_h, cs, bu, bv, sigma, flip, part, FinalU, tmp,
       FinalVT, msg, g, scal, work, plan, rt, msgd, b2, cnt2;
   HBuf h_masked, h_repart, h_msg, h_U;
+  // the results as the host members hold them (pinned, so the copies run at the link's rate)
+  HBuf r_sigma, r_FinalU, r_FinalVT, r_alpha, r_beta;
   void* link = nullptr;  // message link of the one-launch Ralha loop (coherent pinned memory)
   int link_m = 0;
   // Device-memory message exchange among peers on one device: -1 undecided, 0 off, 1 on.
   int boxes_state = -1;
+  // Whether this run uses the one-launch bidiagonalization loop, decided by all peers together.
+  int loop_ok = 0;
   void* box = nullptr;                // this peer's box
   std::vector<void*> boxes;           // every peer's box, opened through IPC (own box at cid)
   std::vector<int> box_nwg;           // every peer's loop grid size
@@ -325,7 +329,7 @@ void DecentralizedClient::decide_device_exchange(int mi, int ci, bool one_launch
   const int P = num_client, cid = client_id;
   // this peer's record: the loop applies, its grid size, and the identity of its device
   struct Rec {
-    int usable, nwg;
+    int usable, nwg, slots;
     char where[160];
   } me{};
   int dev = 0;
@@ -342,21 +346,31 @@ void DecentralizedClient::decide_device_exchange(int mi, int ci, bool one_launch
   snprintf(me.where, sizeof me.where, "%s|%s|%s", host, boot.c_str(), bus);
   me.usable = (opt_.device_exchange && P <= 16) ? 1 : 0;
   me.nwg = one_launch ? exc_ralha_loop_grid(mi, ci, 0) : 0;
+  me.slots = one_launch ? exc_ralha_loop_slots(mi, ci, 0) : 0;
   net.broadcast(&me, sizeof me);
   std::vector<Rec> all(P);
   all[cid] = me;
   for (int j = 0; j < P; j++)
     if (j != cid) net.receive_into(j, &all[j], sizeof(Rec));
-  bool same = true, loops = true;
+  bool samedev = true, usable = true;
+  std::vector<int> nwg(P), slots(P);
   for (int j = 0; j < P; j++) {
-    same = same && all[j].usable == 1 && strncmp(all[j].where, me.where, sizeof me.where) == 0;
-    loops = loops && all[j].nwg > 0;
+    samedev = samedev && strncmp(all[j].where, me.where, sizeof me.where) == 0;
+    usable = usable && all[j].usable == 1;
+    nwg[j] = all[j].nwg;
+    slots[j] = all[j].slots;
   }
+  // Every peer takes the same decision from the same records. The loops wait on each other's
+  // messages, so on one device all of them must be resident together, on TCP as on the boxes.
+  D.loop_ok = exc_ralha_loops_colocate(P, nwg.data(), slots.data(), samedev ? 1 : 0);
+  if (one_launch && !D.loop_ok)
+    log("the peers' bidiagonalization loops do not fit on the device together: one launch per step");
+  const bool same = samedev && usable;
   D.colocated = (same && P > 1) ? 1 : 0;
   D.peer_handle.assign(P, std::string());
   D.peer_outbox.assign(P, nullptr);
   if (D.colocated) log("bulk exchanges among " + std::to_string(P) + " peers in device memory");
-  if (!same || !loops) {
+  if (!same || !D.loop_ok) {
     D.boxes_state = 0;
     return;
   }
@@ -425,6 +439,8 @@ void DecentralizedClient::load_data() {
 
 void DecentralizedClient::init_server_and_client() {
   net_.reset(new exc::PeerNet(client_id, num_client, listen, base_server_port, opt_.recv_timeout_s));
+  my_next_client = (client_id + 1) % num_client;  // client.cpp:163-164
+  my_front_client = (client_id - 1 + num_client) % num_client;
   net_->connect_all(std::max(60.0, opt_.recv_timeout_s));
   // this peer's share of the bidiagonalization columns (client.cpp:613-620)
   const size_t c = m / num_client + (client_id < (int)(m % num_client) ? 1 : 0);
@@ -684,7 +700,7 @@ void DecentralizedClient::run() {
     ck(exc_ralha_tail_update(-1, mi, ci, D.F.p, D.U.p, nullptr, D.beta.p, D.scal.p, s), "ralha init");
     allreduce_dev(D.scal.p, D.scal.p + 1, 1);
     ck(exc_ralha_tail_scale(0, mi, ci, D.F.p, D.U.p, D.scal.p + 1, D.alpha.p, s), "ralha init");
-    const bool one_launch = opt_.ralha_one_launch && exc_ralha_loop_fits(mi, ci, 0) == 1;
+    const bool one_launch = D.loop_ok == 1;
     if (one_launch) {
       // steps -1 .. m-3 in one launch; the messages go through device-memory boxes when every
       // peer shares this device, else this thread all-reduces them over the network
@@ -898,39 +914,31 @@ void DecentralizedClient::run() {
   log("time_combine_svd_remove_mask=" + std::to_string(time_combine_svd_remove_mask));
   log("time_ralha_allreduce=" + std::to_string(time_ralha_allreduce));
 
+  sigma = FinalU = FinalVT = x_diagonal = x_upper_diagonal = nullptr;
+  time_results_d2h = 0;
   if (opt_.keep_results || opt_.dump_results) {
-    sigma.assign(k, 0.0);
-    ckh(hipMemcpyAsync(sigma.data(), D.sigma.p, k * 8, hipMemcpyDeviceToHost, s), "d2h");
-    if (compute_u) {
-      FinalU.assign(m * (size_t)k, 0.0);
-      ckh(hipMemcpyAsync(FinalU.data(), D.FinalU.p, FinalU.size() * 8, hipMemcpyDeviceToHost, s), "d2h");
-    }
-    if (compute_v) {
-      const double* src = lqr ? D.FinalVT.p : D.tmp.p;
-      FinalVT.assign((size_t)k * local_n, 0.0);
-      ckh(hipMemcpyAsync(FinalVT.data(), src, FinalVT.size() * 8, hipMemcpyDeviceToHost, s), "d2h");
-    }
-    sync();
+    const double td = now_seconds();
+    copy_results_to_host(k, compute_u, compute_v, lqr);
+    time_results_d2h = now_seconds() - td;
+    log("time_results_d2h=" + std::to_string(time_results_d2h));
     if (opt_.dump_results) {
-      auto dump = [&](const std::string& name, const std::vector<double>& v) {
+      auto dump = [&](const std::string& name, const double* v, size_t count) {
         FILE* f = fopen((log_dir + "/Client" + std::to_string(cid) + "_" + name + ".bin").c_str(), "wb");
         if (f) {
-          fwrite(v.data(), sizeof(double), v.size(), f);
+          fwrite(v, sizeof(double), count, f);
           fclose(f);
         }
       };
-      dump("sigma", sigma);
-      if (compute_u) dump("FinalU", FinalU);
-      if (compute_v) dump("FinalVT", FinalVT);
+      dump("sigma", sigma, k);
+      if (compute_u) dump("FinalU", FinalU, m * (size_t)k);
+      if (compute_v) dump("FinalVT", FinalVT, (size_t)k * local_n);
       // the bidiagonal and its vectors, for stage-by-stage comparison
-      std::vector<double> a(m), b(m - 1), u(m * (size_t)k), v(m * (size_t)k);
-      ckh(hipMemcpyAsync(a.data(), D.alpha.p, m * 8, hipMemcpyDeviceToHost, s), "d2h");
-      ckh(hipMemcpyAsync(b.data(), D.beta.p, (m - 1) * 8, hipMemcpyDeviceToHost, s), "d2h");
+      std::vector<double> v(m * (size_t)k);
       ckh(hipMemcpyAsync(v.data(), D.bv.p, v.size() * 8, hipMemcpyDeviceToHost, s), "d2h");
       sync();
-      dump("alpha", a);
-      dump("beta", b);
-      dump("bidiag_v", v);
+      dump("alpha", x_diagonal, m);
+      dump("beta", x_upper_diagonal, m - 1);
+      dump("bidiag_v", v.data(), v.size());
     }
   }
   char json[1024];
@@ -938,13 +946,111 @@ void DecentralizedClient::run() {
            "{\"cid\": %d, \"sigma_count\": %d, \"total\": %.6f, \"communication\": %.6f, \"time_init\": %.6f, "
            "\"time_local_qr\": %.6f, \"time_global_qr\": %.6f, \"time_apply_mask\": %.6f, "
            "\"time_exchange_low_dim\": %.6f, \"time_bi_diag\": %.6f, \"time_ralha_allreduce\": %.6f, "
-           "\"time_bsvd\": %.6f, \"time_combine_svd_remove_mask\": %.6f}",
+           "\"time_bsvd\": %.6f, \"time_combine_svd_remove_mask\": %.6f, \"time_results_d2h\": %.6f}",
            cid, k, time_total, time_communication, time_init, time_local_qr, time_global_qr, time_apply_mask,
-           time_exchange_low_dim, time_bi_diag, time_ralha_allreduce, time_bsvd, time_combine_svd_remove_mask);
+           time_exchange_low_dim, time_bi_diag, time_ralha_allreduce, time_bsvd, time_combine_svd_remove_mask,
+           time_results_d2h);
   printf("EXC_RESULT %s\n", json);
   if (opt_.profile) {
     std::vector<char> pb(1 << 16);
     if (exc_prof_report(pb.data(), pb.size()) == EXC_OK) printf("EXC_PROF %s\n", pb.data());
   }
   fflush(stdout);
+}
+
+void DecentralizedClient::copy_results_to_host(int k, bool compute_u, bool compute_v, bool lqr) {
+  exc::ClientDevice& D = *dev_;
+  hipStream_t s = D.s;
+  D.r_sigma.alloc(std::max(k, 1), false);
+  ckh(hipMemcpyAsync(D.r_sigma.p, D.sigma.p, k * 8, hipMemcpyDeviceToHost, s), "d2h");
+  sigma = D.r_sigma.p;
+  D.r_alpha.alloc(m, false);
+  D.r_beta.alloc(m, false);
+  ckh(hipMemcpyAsync(D.r_alpha.p, D.alpha.p, m * 8, hipMemcpyDeviceToHost, s), "d2h");
+  ckh(hipMemcpyAsync(D.r_beta.p, D.beta.p, (m - 1) * 8, hipMemcpyDeviceToHost, s), "d2h");
+  x_diagonal = D.r_alpha.p;
+  x_upper_diagonal = D.r_beta.p;
+  if (compute_u) {
+    D.r_FinalU.alloc(m * (size_t)k, false);
+    ckh(hipMemcpyAsync(D.r_FinalU.p, D.FinalU.p, m * (size_t)k * 8, hipMemcpyDeviceToHost, s), "d2h");
+    FinalU = D.r_FinalU.p;
+  }
+  if (compute_v) {
+    const double* src = lqr ? D.FinalVT.p : D.tmp.p;
+    D.r_FinalVT.alloc((size_t)k * local_n, false);
+    ckh(hipMemcpyAsync(D.r_FinalVT.p, src, (size_t)k * local_n * 8, hipMemcpyDeviceToHost, s), "d2h");
+    FinalVT = D.r_FinalVT.p;
+  }
+  ckh(hipStreamSynchronize(s), "d2h");
+}
+
+void DecentralizedClient::remove_input_data() {  // client.cpp:215-218
+  remove((dpath + "/Client" + std::to_string(client_id) + ".mat").c_str());
+  if (client_id == 0) remove((dpath + "/command.txt").c_str());
+}
+
+std::string DecentralizedClient::filename_with_cid(std::string filename) {  // client.cpp:413-415
+  return log_dir + "/Client" + std::to_string(client_id) + "_" + filename;
+}
+
+void DecentralizedClient::send_to(int cid, char* buffer, size_t size) { net_->send_to(cid, buffer, size); }
+
+char* DecentralizedClient::receive_frame(int cid) {
+  std::vector<char> f = net_->receive(cid);
+  char* p = new char[std::max<size_t>(f.size(), 1)];
+  if (!f.empty()) memcpy(p, f.data(), f.size());
+  return p;
+}
+
+void DecentralizedClient::broadcast(char* buffer, size_t size) { net_->broadcast(buffer, size); }
+
+void DecentralizedClient::ring_all_reduce_sum(double* buffer, int size) {
+  const double t0 = now_seconds();
+  net_->ring_all_reduce_sum(buffer, size);
+  time_communication += now_seconds() - t0;
+}
+
+void DecentralizedClient::all_gather(char** buffer, size_t size, int* c_group, int nc, int src) {
+  std::vector<int> grp(c_group, c_group + nc);
+  std::vector<char> b;
+  if (client_id == src) b.assign(*buffer, *buffer + size);
+  const double t0 = now_seconds();
+  net_->all_gather(b, size, grp, src);
+  time_communication += now_seconds() - t0;
+  if (client_id != src) {
+    char* p = new char[std::max<size_t>(size, 1)];
+    if (size) memcpy(p, b.data(), size);
+    *buffer = p;
+  }
+}
+
+void DecentralizedClient::local_qr_reduction(size_t mr, size_t n, double** raw_x, double* q, bool transpose_x,
+                                             bool compute_q) {
+  if (!(mr > n) || n < 1) throw std::invalid_argument("local_qr_reduction needs m > n >= 1");
+  hipStream_t s = dev_->s;
+  DBuf x, rt, work;
+  x.alloc(mr * n, s, false);
+  rt.alloc(n * n, s);
+  const size_t wb = exc_qr_t_workspace((int)n, (long)mr, compute_q ? 1 : 0);
+  work.alloc((wb + 7) / 8, s, false);
+  // the column-major mr x n input is the row-major n x mr matrix exc_qr_t factorizes as X^T
+  ckh(hipMemcpyAsync(x.p, *raw_x, mr * n * 8, hipMemcpyHostToDevice, s), "h2d");
+  ck(exc_qr_t((int)n, (long)mr, x.p, (long)mr, rt.p, compute_q ? x.p : nullptr, (long)mr, nullptr, 0, nullptr,
+              nullptr, 0, work.p, work.n * 8, s),
+     "local QR");
+  std::vector<double> hrt(n * n);
+  ckh(hipMemcpyAsync(hrt.data(), rt.p, n * n * 8, hipMemcpyDeviceToHost, s), "d2h");
+  // Q (mr x n column-major) is Q^T row-major, the layout exc_qr_t leaves
+  if (compute_q) ckh(hipMemcpyAsync(q, x.p, mr * n * 8, hipMemcpyDeviceToHost, s), "d2h");
+  ckh(hipStreamSynchronize(s), "local QR");
+  double* out = new double[n * n]();
+  // hrt is R^T row-major (lower); the reference's shrink (client.cpp:434-444) leaves R row-major
+  // when transpose_x, else R^T row-major
+  for (size_t i = 0; i < n; i++)
+    for (size_t j = 0; j <= i; j++) {
+      if (transpose_x) out[j * n + i] = hrt[i * n + j];
+      else out[i * n + j] = hrt[i * n + j];
+    }
+  delete[] *raw_x;
+  *raw_x = out;
 }

@@ -1083,6 +1083,45 @@ int exc_ralha_loop_fits(int m, int c, int cols_per_wave) {
   return fits ? 1 : 0;
 }
 
+int exc_ralha_loop_shape(int m, int c, int cols_per_wave, int* nwg, size_t* lds_bytes) {
+  if (nwg) *nwg = 0;
+  if (lds_bytes) *lds_bytes = 0;
+  if (m < 3 || c < 1) return EXC_OK;
+  const LoopShape sh = loop_shape(m, c, cols_per_wave);
+  if (!sh.km || sh.nwg > kLoopMaxWg) return EXC_OK;
+  if (nwg) *nwg = sh.nwg;
+  if (lds_bytes) *lds_bytes = sh.lds;
+  return EXC_OK;
+}
+
+int exc_ralha_loop_slots(int m, int c, int cols_per_wave) {
+  if (m < 3 || c < 1) return 0;
+  const LoopShape sh = loop_shape(m, c, cols_per_wave);
+  const void* fn = loop_kernel_for(sh);
+  if (!fn || sh.nwg < 1 || sh.nwg > kLoopMaxWg) return 0;
+  int dev = 0, cus = 0, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+  if (sh.lds > 64 * 1024 &&
+      hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh.lds) != hipSuccess)
+    return 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, sh.lds) != hipSuccess) return 0;
+  return per_cu * cus;
+}
+
+int exc_ralha_loops_colocate(int P, const int* nwg, const int* slots, int same_device) {
+  if (P < 1 || !nwg || !slots) return 0;
+  long total = 0, fewest = -1;
+  for (int q = 0; q < P; q++) {
+    if (nwg[q] < 1 || slots[q] < nwg[q]) return 0;  // a loop that does not fit even alone
+    total += nwg[q];
+    fewest = fewest < 0 ? slots[q] : std::min<long>(fewest, slots[q]);
+  }
+  // On one device every peer's loop waits on every other's message, so all grids must be
+  // resident together; on separate devices each grid only needs its own device.
+  return (!same_device || total <= fewest) ? 1 : 0;
+}
+
 }  // extern "C"
 
 namespace {

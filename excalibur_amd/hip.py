@@ -80,6 +80,9 @@ SIGNATURES = {
     "exc_ralha_loop_abort": (_I, [_vp]),
     "exc_ralha_loop_finish": (_I, [_vp, _vp]),
     "exc_ralha_loop_grid": (_I, [_I, _I, _I]),
+    "exc_ralha_loop_shape": (_I, [_I, _I, _I, ctypes.POINTER(_I), ctypes.POINTER(_S)]),
+    "exc_ralha_loop_slots": (_I, [_I, _I, _I]),
+    "exc_ralha_loops_colocate": (_I, [_I, ctypes.POINTER(_I), ctypes.POINTER(_I), _I]),
     "exc_ralha_box_bytes": (_S, [_I]),
     "exc_ralha_box_clear": (_I, [_dp, _vp]),
     "exc_ralha_loop_start_boxes": (_I, [_I, _I, _dp, _dp, _dp, _dp, _dp, _dp, _vp, _vp, _S, _I, _D, _I, _I,

@@ -234,6 +234,17 @@ int exc_ralha_loop_abort(void* h_link);
 int exc_ralha_loop_finish(void* h_link, exc_stream_t stream);
 /* Number of workgroups of this peer's loop grid (0 when exc_ralha_loop_fits is 0). */
 int exc_ralha_loop_grid(int m, int c, int cols_per_wave);
+/* Host-only shape of the loop grid (no HIP call): workgroups and LDS bytes per workgroup, both 0
+ * when the loop does not apply to (m, c, cols_per_wave). */
+int exc_ralha_loop_shape(int m, int c, int cols_per_wave, int* nwg, size_t* lds_bytes);
+/* Workgroups of this loop shape that the current device holds at once (occupancy x CUs), 0 when
+ * the loop does not apply. */
+int exc_ralha_loop_slots(int m, int c, int cols_per_wave);
+/* Whether P peers may all run the one-launch loop: each grid nwg[q] fits its device's slots[q],
+ * and when the peers share one device (same_device = 1) the sum of all grids fits the fewest
+ * slots, since every peer's loop waits on every other's messages and all must be resident
+ * together. Host-only, no HIP call. Returns 1 or 0. */
+int exc_ralha_loops_colocate(int P, const int* nwg, const int* slots, int same_device);
 /* The same loop for peers that share one device: the messages go through per-peer boxes in
  * device memory (exc_ralha_box_bytes(m) each; peer q's box opened in the other peers' processes
  * with exc_ipc_open), and the host does nothing per step. Every peer q's box counter must have

@@ -153,16 +153,15 @@ void dlarfg(long n, double* alpha, double* x, long incx, double* tau) {
 
 // Apply H = I - tau v v^T from the left to C (rows x cols, column-major ldc); v has unit stride
 // and v[0] is taken as stored (callers set it to 1).
+// Columns are independent, so they are split over threads; each element keeps the sequential
+// order above, and the results do not depend on the thread count.
 void dlarf_left(long rows, long cols, const double* v, double tau, double* C, long ldc) {
   if (tau == 0.0) return;
-  vec w(cols, 0.0);
-  for (long j = 0; j < cols; j++) {  // w = C^T v (reference dgemv 'T')
-    double t = 0.0;
+#pragma omp parallel for schedule(static) if (rows * cols > (1L << 20))
+  for (long j = 0; j < cols; j++) {
+    double t = 0.0;  // w_j = (C^T v)_j (reference dgemv 'T')
     for (long i = 0; i < rows; i++) t = t + C[i + j * ldc] * v[i];
-    w[j] = t;
-  }
-  for (long j = 0; j < cols; j++) {  // C = C - tau v w^T (reference dger)
-    double t = -tau * w[j];
+    t = -tau * t;  // C_j = C_j - tau v w_j (reference dger)
     for (long i = 0; i < rows; i++) C[i + j * ldc] = C[i + j * ldc] + v[i] * t;
   }
 }
@@ -211,19 +210,30 @@ void dorm2r_left(bool trans, long m, long n, long k, const double* A, long lda, 
   else for (long i = k - 1; i >= 0; i--) apply(i);
 }
 
-// Row-major GEMM C (M x N) = op(A) op(B), sequential dot products.
+// Row-major GEMM C (M x N) = op(A) op(B): every element is the sequential sum over l = 0..K-1
+// starting from 0. Rows of C are split over threads; for a non-transposed B the l loop runs
+// outside the j loop so that B is read along its rows, which adds the same terms in the same
+// order to each element.
 void gemm_rm(bool ta, bool tb, long M, long N, long K, const double* A, long lda, const double* B,
              long ldb, double* C, long ldc) {
-  for (long i = 0; i < M; i++)
-    for (long j = 0; j < N; j++) {
-      double s = 0.0;
+#pragma omp parallel for schedule(static) if ((double)M * N * K > 1e6)
+  for (long i = 0; i < M; i++) {
+    double* c = C + i * ldc;
+    if (!tb) {
+      for (long j = 0; j < N; j++) c[j] = 0.0;
       for (long l = 0; l < K; l++) {
-        double a = ta ? A[l * lda + i] : A[i * lda + l];
-        double b = tb ? B[j * ldb + l] : B[l * ldb + j];
-        s = s + a * b;
+        const double a = ta ? A[l * lda + i] : A[i * lda + l];
+        const double* b = B + l * ldb;
+        for (long j = 0; j < N; j++) c[j] = c[j] + a * b[j];
       }
-      C[i * ldc + j] = s;
+    } else {
+      for (long j = 0; j < N; j++) {
+        double s = 0.0;
+        for (long l = 0; l < K; l++) s = s + (ta ? A[l * lda + i] : A[i * lda + l]) * B[j * ldb + l];
+        c[j] = s;
+      }
     }
+  }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -713,21 +723,30 @@ void sign_normalize(long m, long cnt, double* bu, double* bv) {
 
 // dormqr('L','N') on a row-major C (m x n) with reflectors stored as columns of row-major
 // house (m x m): C <- H_0 H_1 ... H_{m-1} C.
+// Per reflector, w_q = sum over l of C[j+l][q] v_l (sequential in l from 0), then
+// C[j+l][q] += v_l (-tau_j w_q). The loops run l outside q so that C is read along its rows, and
+// column blocks go to threads; each element sees the same operations in the same order.
 void ormqr_house_ln(long m, long n, const double* house, const double* tau, double* C) {
   vec v(m), w(n);
+  const long QB = 64;
   for (long j = m - 1; j >= 0; j--) {
     if (tau[j] == 0.0) continue;
     long len = m - j;
     v[0] = 1.0;
     for (long l = 1; l < len; l++) v[l] = house[(j + l) * m + j];
-    for (long q = 0; q < n; q++) {
-      double t = 0.0;
-      for (long l = 0; l < len; l++) t = t + C[(j + l) * n + q] * v[l];
-      w[q] = t;
-    }
-    for (long q = 0; q < n; q++) {
-      double t = -tau[j] * w[q];
-      for (long l = 0; l < len; l++) C[(j + l) * n + q] = C[(j + l) * n + q] + v[l] * t;
+#pragma omp parallel for schedule(static) if (len * n > (1L << 18))
+    for (long q0 = 0; q0 < n; q0 += QB) {
+      const long q1 = std::min(n, q0 + QB);
+      for (long q = q0; q < q1; q++) w[q] = 0.0;
+      for (long l = 0; l < len; l++) {
+        const double* c = C + (j + l) * n;
+        for (long q = q0; q < q1; q++) w[q] = w[q] + c[q] * v[l];
+      }
+      for (long q = q0; q < q1; q++) w[q] = -tau[j] * w[q];
+      for (long l = 0; l < len; l++) {
+        double* c = C + (j + l) * n;
+        for (long q = q0; q < q1; q++) c[q] = c[q] + v[l] * w[q];
+      }
     }
   }
 }
@@ -1006,6 +1025,7 @@ void* orc_run(int P, long m, const long* local_n, const double* const* x, const 
     vec tmp((size_t)cnt * rl[p], 0.0);
     if (R > m) {
       // tmp = bu^T (cnt x m) . Qg[rnp:rnp+rl, :]^T  (client.cpp:1179)
+#pragma omp parallel for schedule(static)
       for (long a = 0; a < cnt; a++)
         for (long l = 0; l < rl[p]; l++) {
           double s = 0.0;

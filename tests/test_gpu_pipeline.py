@@ -304,17 +304,17 @@ def test_two_peer_run_step_kernels(hip, oracle):
 
 
 def test_run_device_exchange_equals_network_exchange(hip, oracle):
-    """Peers on one device exchange the bidiagonalization messages in device memory (IPC boxes);
-    with --ralha-device-exchange 0 they all-reduce them over TCP. Both add the same values in the
-    same order, so the results are identical."""
+    """Peers on one device exchange the bidiagonalization messages in device memory (IPC boxes)
+    with --device-exchange 1; by default they all-reduce them over TCP. Both add the same values
+    in the same order, so the results are identical."""
     import os
     from excalibur_amd.launch import run_peers
     banner("3-peer run, device exchange vs network exchange")
     rng = np.random.default_rng(12)
     m = 70
     xs = [rng.standard_normal((m, w)) for w in (200, 150, 250)]
-    a = run_peers(xs, seed=5)
-    b = run_peers(xs, seed=5, extra=["--device-exchange", "0"])
+    a = run_peers(xs, seed=5, extra=["--device-exchange", "1"])
+    b = run_peers(xs, seed=5)
     logs = open(os.path.join(a["workdir"], "logs", "Client0.log")).read()
     assert "exchanged in device memory" in logs
     assert "exchanged in device memory" not in open(os.path.join(b["workdir"], "logs", "Client0.log")).read()

@@ -1,0 +1,154 @@
+"""Whole runs of the GPU DecentralizedClient beyond the default mode: modes 0, 2 and 3, top-k
+clipping, the reference's 14-argument constructor with host results, and the full-size
+2 x 50,000 x 1,000 workload with its accuracy bars.
+
+Mode 0 is the reference's LAPACKE_dbdsdc branch (client.cpp:975-980); this build routes it through
+the same bisection + twisted factorization as mode 1, so its results equal mode 1's bit for bit.
+Mode 2 computes only U and mode 3 only V (client.cpp:1002, 1119); the shared stages are the same
+kernels on the same data, so what they compute equals mode 1's bit for bit.
+"""
+import os
+import subprocess
+import tempfile
+import time
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def banner(msg):
+    print(f"[gpu-run-modes] {msg}", flush=True)
+
+
+def config1():
+    rng = np.random.default_rng(0)
+    A = rng.standard_normal((1000, 100))
+    return [A[:500].T.copy(), A[500:].T.copy()]
+
+
+@pytest.fixture(scope="module")
+def mode1(hip):
+    from excalibur_amd.launch import run_peers
+    banner("mode 1 baseline, 2 x 500 x 100")
+    return run_peers(config1(), seed=0, mode=1)
+
+
+@pytest.mark.parametrize("mode", [0, 2, 3])
+def test_modes_equal_mode1(hip, oracle, mode1, mode):
+    from excalibur_amd.launch import run_peers
+    banner(f"mode {mode}, 2 x 500 x 100")
+    xs = config1()
+    res = run_peers(xs, seed=0, mode=mode)
+    run = oracle.Run(xs, [0, 0])
+    for pa, pb in zip(res["peers"], mode1["peers"]):
+        assert np.array_equal(pa["sigma"], pb["sigma"])
+        if mode in (0, 2):
+            assert np.array_equal(pa["FinalU"], pb["FinalU"])
+        else:
+            assert "FinalU" not in pa
+        if mode in (0, 3):
+            assert np.array_equal(pa["FinalVT"], pb["FinalVT"])
+        else:
+            assert "FinalVT" not in pa
+    # and against the oracle's run
+    so = run.get("sigma", 0)
+    assert np.max(np.abs(res["peers"][0]["sigma"] - so)) <= 1e-12 * so[0]
+
+
+def test_top_k_64(hip, oracle):
+    """top_k clipping (client.cpp:970-971): the 64 largest triplets, against the oracle's run."""
+    from excalibur_amd.launch import run_peers
+    banner("top_k = 64, 2 x 500 x 100")
+    xs = config1()
+    k = 64
+    res = run_peers(xs, seed=0, topk=k)
+    run = oracle.Run(xs, [0, 0], top_k=k)
+    X = np.hstack(xs)
+    p0 = res["peers"][0]
+    assert p0["result"]["sigma_count"] == k
+    sg, U = p0["sigma"], p0["FinalU"]
+    so, Uo = run.get("sigma", 0), run.get("FinalU", 0).reshape(100, k)
+    assert np.max(np.abs(sg - so)) <= 1e-12 * so[0]
+    s_np = np.linalg.svd(X, compute_uv=False)
+    assert np.max(np.abs(sg - s_np[:k])) <= 1e-12 * s_np[0]
+    VT = np.hstack([p["FinalVT"] for p in res["peers"]])
+    VTo = np.hstack([run.get("FinalVT", c).reshape(k, x.shape[1]) for c, x in enumerate(xs)])
+    assert np.abs(U.T @ U - np.eye(k)).max() <= 1e-12
+    assert np.abs(VT @ VT.T - np.eye(k)).max() <= 1e-12
+    # U^T X = diag(sigma) V^T for the leading triplets
+    assert np.linalg.norm(U.T @ X - sg[:, None] * VT) <= 1e-12 * np.linalg.norm(X)
+    assert np.abs(U - Uo).max() <= 1e-9 and np.abs(VT - VTo).max() <= 1e-9
+
+
+def test_reference_constructor_two_processes(hip, mode1):
+    """Reference code constructs the client with the 14 arguments of client.hpp:77 and reads
+    sigma, FinalU and FinalVT from its public host members afterwards."""
+    from excalibur_amd.launch import free_port, write_peer_files
+    banner("14-argument constructor, two processes")
+    exe = os.path.join(ROOT, "tests", "cpp", "_build", "ref_ctor_main")
+    assert os.path.exists(exe), "build the library first (__graft_entry__.build())"
+    xs = config1()
+    work = tempfile.mkdtemp(prefix="exc_ctor_")
+    dpath, logdir, outdir = (os.path.join(work, d) for d in ("data", "logs", "out"))
+    for d in (logdir, outdir):
+        os.makedirs(d)
+    write_peer_files(xs, dpath)
+    port = free_port(3)
+    procs = [subprocess.Popen([exe, str(c), "2", "1", "-1", "100", "500", str(port), dpath, logdir, "0", outdir],
+                              stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True) for c in range(2)]
+    outs = [p.communicate(timeout=100)[0] for p in procs]
+    print(outs)
+    assert all(p.returncode == 0 for p in procs), outs
+    for c, base in enumerate(mode1["peers"]):
+        pre = os.path.join(outdir, f"ctor{c}_")
+        assert np.array_equal(np.fromfile(pre + "sigma.bin"), base["sigma"])
+        assert np.array_equal(np.fromfile(pre + "FinalU.bin").reshape(100, 100), base["FinalU"])
+        assert np.array_equal(np.fromfile(pre + "FinalVT.bin").reshape(100, 500), base["FinalVT"])
+        assert np.array_equal(np.fromfile(pre + "alpha.bin"), base["alpha"])
+
+
+def c2_blocks(rows=50_000, n=1_000, seed=0):
+    rng = np.random.default_rng(seed)
+    sigma = 10.0 ** (-6.0 * np.arange(n) / (n - 1))
+    Q1 = np.linalg.qr(rng.standard_normal((2 * rows, n)))[0]
+    Q2 = np.linalg.qr(rng.standard_normal((n, n)))[0]
+    A = (Q1 * sigma) @ Q2.T
+    return [np.ascontiguousarray(A[i * rows:(i + 1) * rows].T) for i in range(2)], sigma
+
+
+def test_config2_full_size_properties(hip):
+    """2 peers x 50,000 x 1,000 rows of A = Q1 diag(sigma) Q2^T, sigma_i = 10^(-6i/999), full SVD
+    over host TCP. Bars: normwise sigma error and ||A - U S V^T||_F / ||A||_F <= 1e-12;
+    ||U^T U - I|| and ||V^T V - I|| <= 1e-12 in the max-entry and 2-norms (the Frobenius norm,
+    which sums 10^6 entries, is reported and held to 3e-11)."""
+    from excalibur_amd.launch import run_peers
+    banner("config 2 full size, 2 x 50000 x 1000")
+    t0 = time.time()
+    xs, sigma = c2_blocks()
+    banner(f"generated in {time.time() - t0:.1f} s")
+    res = run_peers(xs, seed=0, timeout=300)
+    banner(f"peers done in {time.time() - t0:.1f} s")
+    p0 = res["peers"][0]
+    k = p0["result"]["sigma_count"]
+    assert k == 1000
+    sg, U = p0["sigma"], p0["FinalU"]
+    VT = np.hstack([p["FinalVT"] for p in res["peers"]])
+    G = U.T @ U - np.eye(k)
+    H = VT @ VT.T - np.eye(k)
+    X = np.hstack(xs)
+    st = {"sigma_rel_err": float(np.max(np.abs(sg - sigma)) / sigma[0]),
+          "orth_u_max": float(np.abs(G).max()), "orth_u_fro": float(np.linalg.norm(G)),
+          "orth_u_2": float(np.linalg.norm(G, 2)),
+          "orth_v_max": float(np.abs(H).max()), "orth_v_fro": float(np.linalg.norm(H)),
+          "orth_v_2": float(np.linalg.norm(H, 2)),
+          "residual_rel": float(np.linalg.norm(X - (U * sg) @ VT) / np.linalg.norm(X)),
+          "total_s": [p["result"]["total"] for p in res["peers"]]}
+    print(st)
+    assert st["sigma_rel_err"] <= 1e-12 and st["residual_rel"] <= 1e-12
+    assert st["orth_u_max"] <= 1e-12 and st["orth_v_max"] <= 1e-12
+    assert st["orth_u_2"] <= 1e-12 and st["orth_v_2"] <= 1e-12
+    assert st["orth_u_fro"] <= 3e-11 and st["orth_v_fro"] <= 3e-11

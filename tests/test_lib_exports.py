@@ -101,3 +101,38 @@ int main() {
     r = subprocess.run(["g++", "-std=c++17", "-I", inc, str(src), "-L", libdir, "-lexcalibur_amd",
                         f"-Wl,-rpath,{libdir}", "-o", str(out)], capture_output=True, text=True)
     assert r.returncode == 0, r.stderr[-2000:]
+
+
+def test_loop_colocation_decision():
+    """All peers' one-launch bidiagonalization loops must be resident together on one device,
+    since each waits on the others' messages (host-only decision, no GPU call)."""
+    import ctypes
+    from excalibur_amd import hip
+    lib = hip.lib
+
+    def shape(m, c):
+        nwg = ctypes.c_int(); lds = ctypes.c_size_t()
+        assert lib.exc_ralha_loop_shape(m, c, 0, ctypes.byref(nwg), ctypes.byref(lds)) == 0
+        return nwg.value, lds.value
+
+    def decide(P, nwg, slots, same):
+        a = (ctypes.c_int * P)(*nwg); b = (ctypes.c_int * P)(*slots)
+        return lib.exc_ralha_loops_colocate(P, a, b, 1 if same else 0)
+
+    # P = 3, m = 2048: c = 683, 683, 682 -> 86 workgroups each of 98 KB LDS, so one per CU:
+    # 256 slots on 256 CUs hold each grid alone but not all three (258)
+    m = 2048
+    cs = [683, 683, 682]
+    shapes = [shape(m, c) for c in cs]
+    assert all(nw == 86 for nw, _ in shapes)
+    assert all(160 * 1024 // 2 < lds <= 160 * 1024 for _, lds in shapes)
+    assert decide(3, [86, 86, 86], [256, 256, 256], True) == 0
+    assert decide(3, [86, 86, 86], [256, 256, 256], False) == 1
+    # config 2: two peers of c = 500 at m = 1000 -> 63 workgroups each, two per CU fit
+    nw, lds = shape(1000, 500)
+    assert nw == 63 and lds * 2 <= 160 * 1024
+    assert decide(2, [63, 63], [512, 512], True) == 1
+    # a peer without the loop (0 workgroups) makes every peer take the per-step path
+    assert decide(2, [63, 0], [512, 0], True) == 0
+    # the fewest slots decide
+    assert decide(2, [100, 100], [512, 150], True) == 0
