@@ -1,16 +1,21 @@
 #!/usr/bin/env python3
-"""Benchmark of the per-peer bbtsvd pipeline on one MI355X.
+"""Benchmark of the per-peer bbtsvd pipeline on one MI355X: the mixed-precision form against the
+plain FP64 form.
 
 Workload (default): 2 peers, each holding a 50,000 x 1,000 float64 row block of
 A = Q1 diag(sigma) Q2^T (100,000 x 1,000), Q1, Q2 random orthogonal (QR of seeded Gaussians),
-sigma_i = 10^(-6 i / 999); full SVD (all 1,000 triplets) with Givens masking, the reference's
-mode 1 (bisection + twisted factorization), opt 3. The peers are two processes of
-excalibur_amd/excalibur_peer sharing the GPU and talking over loopback TCP, exactly the
-reference's DecentralizedClient::run() workflow; a step is one run() of both peers with the input
-already resident in HBM, timed from "All Clients Connected" to "Finish" (max over the peers).
+sigma_i = 10^(-6 i / 999); full SVD (all 1,000 triplets). The peers are two processes of
+excalibur_amd/excalibur_peer sharing the GPU and talking over loopback TCP, the reference's
+DecentralizedClient::run() workflow; a step is one run() of both peers with the input already
+resident in HBM, timed from "All Clients Connected" to "Finish" (max over the peers).
 
-Prints one JSON line (metric, value = seconds per SVD, roofline of the dominant kernel, CPU
-baseline). --gpus N > 1 exits: this build measures one GPU.
+`value` is the seconds per SVD of the mixed form (FP32-MFMA CholeskyQR passes, one FP64 pass,
+FP64 bisection + twisted factorization of the shared factor and the FP64 Ogita-Aishima
+refinement); the plain FP64 form (the reference's algorithm: local QR, Givens masks, global QR,
+per-column all-reduce bidiagonalization, bisection + twisted factorization) is measured beside it
+on the same input, and both again on the short-wide 2 x 2,000 x 100,000 workload with a
+clustered spectrum. Accuracy against the generator's exact singular values is reported for
+every run. Prints one JSON line. --gpus N > 1 exits: this build measures one GPU.
 """
 import argparse
 import json
@@ -31,10 +36,28 @@ PEAK_HBM_GBS = 8000.0
 CONFIGS = {
     # name: (peers, rows per peer, columns, spectrum)
     "c2": (2, 50_000, 1_000, "geometric6"),
+    "c3": (2, 100_000, 2_000, "clustered8"),
     "c1": (2, 500, 100, "gaussian"),
     "small": (2, 6_000, 200, "geometric6"),
+    "small3": (2, 10_000, 400, "clustered8"),
     "cpu_sample": (2, 10_000, 1_000, "geometric6"),
 }
+
+WORKLOADS = {
+    "geometric6": "float64 row blocks of A = Q1 diag(sigma) Q2^T, sigma_i = 10^(-6i/(n-1))",
+    "clustered8": "float64 column blocks of a short-wide A = Q1 diag(sigma) Q2^T, sigma = 20 values "
+                  "1 + 1e-8 linspace(-1, 1, 20), then a geometric decay to 1e-8",
+    "gaussian": "N(0,1) row blocks",
+}
+
+
+def spectrum(n, spec):
+    if spec == "geometric6":
+        return 10.0 ** (-6.0 * np.arange(n) / (n - 1))
+    if spec == "clustered8":
+        cl = np.sort(1.0 + 1e-8 * np.linspace(-1, 1, 20))[::-1]
+        return np.concatenate([cl, np.geomspace(cl[-1] * 0.99, 1e-8, n - 20)])
+    raise ValueError(spec)
 
 
 def log(msg):
@@ -49,7 +72,7 @@ def make_blocks(cfg, seed=0):
         A = rng.standard_normal((P * rows, n))
         sigma = None
     else:
-        sigma = 10.0 ** (-6.0 * np.arange(n) / (n - 1))
+        sigma = spectrum(n, spec)
         Q1 = np.linalg.qr(rng.standard_normal((P * rows, n)))[0]
         Q2 = np.linalg.qr(rng.standard_normal((n, n)))[0]
         A = (Q1 * sigma) @ Q2.T
@@ -202,18 +225,142 @@ def run_variant(xs, form, device_exchange, steps, warmup, dump, workdir):
             "agg": agg, "steps": len(timed), "results_d2h_s": d2h, "res": res}
 
 
+def rate(rec):
+    return rec["flops"] / (rec["ms"] * 1e-3) / 1e12 if rec["ms"] > 0 else 0.0
+
+
+def summarize(run):
+    """Per-step kernel records and the FP64 GEMM roofline entry of one measured variant."""
+    agg, nsteps = run["agg"], run["steps"]
+    g = agg.get("gemm_f64", {"launches": 0, "ms": 0.0, "flops": 0.0, "bytes": 0.0})
+    fv = agg.get("gemm_f64_final_v", {"launches": 0, "ms": 0.0, "flops": 0.0, "bytes": 0.0})
+    achieved = rate(g)
+    roof = {"bound": "mfma", "kernel": "k_gemm<double> (all FP64 MFMA GEMM launches of both peers)",
+            "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": None,
+            "launches_per_step": g["launches"] / max(1, nsteps),
+            "avg_launch_us": round(1000.0 * g["ms"] / max(1, g["launches"]), 2),
+            "flops_per_step": g["flops"] / max(1, nsteps),
+            "final_v": {"launches_per_step": fv["launches"] / max(1, nsteps),
+                        "avg_launch_us": round(1000.0 * fv["ms"] / max(1, fv["launches"]), 2),
+                        "achieved": round(rate(fv), 3), "frac": round(rate(fv) / PEAK_FP64_TFLOPS, 4)}}
+    kernels = {k: {"launches": v["launches"] / nsteps, "ms": v["ms"] / nsteps, "achieved": rate(v),
+                   "gbs": (v["bytes"] / (v["ms"] * 1e-3) / 1e9) if v["ms"] > 0 else 0.0}
+               for k, v in agg.items()}
+    return roof, kernels
+
+
+def measure(xs, sigma_true, A, form, device_exchange, steps, warmup, check, workdir=None):
+    t0 = time.time()
+    run = run_variant(xs, form, device_exchange, steps, warmup, check, workdir)
+    log(f"{form} ({'TCP' if device_exchange == 0 else 'device exchange'}): {run['ms_per_step']:.2f} ms per SVD "
+        f"({time.time() - t0:.1f} s)")
+    roof, kernels = summarize(run)
+    out = {"ms_per_step": round(run["ms_per_step"], 3), "step_times_s": [round(v, 6) for v in run["step_times"]],
+           "stages_s": run["stages"], "results_d2h_s": round(run["results_d2h_s"], 6), "roofline": roof,
+           "kernels": kernels}
+    if check:
+        t0 = time.time()
+        out["accuracy"] = accuracy(run["res"]["peers"], xs, sigma_true, A)
+        log(f"accuracy of {form} in {time.time() - t0:.1f} s: {out['accuracy']}")
+    return out
+
+
+def bench_batched(nb=32, m=512, lo=2000, hi=60000, seed=4):
+    """32 peers, 512 columns, rows uniform in [2000, 60000], entries N(0,1) times a lognormal(0,1)
+    factor per peer: the local QR and masks of all peers at once (batched, FP32 bulk + FP64
+    refinement), merged by the global QR, against the same stage one peer after another."""
+    from excalibur_amd import batched as B
+    t0 = time.time()
+    rng = np.random.default_rng(seed)
+    cols = rng.integers(lo, hi + 1, nb)
+    scale = np.exp(rng.standard_normal(nb))
+    xs = [(rng.standard_normal((m, int(c))) * s_) for c, s_ in zip(cols, scale)]
+    log(f"batched peers: generated {nb} blocks ({int(cols.sum())} rows) in {time.time() - t0:.1f} s")
+    seeds = list(range(100, 100 + nb))
+    B.local_stage(xs[:2], 99, seeds[:2])  # warm the kernels
+    runs = []
+    for _ in range(3):
+        t = {}
+        st = B.local_stage(xs, 99, seeds, timings=t)
+        rt, cat = B.merge(st["masked"], m, timings=t)
+        runs.append(t)
+    base = min(B.per_peer_baseline(xs, 99, seeds) for _ in range(2))
+    best = min(runs, key=lambda t: t["local_qr_s"] + t["mask_s"])
+    # accuracy: R R^T against the masked blocks' Gram matrix, and each block's R against X X^T
+    S = cat.to_host()
+    R = rt.to_host()
+    G = S @ S.T
+    err_merge = float(np.max(np.abs(R @ R.T - G)) / np.abs(G).max())
+    err_blocks = 0.0
+    for b in np.flatnonzero(st["reduced"])[:8]:
+        r = st["rt"][b].to_host()
+        Gb = xs[b] @ xs[b].T
+        err_blocks = max(err_blocks, float(np.max(np.abs(r @ r.T - Gb)) / np.abs(Gb).max()))
+    flops = float(sum(2.0 * m * m * c for c in cols))  # Gram (half) + the FP64 solve, per pass
+    out = {"config": {"workload": f"{nb} peers x {m} columns, rows uniform in [{lo}, {hi}] (sum {int(cols.sum())}), "
+                                  "N(0,1) x lognormal(0,1) per peer; local QR + Givens masks of all peers at once, "
+                                  "merged by the global QR", "peers": nb, "columns": m},
+           "batched_local_s": round(best["local_qr_s"] + best["mask_s"], 6),
+           "batched_local_qr_s": round(best["local_qr_s"], 6), "batched_mask_s": round(best["mask_s"], 6),
+           "merge_s": round(best["merge_s"], 6), "per_peer_loop_s": round(base, 6),
+           "speedup_vs_per_peer": round(base / (best["local_qr_s"] + best["mask_s"]), 3),
+           "rel_err_merged_gram": err_merge, "rel_err_block_gram_max": err_blocks,
+           "fp64_passes": [int(v) for v in st["stats"][:, 2]], "autotuner": json.loads(B.tuning())}
+    log(f"batched peers: {out['batched_local_s'] * 1e3:.1f} ms batched vs {base * 1e3:.1f} ms per peer")
+    return out
+
+
+def bench_learned(n_train=10_000, steps=2000, seed=10):
+    """Learned bisection brackets: 10,000 generated bidiagonals with n in [512, 4096] (four spectrum
+    families), the network trained by the library's FP32 kernels, then every singular value of
+    n = 16,384 bidiagonals of the four families bisected from its BF16 brackets and plainly."""
+    from excalibur_amd import learned as Lm
+    t0 = time.time()
+    train = Lm.make_set(n_train, (512, 4096), seed=seed)
+    X, y = Lm.training_data(train, per=32, seed=seed + 1)
+    t_data = time.time() - t0
+    net = Lm.BracketNet(seed=0)
+    hist = net.train(X, y, steps=steps, batch=4096, lr=2e-3, seed=1)
+    held = Lm.make_set(500, (512, 4096), seed=seed + 5)
+    k_lo, k_hi = net.calibrate(*Lm.training_data(held, per=32, seed=seed + 6))
+    test = Lm.make_set(4, (16384, 16384), seed=seed + 10)
+    fams = {}
+    for b in range(test.nb):
+        r = Lm.evaluate(net, test, b)
+        fams[test.families[b]] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()
+                                  if k not in ("sigma", "leaves")}
+    saved = float(np.mean([f["saved_frac"] for f in fams.values()]))
+    saved_shifts = float(np.mean([f["saved_frac_shifts_only"] for f in fams.values()]))
+    out = {"config": {"workload": "bidiagonals with n = 16,384 from Golub-Kahan of matrices with geometric, linear, "
+                                  "clustered and Marchenko-Pastur spectra; MLP 32-256-256-2, FP32 training on "
+                                  f"{n_train} generated bidiagonals (n in [512, 4096]), BF16 inference"},
+           "train_samples": int(X.shape[0]), "data_s": round(t_data, 3), "train_s": round(hist["seconds"], 3),
+           "loss_first_last": [hist["loss"][0][1], hist["loss"][-1][1]], "families": fams,
+           "bracket_ends_laplace_scales": [round(k_lo, 3), round(k_hi, 3)],
+           "sturm_counts_saved_frac": round(saved, 4),
+           "sturm_counts_saved_frac_shifts_only": round(saved_shifts, 4),
+           "all_identical": all(f["identical"] for f in fams.values())}
+    log(f"learned brackets: {100 * saved:.1f}% of Sturm counts saved, identical={out['all_identical']}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
-    ap.add_argument("--form", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--form", default="mixed", choices=["fp64", "mixed"],
+                    help="the form whose time is `value`; the other form is measured beside it")
     ap.add_argument("--device-exchange", type=int, default=0, choices=[0, 1],
                     help="1: peers on one device exchange data in device memory instead of host TCP")
     ap.add_argument("--no-variants", action="store_true",
-                    help="skip the extra runs (device-memory exchange, the other form, the CPU sample on the GPU)")
+                    help="skip the extra runs (the other form, device-memory exchange, the second workload)")
+    ap.add_argument("--second-config", default="c3", help="workload measured in both forms beside the first")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batched", action="store_true", help="skip the 32 batched peers workload")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-bracket workload")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--workdir", default=None)
     ap.add_argument("--prepare", default=None,
@@ -249,59 +396,53 @@ def main():
         log(f"wrote {len(xs)} peer files ({xs[0].shape[0]} x {xs[0].shape[1]}) to {args.prepare}")
         return
 
+    check = not args.no_accuracy
     t0 = time.time()
     xs, sigma_true, A = make_blocks(args.config)
     log(f"generated {args.config} in {time.time() - t0:.1f} s: {len(xs)} peers x {xs[0].shape[1]} x {xs[0].shape[0]}")
-    workdir = args.workdir or tempfile.mkdtemp(prefix="exc_bench_")
-    t0 = time.time()
-    main_run = run_variant(xs, args.form, args.device_exchange, args.steps, args.warmup, not args.no_accuracy,
-                           workdir)
-    log(f"peers finished in {time.time() - t0:.1f} s: {main_run['ms_per_step']:.2f} ms per SVD")
-    step_times, ms_per_step, stages, agg = (main_run["step_times"], main_run["ms_per_step"], main_run["stages"],
-                                            main_run["agg"])
-    nsteps = main_run["steps"]
-
-    def rate(rec):
-        return rec["flops"] / (rec["ms"] * 1e-3) / 1e12 if rec["ms"] > 0 else 0.0
-    gname = "gemm_f64"
-    g = agg.get(gname, {"launches": 0, "ms": 0.0, "flops": 0.0, "bytes": 0.0})
-    fv = agg.get("gemm_f64_final_v", {"launches": 0, "ms": 0.0, "flops": 0.0, "bytes": 0.0})
-    achieved = rate(g)
-    roofline = {"bound": "mfma", "kernel": "k_gemm<double> (all FP64 MFMA GEMM launches of both peers)",
-                "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": None,
-                "traffic_final_v": pmc_traffic(),
-                "launches_per_step": g["launches"] / max(1, nsteps),
-                "avg_launch_us": round(1000.0 * g["ms"] / max(1, g["launches"]), 2),
-                "flops_per_step": g["flops"] / max(1, nsteps),
-                "final_v": {"launches_per_step": fv["launches"] / max(1, nsteps),
-                            "avg_launch_us": round(1000.0 * fv["ms"] / max(1, fv["launches"]), 2),
-                            "achieved": round(rate(fv), 3), "frac": round(rate(fv) / PEAK_FP64_TFLOPS, 4)},
-                "note": "in-pipeline: both peer processes share the GPU, so a launch's duration includes "
-                        "the other peer's concurrent work; 'isolated' is the same kernel alone"}
-    kernels = {k: {"launches": v["launches"] / nsteps, "ms": v["ms"] / nsteps,
-                   "achieved": (v["flops"] / (v["ms"] * 1e-3) / 1e12) if v["ms"] > 0 else 0.0,
-                   "gbs": (v["bytes"] / (v["ms"] * 1e-3) / 1e9) if v["ms"] > 0 else 0.0}
-               for k, v in agg.items()}
-    acc = {}
-    if not args.no_accuracy:
-        t0 = time.time()
-        acc = accuracy(main_run["res"]["peers"], xs, sigma_true, A)
-        log(f"accuracy checked in {time.time() - t0:.1f} s: {acc}")
-    P, rows, n, _ = CONFIGS[args.config]
-    try:
-        roofline["isolated"] = isolated_gemms(n, rows)
-    except Exception as e:  # the measurement is extra information; the run itself already finished
-        log(f"isolated GEMM timing failed: {e}")
+    main_run = measure(xs, sigma_true, A, args.form, args.device_exchange, args.steps, args.warmup, check,
+                       args.workdir)
+    other = "fp64" if args.form == "mixed" else "mixed"
+    forms = {args.form: main_run}
     variants = {}
+    P, rows, n, spec = CONFIGS[args.config]
     if not args.no_variants:
-        # the same workload with the exchanges in device memory (IPC), a deviation from host TCP
+        forms[other] = measure(xs, sigma_true, A, other, args.device_exchange, args.steps, args.warmup, check)
+        # the plain form with its exchanges in device memory (IPC), a deviation from host TCP
+        dx = measure(xs, sigma_true, A, "fp64", 1 - args.device_exchange, args.steps, args.warmup, False)
+        variants["fp64_" + ("device_exchange" if args.device_exchange == 0 else "host_tcp")] = {
+            k: dx[k] for k in ("ms_per_step", "step_times_s", "stages_s")}
+    try:
+        main_run["roofline"]["isolated"] = isolated_gemms(n, rows)
+    except Exception as e:  # extra information; the runs themselves already finished
+        log(f"isolated GEMM timing failed: {e}")
+    second = None
+    if not args.no_variants and args.second_config and args.second_config != args.config:
+        del xs, A
         t0 = time.time()
-        dx = run_variant(xs, args.form, 1 - args.device_exchange, args.steps, args.warmup, False, None)
-        variants["device_exchange" if args.device_exchange == 0 else "host_tcp"] = {
-            "ms_per_step": round(dx["ms_per_step"], 3), "step_times_s": [round(s, 6) for s in dx["step_times"]],
-            "stages_s": dx["stages"]}
-        log(f"device-exchange variant in {time.time() - t0:.1f} s: {dx['ms_per_step']:.2f} ms")
+        xs2, s2, A2 = make_blocks(args.second_config)
+        P2, rows2, n2, spec2 = CONFIGS[args.second_config]
+        log(f"generated {args.second_config} in {time.time() - t0:.1f} s")
+        second = {"config": {"workload": f"{P2} peers x {n2} x {rows2} " + WORKLOADS[spec2] + "; full SVD",
+                             "peers": P2, "rows_per_peer": rows2, "columns": n2}}
+        for f in ("fp64", "mixed"):
+            second[f] = measure(xs2, s2, A2, f, args.device_exchange, max(1, args.steps // 2), 1, check)
+        second["speedup_mixed_vs_plain"] = round(second["fp64"]["ms_per_step"] / second["mixed"]["ms_per_step"], 3)
+        del xs2, A2
+    batched = None
+    if not args.no_variants and not args.no_batched:
+        try:
+            batched = bench_batched()
+        except Exception as e:  # reported, not fatal: the headline runs already finished
+            log(f"batched peers failed: {e}")
+            batched = {"error": str(e)}
+    learned = None
+    if not args.no_variants and not args.no_learned:
+        try:
+            learned = bench_learned()
+        except Exception as e:
+            log(f"learned brackets failed: {e}")
+            learned = {"error": str(e)}
     cpu = None
     if not args.no_cpu:
         t0 = time.time()
@@ -311,14 +452,18 @@ def main():
             sx, _, _ = make_blocks("cpu_sample", seed=1)
             gs = run_variant(sx, args.form, args.device_exchange, 2, 1, False, None)
             cpu["gpu_same_sample_s"] = round(gs["ms_per_step"] / 1000.0, 6)
+    roofline = dict(main_run["roofline"])
+    roofline["traffic_final_v_plain"] = pmc_traffic()
+    roofline["note"] = ("in-pipeline: both peer processes share the GPU, so a launch's duration includes the "
+                        "other peer's concurrent work; 'isolated' is the same kernel alone")
     line = {
         "metric": "SVD wall-time (s) + % roofline, mixed vs plain FP64, at <=1e-12 rel sigma err",
-        "value": round(ms_per_step / 1000.0, 6),
+        "value": round(main_run["ms_per_step"] / 1000.0, 6),
         "unit": "s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 3),
+        "ms_per_step": main_run["ms_per_step"],
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
@@ -326,21 +471,25 @@ def main():
         "data": "synthetic",
         "form": args.form,
         "transport": "host TCP (loopback)" if args.device_exchange == 0 else "device memory (IPC)",
-        "config": {"workload": f"{P} peers x {rows} x {n} float64 row blocks of A = Q1 diag(sigma) Q2^T, "
-                               f"sigma_i = 10^(-6i/{n - 1}); full SVD, Givens masks, bisection + twisted "
-                               f"(mode 1, opt 3)" if args.config != "c1" else
-                               f"{P} peers x {rows} x {n} N(0,1) row blocks, full SVD",
+        "config": {"workload": f"{P} peers x {rows} x {n} " + WORKLOADS[spec] + "; full SVD, Givens masks "
+                               "(plain: bisection + twisted, mode 1, opt 3)",
                    "peers": P, "rows_per_peer": rows, "columns": n},
-        "step_times_s": [round(s, 6) for s in step_times],
-        "stages_s": stages,
-        "results_d2h_s": round(main_run["results_d2h_s"], 6),
-        "accuracy": acc,
+        "step_times_s": main_run["step_times_s"],
+        "stages_s": main_run["stages_s"],
+        "results_d2h_s": main_run["results_d2h_s"],
+        "accuracy": main_run.get("accuracy", {}),
         "roofline": roofline,
-        "kernels": kernels,
+        "kernels": main_run["kernels"],
+        "forms": {f: {k: v for k, v in r.items() if k != "kernels"} for f, r in forms.items() if f != args.form},
         "variants": variants,
+        "second_workload": second,
+        "batched_peers": batched,
+        "learned_brackets": learned,
         "cpu_baseline": cpu,
         "reference_cpu_measured_s": 4.99,
     }
+    if other in forms:
+        line["speedup_mixed_vs_plain"] = round(forms["fp64"]["ms_per_step"] / forms["mixed"]["ms_per_step"], 3)
     print(json.dumps(line))
 
 

@@ -41,9 +41,12 @@ def _p(a):
     return a.ctypes.data if a.size else None
 
 
+_EMPTY_PAD = np.zeros(1)  # lives as long as the module, so its pointer never dangles
+
+
 def _pad(a):
     """A pointer that is valid even for an empty array (n = 1 bidiagonals have no beta)."""
-    return _p(a) if a.size else _p(np.zeros(1))
+    return _p(a) if a.size else _p(_EMPTY_PAD)
 
 
 def _call(name, *args):

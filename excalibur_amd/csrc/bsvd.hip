@@ -6,6 +6,8 @@
 // below), correctly rounded division and square root, so counts, twist indices, singular values
 // and vectors are bit-identical to a sequential evaluation of the same formulas.
 #include "exc_common.hpp"
+#include "sturm.hpp"
+
 
 #pragma clang fp contract(off)
 
@@ -20,24 +22,6 @@ struct BisectHdr {
   int flags;  // bit 1: a non-monotone Sturm count was met on some target's path
 };
 
-// Number of singular values of B below mu (bbtsvd.cpp:7-17).
-__device__ __forceinline__ int sturm_count(int n, const double* __restrict__ a, const double* __restrict__ b,
-                                           double mu) {
-  const double mu2 = mu * mu;
-  double t = -mu2;
-  double d = a[0] * a[0] + t;
-  int count = d < 0 ? 1 : 0;
-  // unrolled so the wave-uniform loads of a and b are issued ahead of the dependent chain
-#pragma unroll 8
-  for (int i = 1; i < n; i++) {
-    const double bb = b[i - 1] * b[i - 1];
-    const double aa = a[i] * a[i];
-    t = t * (bb / d) - mu2;
-    d = aa + t;
-    count += d < 0 ? 1 : 0;
-  }
-  return count;
-}
 
 __global__ __launch_bounds__(256) void k_sturm_count(int n, const double* __restrict__ a,
                                                      const double* __restrict__ b, const double* __restrict__ mu,

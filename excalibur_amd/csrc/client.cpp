@@ -5,6 +5,8 @@
 #include "client.hpp"
 
 #include <hip/hip_runtime.h>
+#include <fcntl.h>
+#include <sys/mman.h>
 #include <sys/stat.h>
 #include <unistd.h>
 
@@ -24,6 +26,7 @@
 #include "exc_common.hpp"
 #include "excalibur_hip.h"
 #include "net.hpp"
+#include "gemm.hpp"
 
 namespace exc {
 
@@ -96,6 +99,10 @@ struct ClientDevice {
   DBuf masked, P, Vt, tau_g, Qg, F, U, alpha, beta, house, tau_h, cs, bu, bv, sigma, flip, part, FinalU, tmp,
       FinalVT, msg, g, scal, work, plan, rt, msgd, b2, cnt2;
   HBuf h_masked, h_repart, h_msg, h_U;
+  // the mixed-precision form: FP32 copies of the data, the exchanged Grams, the Cholesky factors
+  // of the passes, and the host staging of the Grams and of the shared small SVD
+  DBuf x32, y32, g32, li32, Gm, L1, L2, L3, Lt, Uf, idm;
+  HBuf h_gram, h_svd;
   // the results as the host members hold them (pinned, so the copies run at the link's rate)
   HBuf r_sigma, r_FinalU, r_FinalVT, r_alpha, r_beta;
   void* link = nullptr;  // message link of the one-launch Ralha loop (coherent pinned memory)
@@ -231,11 +238,13 @@ class RunPlans {
       if (j->K == K && j->mask == mask && j->seed == seed) return *j;
     throw std::runtime_error("givens plan not prepared");
   }
-
- private:
+  // The plan thread is the only user of the C library's generator while it runs; callers that draw
+  // plans themselves wait for it first.
   void join() {
     if (th_.joinable()) th_.join();
   }
+
+ private:
   std::vector<std::unique_ptr<PlanJob>> jobs_;
   std::thread th_;
   bool failed_ = false;
@@ -246,13 +255,30 @@ class RunPlans {
 void givens_mask(double* d_x, long m, long n, int mask, int is_left, unsigned seed, RunPlans& plans, hipStream_t s) {
   const long K = is_left ? m : n;
   if (K <= 1) return;
-  if (K > 10000)
-    throw std::runtime_error("Givens masks with more than 10000 rotated lines use the virtual-matrix path, "
-                             "which this build does not implement");
+  if (K > 10000) {  // the virtual-matrix mask (bbtsvd.cpp:548-668); draws its own plans
+    plans.join();
+    DBuf w;
+    w.alloc((exc_block_shift_mask_workspace(m, n, 32, is_left) + 7) / 8, s, false);
+    ck(exc_block_shift_mask(d_x, m, n, 32, mask, is_left, seed, w.p, w.n * 8, s), "virtual-matrix mask");
+    return;
+  }
   PlanJob& p = plans.get(K, mask, seed);
   p.dev.alloc(p.count, s);
   ckh(hipMemcpyAsync(p.dev.p, p.host.p, p.count * sizeof(double), hipMemcpyHostToDevice, s), "plan");
   ck(exc_givens_apply(d_x, m, n, (const int32_t*)p.dev.p, p.dev.p + p.pair_words, 32, mask, is_left, s),
+     "givens apply");
+}
+
+// The same sweep on FP32 data (the rotation values stay FP64 in the plan, rounded per use).
+void givens_mask_f32(float* d_x, long m, long n, int mask, int is_left, unsigned seed, RunPlans& plans,
+                     hipStream_t s) {
+  const long K = is_left ? m : n;
+  if (K <= 1) return;
+  if (K > 10000) throw std::runtime_error("Givens masks of more than 10000 lines are not available in FP32");
+  PlanJob& p = plans.get(K, mask, seed);
+  p.dev.alloc(p.count, s);
+  ckh(hipMemcpyAsync(p.dev.p, p.host.p, p.count * sizeof(double), hipMemcpyHostToDevice, s), "plan");
+  ck(exc_givens_apply_f32(d_x, m, n, (const int32_t*)p.dev.p, p.dev.p + p.pair_words, 32, mask, is_left, s),
      "givens apply");
 }
 
@@ -293,7 +319,7 @@ DecentralizedClient::DecentralizedClient(int cid, int nclient, int mode, size_t 
   ckh(hipStreamCreateWithFlags(&dev_->s, hipStreamNonBlocking), "stream");
   load_data();
   const int repeat = std::max(1, opt_.repeat);
-  if (repeat > 1) {
+  if (repeat > 1 || evaluate) {  // runs overwrite X; the evaluation needs the raw data (raw_x)
     dev_->X0.alloc(m * local_n, dev_->s);
     ckh(hipMemcpyAsync(dev_->X0.p, dev_->X.p, m * local_n * 8, hipMemcpyDeviceToDevice, dev_->s), "copy");
     ckh(hipStreamSynchronize(dev_->s), "copy");
@@ -422,6 +448,34 @@ void DecentralizedClient::load_data() {
   dev_->X.alloc(count, s);
   if (opt_.data) {
     ckh(hipMemcpyAsync(dev_->X.p, opt_.data, count * sizeof(double), hipMemcpyHostToDevice, s), "upload");
+  } else if (is_memmap) {
+    // -f 1 (utils.cpp:229-252): the file is mapped, not read into a host buffer of its size; it
+    // streams to the device through a 64 MiB pinned staging buffer, so host memory stays bounded
+    // whatever the block's size
+    const std::string path = dpath + "/Client" + std::to_string(client_id) + ".mat";
+    const int fd = open(path.c_str(), O_RDONLY);
+    if (fd < 0) throw std::runtime_error("cannot open " + path);
+    const off_t size = lseek(fd, 0, SEEK_END);
+    if (size < (off_t)(count * sizeof(double))) {
+      close(fd);
+      throw std::runtime_error(path + " holds fewer than m * local_n doubles");
+    }
+    log("Load array size " + std::to_string(size / sizeof(double)));
+    void* map = mmap(nullptr, count * sizeof(double), PROT_READ, MAP_SHARED, fd, 0);
+    close(fd);
+    if (map == MAP_FAILED) throw std::runtime_error("cannot map " + path);
+    const size_t chunk = (size_t)8 << 20;  // doubles
+    HBuf st[2];
+    st[0].alloc(std::min(chunk, count), false);
+    st[1].alloc(std::min(chunk, count), false);
+    for (size_t o = 0, q = 0; o < count; o += chunk, q ^= 1) {
+      const size_t n = std::min(chunk, count - o);
+      ckh(hipStreamSynchronize(s), "upload");  // the staging buffer q is free again (two in flight)
+      memcpy(st[q].p, (const double*)map + o, n * sizeof(double));
+      ckh(hipMemcpyAsync(dev_->X.p + o, st[q].p, n * sizeof(double), hipMemcpyHostToDevice, s), "upload");
+    }
+    ckh(hipStreamSynchronize(s), "upload");
+    munmap(map, count * sizeof(double));
   } else {
     const std::string path = dpath + "/Client" + std::to_string(client_id) + ".mat";
     FILE* f = fopen(path.c_str(), "rb");
@@ -465,7 +519,8 @@ void DecentralizedClient::run() {
   if (opt_control == 0 || opt_control == 2) dense_mask_opt = false;
   if (!dense_mask_opt) throw std::runtime_error("opt 0/2 (dense random orthogonal masks) is not supported");
   if (svd_mode == 4) throw std::runtime_error("mode 4 (linear regression) is not supported");
-  if (opt_.form != "fp64") throw std::runtime_error("form '" + opt_.form + "' is not available in this build");
+  if (opt_.form != "fp64" && opt_.form != "mixed")
+    throw std::runtime_error("form '" + opt_.form + "' is not available (fp64 or mixed)");
   const bool compute_u = svd_mode != 3;
   const bool compute_v = svd_mode != 2;
 
@@ -506,6 +561,11 @@ void DecentralizedClient::run() {
   if (m >= global_n && compute_v)
     throw std::runtime_error("m == global_n takes the reference's tall-and-skinny combine branch, not supported");
   if (m < 3) throw std::runtime_error("m must be at least 3");
+  if (opt_.form == "mixed") {
+    const int k = run_mixed(compute_u, compute_v);
+    finish_run(k, compute_u, compute_v, true, t_run);
+    return;
+  }
 
   // 2) local QR (client.cpp:572-589)
   const size_t rl = reduced_local_ns[cid];
@@ -898,8 +958,16 @@ void DecentralizedClient::run() {
   }
   sync();
   time_combine_svd_remove_mask = now_seconds() - t0;
+  finish_run(k, compute_u, compute_v, lqr, t_run);
+}
+
+// Logs, host results and the result line of one run() (client.cpp:1200-1213).
+void DecentralizedClient::finish_run(int k, bool compute_u, bool compute_v, bool lqr, double t_run) {
+  exc::ClientDevice& D = *dev_;
+  hipStream_t s = D.s;
+  auto sync = [&] { ckh(hipStreamSynchronize(s), "stream sync"); };
   time_total = now_seconds() - t_run;
-  time_communication = net.comm_seconds();
+  time_communication = net_->comm_seconds();
 
   log("Finish DecFedSVD");
   log("DecFedSVD totally Costs " + std::to_string(time_total));
@@ -913,6 +981,7 @@ void DecentralizedClient::run() {
   log("time_bsvd=" + std::to_string(time_bsvd));
   log("time_combine_svd_remove_mask=" + std::to_string(time_combine_svd_remove_mask));
   log("time_ralha_allreduce=" + std::to_string(time_ralha_allreduce));
+  if (opt_.form == "mixed") log("time_refine=" + std::to_string(time_refine));
 
   sigma = FinalU = FinalVT = x_diagonal = x_upper_diagonal = nullptr;
   time_results_d2h = 0;
@@ -923,7 +992,7 @@ void DecentralizedClient::run() {
     log("time_results_d2h=" + std::to_string(time_results_d2h));
     if (opt_.dump_results) {
       auto dump = [&](const std::string& name, const double* v, size_t count) {
-        FILE* f = fopen((log_dir + "/Client" + std::to_string(cid) + "_" + name + ".bin").c_str(), "wb");
+        FILE* f = fopen((log_dir + "/Client" + std::to_string(client_id) + "_" + name + ".bin").c_str(), "wb");
         if (f) {
           fwrite(v, sizeof(double), count, f);
           fclose(f);
@@ -933,23 +1002,27 @@ void DecentralizedClient::run() {
       if (compute_u) dump("FinalU", FinalU, m * (size_t)k);
       if (compute_v) dump("FinalVT", FinalVT, (size_t)k * local_n);
       // the bidiagonal and its vectors, for stage-by-stage comparison
-      std::vector<double> v(m * (size_t)k);
-      ckh(hipMemcpyAsync(v.data(), D.bv.p, v.size() * 8, hipMemcpyDeviceToHost, s), "d2h");
-      sync();
       dump("alpha", x_diagonal, m);
       dump("beta", x_upper_diagonal, m - 1);
-      dump("bidiag_v", v.data(), v.size());
+      if (opt_.form != "mixed") {
+        std::vector<double> v(m * (size_t)k);
+        ckh(hipMemcpyAsync(v.data(), D.bv.p, v.size() * 8, hipMemcpyDeviceToHost, s), "d2h");
+        sync();
+        dump("bidiag_v", v.data(), v.size());
+      }
     }
   }
+  if (evaluate) evaluate_results(k, compute_u, compute_v, lqr);
   char json[1024];
   snprintf(json, sizeof json,
            "{\"cid\": %d, \"sigma_count\": %d, \"total\": %.6f, \"communication\": %.6f, \"time_init\": %.6f, "
            "\"time_local_qr\": %.6f, \"time_global_qr\": %.6f, \"time_apply_mask\": %.6f, "
            "\"time_exchange_low_dim\": %.6f, \"time_bi_diag\": %.6f, \"time_ralha_allreduce\": %.6f, "
-           "\"time_bsvd\": %.6f, \"time_combine_svd_remove_mask\": %.6f, \"time_results_d2h\": %.6f}",
-           cid, k, time_total, time_communication, time_init, time_local_qr, time_global_qr, time_apply_mask,
+           "\"time_bsvd\": %.6f, \"time_combine_svd_remove_mask\": %.6f, \"time_results_d2h\": %.6f, "
+           "\"time_refine\": %.6f, \"form\": \"%s\"}",
+           client_id, k, time_total, time_communication, time_init, time_local_qr, time_global_qr, time_apply_mask,
            time_exchange_low_dim, time_bi_diag, time_ralha_allreduce, time_bsvd, time_combine_svd_remove_mask,
-           time_results_d2h);
+           time_results_d2h, time_refine, opt_.form.c_str());
   printf("EXC_RESULT %s\n", json);
   if (opt_.profile) {
     std::vector<char> pb(1 << 16);
@@ -1053,4 +1126,452 @@ void DecentralizedClient::local_qr_reduction(size_t mr, size_t n, double** raw_x
     }
   delete[] *raw_x;
   *raw_x = out;
+}
+
+// The mixed-precision form (ClientOptions::form = "mixed"). The m x m factor R of A (A = Q R,
+// A^T = [X_0 ... X_{P-1}]) comes from a CholeskyQR over all peers' Gram matrices instead of the
+// local QR, the masked exchange and the global QR of client.cpp:572-798, and its SVD is computed
+// once, by peer 0, instead of the per-column all-reduce of client.cpp:829-910:
+//   pass 1 (FP32 MFMA): G = sum_i X_i X_i^T, masked with the public Givens plan on both sides
+//     before the exchange, shifted by u32 * trace(G) until its Cholesky factor L1 exists;
+//   pass 2 (FP32 MFMA): Y_i = L1^-1 X_i and the Cholesky factor L2 of sum_i Y_i Y_i^T: R1 = L1 L2
+//     is the R^T of a shifted CholeskyQR2 in single precision;
+//   pass 3 (FP64, the refinement of Q): Z_i = R1^-1 X_i by triangular solve, L3 from
+//     sum_i Z_i Z_i^T, which is I to within FP32 error; Q_i^T = L3^-1 Z_i and R^T = Lt = R1 L3
+//     (repeated, Z_i <- L3^-1 Z_i, while ||sum_i Z_i Z_i^T - I||_F > 1/2: at most 3 passes);
+//   peer 0: one-sided bidiagonalization of Lt^T (one owner, no exchange), bisection + twisted
+//     factorization (FP64, as the plain form), then the FP64 Ogita-Aishima refinement of
+//     Lt = U diag(sigma) W^T (exc_oa_refine); it sends sigma, U and W' = L3^-T W to the others;
+//   every peer: FinalVT_i = W'^T Z_i = W^T Q_i^T.
+// Every exchange is a sum of Gram matrices (bit-identical on every peer) or peer 0's broadcast.
+int DecentralizedClient::run_mixed(bool compute_u, bool compute_v) {
+  hipStream_t s = dev_->s;
+  exc::ClientDevice& D = *dev_;
+  exc::PeerNet& net = *net_;
+  const int cid = client_id;
+  auto sync = [&] { ckh(hipStreamSynchronize(s), "stream sync"); };
+  const int mi = (int)m;
+  const long r = (long)local_n;
+  const size_t mm = m * m;
+  const double u32 = std::ldexp(1.0, -24);
+  RunPlans plans;
+  plans.start({{(long)m, 1, public_seed}, {(long)m, 0, public_seed}});
+  double t_mask = 0, t_comm = 0;
+  double t0 = now_seconds();
+
+  // workspaces: split-K partials of the Grams and the factorization workspace
+  const size_t gws = std::max(exc_gemm_workspace(m, m, r), exc_gemm_workspace(m, m, m));
+  const size_t fws = exc_fact_workspace(mi, std::max<long>(r, (long)m));
+  D.work.alloc((std::max(gws, fws) + 7) / 8, s, false);
+  void* work = D.work.p;
+  const size_t work_bytes = D.work.n * 8;
+  ExcGemmOpts gram;
+  gram.lower_only = 1;
+  gram.mirror = 1;
+  gram.work = work;
+  gram.work_bytes = work_bytes;
+  ExcGemmOpts plain;
+  plain.work = work;
+  plain.work_bytes = work_bytes;
+
+  D.x32.alloc((m * local_n + 1) / 2, s, false);
+  D.y32.alloc((m * local_n + 1) / 2, s, false);
+  D.g32.alloc((mm + 1) / 2, s, false);
+  D.li32.alloc((mm + 1) / 2, s, false);
+  float* x32 = (float*)D.x32.p;
+  float* y32 = (float*)D.y32.p;
+  float* g32 = (float*)D.g32.p;
+  float* li32 = (float*)D.li32.p;
+  D.Gm.alloc(mm, s, false);
+  D.L1.alloc(mm, s, false);
+  D.L2.alloc(mm, s, false);
+  D.L3.alloc(mm, s, false);
+  D.Lt.alloc(mm, s, false);
+  D.idm.alloc(mm, s, false);
+  D.scal.alloc(4, s);
+  D.h_gram.alloc(mm, false);
+
+  // masked FP32 Gram in g32 -> sum over the peers -> unmasked FP64 Gram in D.Gm
+  auto exchange_f32 = [&]() {
+    double ta = now_seconds();
+    givens_mask_f32(g32, (long)m, (long)m, 1, 1, public_seed, plans, s);
+    givens_mask_f32(g32, (long)m, (long)m, 1, 0, public_seed, plans, s);
+    ck(exc_convert_f32_f64((long)m, (long)m, g32, (long)m, D.Gm.p, (long)m, s), "convert");
+    ckh(hipMemcpyAsync(D.h_gram.p, D.Gm.p, mm * 8, hipMemcpyDeviceToHost, s), "d2h");
+    sync();
+    double tb = now_seconds();
+    net.all_reduce_sum_same(D.h_gram.p, (long)mm);
+    double tc = now_seconds();
+    ckh(hipMemcpyAsync(D.Gm.p, D.h_gram.p, mm * 8, hipMemcpyHostToDevice, s), "h2d");
+    givens_mask(D.Gm.p, (long)m, (long)m, 0, 1, public_seed, plans, s);
+    givens_mask(D.Gm.p, (long)m, (long)m, 0, 0, public_seed, plans, s);
+    sync();
+    t_mask += (tb - ta) + (now_seconds() - tc);
+    t_comm += tc - tb;
+  };
+  // Cholesky of D.Gm + shift I into L, the shift doubled from `shift` (0: first without one)
+  auto shifted_chol = [&](DBuf& L, double shift, const char* what) {
+    double tr = 0.0;
+    ck(exc_trace(mi, D.Gm.p, (long)m, D.scal.p, s), "trace");
+    ckh(hipMemcpyAsync(&tr, D.scal.p, 8, hipMemcpyDeviceToHost, s), "d2h");
+    sync();
+    const double step = u32 * tr;
+    if (shift < 0) shift = step;  // a first pass in single precision always needs one
+    for (int attempt = 0; attempt < 60; attempt++) {
+      ckh(hipMemcpyAsync(L.p, D.Gm.p, mm * 8, hipMemcpyDeviceToDevice, s), "copy");
+      if (shift > 0) ck(exc_add_diag(mi, L.p, (long)m, shift, s), "shift");
+      int info = 0;
+      ck(exc_potrf_lower(mi, L.p, (long)m, (int*)(D.scal.p + 1), work, work_bytes, s), what);
+      ckh(hipMemcpyAsync(&info, D.scal.p + 1, sizeof(int), hipMemcpyDeviceToHost, s), "d2h");
+      sync();
+      if (info == 0) {
+        if (shift > 0) log(std::string(what) + ": shift " + std::to_string(shift));
+        return;
+      }
+      shift = shift > 0 ? 2 * shift : step;
+    }
+    throw std::runtime_error(std::string(what) + ": no shift made the Gram matrix positive definite");
+  };
+
+  // ---- pass 1 (FP32): G = X X^T
+  ck(exc_convert_f64_f32((long)m, r, D.X.p, r, x32, r, s), "convert");
+  ck(exc_sgemm_opts(0, 1, (long)m, (long)m, r, 1.0, x32, r, x32, r, 0.0, g32, (long)m, gram, s), "gram 1");
+  sync();
+  double tf = now_seconds() - t0;
+  exchange_f32();
+  t0 = now_seconds();
+  shifted_chol(D.L1, -1.0, "pass 1 Cholesky");
+  // ---- pass 2 (FP32): Y = L1^-1 X through the explicit inverse, G2 = Y Y^T
+  ck(exc_fill2d((long)m, (long)m, 0.0, D.idm.p, (long)m, s), "identity");
+  ck(exc_add_diag(mi, D.idm.p, (long)m, 1.0, s), "identity");
+  ck(exc_trsm_left(0, 0, mi, (long)m, D.L1.p, (long)m, D.idm.p, (long)m, work, work_bytes, s), "inverse");
+  ck(exc_convert_f64_f32((long)m, (long)m, D.idm.p, (long)m, li32, (long)m, s), "convert");
+  {
+    ExcGemmOpts tri = plain;
+    tri.tri = EXC_TRI_A_LOWER;
+    ck(exc_sgemm_opts(0, 0, (long)m, r, (long)m, 1.0, li32, (long)m, x32, r, 0.0, y32, r, tri, s), "pass 2 solve");
+  }
+  ck(exc_sgemm_opts(0, 1, (long)m, (long)m, r, 1.0, y32, r, y32, r, 0.0, g32, (long)m, gram, s), "gram 2");
+  sync();
+  tf += now_seconds() - t0;
+  exchange_f32();
+  t0 = now_seconds();
+  shifted_chol(D.L2, 0.0, "pass 2 Cholesky");
+  // R1 = L1 L2 (lower)
+  {
+    ExcGemmOpts tri = plain;
+    tri.tri = EXC_TRI_A_LOWER;
+    ck(exc_gemm_opts(0, 0, (long)m, (long)m, (long)m, 1.0, D.L1.p, (long)m, D.L2.p, (long)m, 0.0, D.Lt.p, (long)m,
+                     tri, s),
+       "R1");
+  }
+  // ---- pass 3 (FP64): Z = R1^-1 X in place, G3 = Z Z^T
+  ck(exc_trsm_left(0, 0, mi, r, D.Lt.p, (long)m, D.X.p, r, work, work_bytes, s), "pass 3 solve");
+  // G3 summed over the peers, masked for the exchange like the FP32 Grams
+  auto gram_f64 = [&]() {
+    ck(exc_gemm_opts(0, 1, (long)m, (long)m, r, 1.0, D.X.p, r, D.X.p, r, 0.0, D.Gm.p, (long)m, gram, s), "gram 3");
+    sync();
+    tf += now_seconds() - t0;
+    double ta = now_seconds();
+    givens_mask(D.Gm.p, (long)m, (long)m, 1, 1, public_seed, plans, s);
+    givens_mask(D.Gm.p, (long)m, (long)m, 1, 0, public_seed, plans, s);
+    ckh(hipMemcpyAsync(D.h_gram.p, D.Gm.p, mm * 8, hipMemcpyDeviceToHost, s), "d2h");
+    sync();
+    double tb = now_seconds();
+    net.all_reduce_sum_same(D.h_gram.p, (long)mm);
+    double tc = now_seconds();
+    ckh(hipMemcpyAsync(D.Gm.p, D.h_gram.p, mm * 8, hipMemcpyHostToDevice, s), "h2d");
+    givens_mask(D.Gm.p, (long)m, (long)m, 0, 1, public_seed, plans, s);
+    givens_mask(D.Gm.p, (long)m, (long)m, 0, 0, public_seed, plans, s);
+    sync();
+    t_mask += (tb - ta) + (now_seconds() - tc);
+    t_comm += tc - tb;
+    t0 = now_seconds();
+  };
+  auto lt_times_l3 = [&]() {
+    // Lt = Lt L3 (both lower); D.L1 is free after pass 2
+    ExcGemmOpts tri = plain;
+    tri.tri = EXC_TRI_A_LOWER;
+    ckh(hipMemcpyAsync(D.L1.p, D.Lt.p, mm * 8, hipMemcpyDeviceToDevice, s), "copy");
+    ck(exc_gemm_opts(0, 0, (long)m, (long)m, (long)m, 1.0, D.L1.p, (long)m, D.L3.p, (long)m, 0.0, D.Lt.p, (long)m,
+                     tri, s),
+       "Lt");
+  };
+  // One FP64 CholeskyQR pass leaves ||Q^T Q - I|| ~ u kappa(Z)^2, with kappa(Z)^2 <=
+  // (1 + e) / (1 - e) for e = ||Z Z^T - I||_F. When the FP32 passes left e above 1/2 (a condition
+  // number near the reach of single precision), Z is first replaced by L3^-1 Z and the pass
+  // repeated. The Gram matrix is bit-identical on every peer, so every peer takes the same branch.
+  int fp64_passes = 0;
+  for (;;) {
+    gram_f64();
+    fp64_passes++;
+    ck(exc_dev_identity(mi, D.Gm.p, (long)m, D.scal.p + 2, s), "deviation");
+    double dev[2] = {0.0, 0.0};
+    ckh(hipMemcpyAsync(dev, D.scal.p + 2, 16, hipMemcpyDeviceToHost, s), "d2h");
+    sync();
+    log("mixed form: FP64 pass " + std::to_string(fp64_passes) + ", max |Z Z^T - I| " + std::to_string(dev[0]) +
+        ", ||Z Z^T - I||_F " + std::to_string(dev[1]));
+    shifted_chol(D.L3, 0.0, "pass 3 Cholesky");
+    lt_times_l3();
+    if (dev[1] <= 0.5 || fp64_passes == 3) break;
+    ck(exc_trsm_left(0, 0, mi, r, D.L3.p, (long)m, D.X.p, r, work, work_bytes, s), "extra pass solve");
+  }
+  sync();
+  tf += now_seconds() - t0;
+  time_local_qr = tf;
+  time_apply_mask = t_mask;
+  time_exchange_low_dim = t_comm;
+  time_global_qr = 0;
+  log("Mixed CholeskyQR (FP32 x 2, FP64 x " + std::to_string(fp64_passes) + ") Costs " + std::to_string(tf));
+
+  // ---- the SVD of Lt on peer 0, sent to the others
+  int k = 0;
+  D.alpha.alloc(m, s);
+  D.beta.alloc(m, s);
+  // message to the other peers: k, then sigma (m), U (m x m), W' (m x m), alpha (m), beta (m)
+  const size_t msg_doubles = 1 + m + 2 * mm + 2 * m;
+  D.h_svd.alloc(msg_doubles, false);
+  D.Uf.alloc(mm, s, false);
+  D.bu.alloc(mm, s);
+  D.sigma.alloc(m, s);
+  if (cid == 0) {
+    t0 = now_seconds();
+    D.F.alloc(mm, s, false);
+    D.U.alloc(mm, s);
+    D.house.alloc(mm, s);
+    D.tau_h.alloc(m, s);
+    D.cs.alloc(2 * m, s);
+    ckh(hipMemcpyAsync(D.F.p, D.Lt.p, mm * 8, hipMemcpyDeviceToDevice, s), "copy");  // F = Lt^T column-major
+    if (!D.link || D.link_m != mi) {
+      if (D.link) ck(exc_ralha_link_free(D.link), "ralha link");
+      D.link = nullptr;
+      ck(exc_ralha_link_alloc(mi, &D.link), "ralha link");
+      D.link_m = mi;
+    }
+    D.msgd.alloc((exc_bidiag_local_workspace(mi, mi) + 7) / 8, s, false);
+    ck(exc_bidiag_local(mi, mi, D.F.p, D.U.p, D.house.p, D.tau_h.p, D.alpha.p, D.beta.p, D.cs.p,
+                        opt_.ralha_one_launch ? D.link : nullptr, D.msgd.p, D.msgd.n * 8,
+                        std::max(30.0, opt_.recv_timeout_s), s),
+       "bidiagonalization");
+    sync();
+    time_bi_diag = now_seconds() - t0;
+    t0 = now_seconds();
+    DBuf& b2 = D.b2;
+    b2.alloc(2, s);
+    DBuf& cnt2 = D.cnt2;
+    cnt2.alloc(1, s);
+    ck(exc_bisection_bound(mi, D.alpha.p, D.beta.p, b2.p, s), "bound");
+    ck(exc_sturm_count(mi, D.alpha.p, D.beta.p, b2.p, 2, (int*)cnt2.p, s), "count");
+    int hc[2];
+    ckh(hipMemcpyAsync(hc, cnt2.p, sizeof hc, hipMemcpyDeviceToHost, s), "d2h");
+    sync();
+    k = std::min(hc[0] - hc[1], mi);
+    D.bv.alloc(mm, s);
+    D.flip.alloc(m, s);
+    const size_t wb = exc_bidiag_svd_workspace(mi, k);
+    D.work.alloc(std::max(D.work.n, (wb + 7) / 8), s, false);
+    ck(exc_bidiag_svd(mi, D.alpha.p, D.beta.p, D.bu.p, D.bv.p, D.sigma.p, k, nullptr, D.work.p, D.work.n * 8, s),
+       "bidiagonal SVD");
+    ck(exc_sign_normalize(mi, k, D.bu.p, D.bv.p, D.flip.p, s), "sign rule");
+    // U = (bv U_r)^T: the left vectors of Lt; W = H (rotations back) bu: the right vectors
+    ck(exc_dgemm(0, 1, (long)m, k, (long)m, 1.0, D.U.p, (long)m, D.bv.p, (long)m, 0.0, D.Uf.p, k, nullptr, 0, s),
+       "U");
+    ck(exc_rot_back(mi, k, D.bu.p, D.cs.p, s), "back rotation");
+    {
+      const size_t ow = exc_ormqr_house_workspace(mi, k);
+      D.work.alloc(std::max(D.work.n, (ow + 7) / 8), s, false);
+      ck(exc_ormqr_house_ln(mi, k, D.house.p, D.tau_h.p, D.bu.p, k, D.work.p, D.work.n * 8, s), "ormqr house");
+    }
+    sync();
+    time_bsvd = now_seconds() - t0;
+    t0 = now_seconds();
+    if (k == mi && opt_.refine_iters > 0) {
+      const size_t rw = exc_oa_refine_workspace(mi);
+      D.work.alloc(std::max(D.work.n, (rw + 7) / 8), s, false);
+      ck(exc_oa_refine(mi, D.Lt.p, (long)m, 1, D.Uf.p, (long)m, D.bu.p, (long)m, D.sigma.p, opt_.refine_iters, 1e-6,
+                       1e-4, D.work.p, D.work.n * 8, s),
+         "refinement");
+    }
+    // W' = L3^-T W through the upper triangle L3^T
+    ck(exc_transpose((long)m, (long)m, D.L3.p, (long)m, D.idm.p, (long)m, s), "transpose");
+    ck(exc_trsm_left(1, 0, mi, k, D.idm.p, (long)m, D.bu.p, k, D.work.p, D.work.n * 8, s), "W'");
+    sync();
+    time_refine = now_seconds() - t0;
+    double* h = D.h_svd.p;
+    h[0] = k;
+    ckh(hipMemcpyAsync(h + 1, D.sigma.p, (size_t)k * 8, hipMemcpyDeviceToHost, s), "d2h");
+    ckh(hipMemcpyAsync(h + 1 + m, D.Uf.p, m * (size_t)k * 8, hipMemcpyDeviceToHost, s), "d2h");
+    ckh(hipMemcpyAsync(h + 1 + m + mm, D.bu.p, m * (size_t)k * 8, hipMemcpyDeviceToHost, s), "d2h");
+    ckh(hipMemcpyAsync(h + 1 + m + 2 * mm, D.alpha.p, m * 8, hipMemcpyDeviceToHost, s), "d2h");
+    ckh(hipMemcpyAsync(h + 1 + 2 * m + 2 * mm, D.beta.p, m * 8, hipMemcpyDeviceToHost, s), "d2h");
+    sync();
+    const double tb = now_seconds();
+    if (num_client > 1) net.broadcast(h, msg_doubles * 8);
+    t_comm += now_seconds() - tb;
+    net.add_comm_seconds(now_seconds() - tb);
+  } else {
+    t0 = now_seconds();
+    net.receive_into(0, D.h_svd.p, msg_doubles * 8);
+    t_comm += now_seconds() - t0;
+    net.add_comm_seconds(now_seconds() - t0);
+    const double* h = D.h_svd.p;
+    k = (int)h[0];
+    ckh(hipMemcpyAsync(D.sigma.p, h + 1, (size_t)k * 8, hipMemcpyHostToDevice, s), "h2d");
+    ckh(hipMemcpyAsync(D.Uf.p, h + 1 + m, m * (size_t)k * 8, hipMemcpyHostToDevice, s), "h2d");
+    ckh(hipMemcpyAsync(D.bu.p, h + 1 + m + mm, m * (size_t)k * 8, hipMemcpyHostToDevice, s), "h2d");
+    ckh(hipMemcpyAsync(D.alpha.p, h + 1 + m + 2 * mm, m * 8, hipMemcpyHostToDevice, s), "h2d");
+    ckh(hipMemcpyAsync(D.beta.p, h + 1 + 2 * m + 2 * mm, m * 8, hipMemcpyHostToDevice, s), "h2d");
+    sync();
+  }
+  time_exchange_low_dim = t_comm;
+
+  // ---- results: the leading kk triplets
+  t0 = now_seconds();
+  const int kk = top_k > 0 ? std::min(top_k, k) : k;
+  sigma_count = kk;
+  if (compute_u) {
+    D.FinalU.alloc(m * (size_t)kk, s, false);
+    ck(exc_copy2d((long)m, kk, D.Uf.p, k, D.FinalU.p, kk, s), "FinalU");
+  }
+  if (compute_v) {
+    D.FinalVT.alloc((size_t)kk * local_n, s, false);
+    exc_prof::label_next("gemm_f64_final_v");
+    ck(exc_dgemm(1, 0, kk, r, (long)m, 1.0, D.bu.p, k, D.X.p, r, 0.0, D.FinalVT.p, r, nullptr, 0, s), "final V");
+  }
+  sync();
+  time_combine_svd_remove_mask = (cid == 0 ? time_refine : 0.0) + (now_seconds() - t0);
+  D.x32.release();
+  D.y32.release();
+  return kk;
+}
+
+// The reference's evaluation report (client.cpp:1215-1309, short-and-wide branch), computed on
+// the device with the reductions of eval.hip; ground truth from dpath/u.mat (k x m, rows u_i) and
+// dpath/vt.mat (k x global_n, rows v_i) as generate_data.cpp writes them. The values go to the log
+// under the reference's texts and to one EXC_EVAL line.
+extern "C" {
+size_t exc_eval_workspace(void);
+int exc_eval_reduce(int op, long rows, long cols, const double* d_x, long ldx, const double* d_y, long ldy, long row0,
+                    double* d_out, void* d_work, size_t work_bytes, exc_stream_t stream);
+}
+
+namespace {
+std::vector<double> read_doubles(const std::string& path, size_t count) {
+  std::vector<double> v(count);
+  FILE* f = fopen(path.c_str(), "rb");
+  if (!f) throw std::runtime_error("cannot open " + path);
+  const size_t got = fread(v.data(), sizeof(double), count, f);
+  fclose(f);
+  if (got != count) throw std::runtime_error(path + " is shorter than expected");
+  return v;
+}
+}  // namespace
+
+void DecentralizedClient::evaluate_results(int k, bool compute_u, bool compute_v, bool lqr) {
+  exc::ClientDevice& D = *dev_;
+  hipStream_t s = D.s;
+  auto sync = [&] { ckh(hipStreamSynchronize(s), "stream sync"); };
+  DBuf ew, out, tmp, gt;
+  ew.alloc((exc_eval_workspace() + 7) / 8, s, false);
+  out.alloc(2, s);
+  auto reduce = [&](int op, long rows, long cols, const double* x, long ldx, const double* y, long ldy, long row0) {
+    ck(exc_eval_reduce(op, rows, cols, x, ldx, y, ldy, row0, out.p, ew.p, ew.n * 8, s), "evaluation");
+    double h[2];
+    ckh(hipMemcpyAsync(h, out.p, sizeof h, hipMemcpyDeviceToHost, s), "d2h");
+    sync();
+    return std::make_pair(h[0], h[1]);
+  };
+  const double* fvt = compute_v ? (lqr ? D.FinalVT.p : D.tmp.p) : nullptr;
+  std::string js = "{\"cid\": " + std::to_string(client_id);
+  char num[64];
+  auto add = [&](const char* key, double v) {
+    snprintf(num, sizeof num, "%.17g", v);
+    js += std::string(", \"") + key + "\": " + num;
+  };
+  if (compute_u && client_id == 0) {  // orthogonal_test(FinalU, m, k, false), utils.cpp:38-66
+    tmp.alloc((size_t)k * k, s, false);
+    ck(exc_dgemm(1, 0, k, k, (long)m, 1.0, D.FinalU.p, k, D.FinalU.p, k, 0.0, tmp.p, k, nullptr, 0, s), "U^T U");
+    const auto r = reduce(3, k, k, tmp.p, k, nullptr, 0, 0);
+    const double mean = k > 1 ? r.first / ((double)k * k - k) : 0.0;
+    log("Orthogonal Test " + std::to_string(mean) + " Max " + std::to_string(r.second));
+    log("Final U OrthTest " + std::to_string(mean));
+    add("u_orth_mean", mean);
+    add("u_orth_max", r.second);
+  }
+  if (svd_mode == 0 && compute_v && compute_u) {
+    // singular vector error (RMSE of |v| entries) against vt.mat, and the SVD error
+    const size_t gn = global_n;
+    std::vector<double> h = read_doubles(dpath + "/vt.mat", (size_t)k * gn);
+    gt.alloc((size_t)k * local_n, s, false);
+    ckh(hipMemcpy2DAsync(gt.p, local_n * 8, h.data() + n_pos[client_id], gn * 8, local_n * 8, k,
+                         hipMemcpyHostToDevice, s),
+        "h2d");
+    double err = reduce(2, k, (long)local_n, fvt, (long)local_n, gt.p, (long)local_n, 0).first;
+    net_->ring_all_reduce_sum(&err, 1);
+    const double rmse = std::sqrt(err / ((double)k * gn));
+    add("singular_vector_error_vt", rmse);
+    // mean |U diag(sigma) V^T - X| (utils.cpp:96-150)
+    tmp.alloc(m * (size_t)k, s, false);
+    ckh(hipMemcpyAsync(tmp.p, D.FinalU.p, m * (size_t)k * 8, hipMemcpyDeviceToDevice, s), "copy");
+    DBuf sc;
+    sc.alloc(m * (size_t)k, s, false);
+    // U diag(sigma): scale the columns through a diagonal product
+    DBuf dg;
+    dg.alloc((size_t)k * k, s);
+    std::vector<double> hs(k);
+    ckh(hipMemcpyAsync(hs.data(), D.sigma.p, k * 8, hipMemcpyDeviceToHost, s), "d2h");
+    sync();
+    std::vector<double> hd((size_t)k * k, 0.0);
+    for (int i = 0; i < k; i++) hd[(size_t)i * k + i] = hs[i];
+    ckh(hipMemcpyAsync(dg.p, hd.data(), hd.size() * 8, hipMemcpyHostToDevice, s), "h2d");
+    ck(exc_dgemm(0, 0, (long)m, k, k, 1.0, tmp.p, k, dg.p, k, 0.0, sc.p, k, nullptr, 0, s), "U S");
+    DBuf rx;
+    rx.alloc(m * local_n, s, false);
+    ckh(hipMemcpyAsync(rx.p, D.X0.p, m * local_n * 8, hipMemcpyDeviceToDevice, s), "copy");
+    ck(exc_dgemm(0, 0, (long)m, (long)local_n, k, 1.0, sc.p, k, fvt, (long)local_n, -1.0, rx.p, (long)local_n,
+                 nullptr, 0, s),
+       "U S V^T - X");
+    const double e = reduce(0, (long)m, (long)local_n, rx.p, (long)local_n, nullptr, 0, 0).first / ((double)m * local_n);
+    log("Final SVD Error " + std::to_string(e));
+    add("svd_error", e);
+  }
+  if ((svd_mode == 1 || svd_mode == 2) && compute_u && client_id == 0) {
+    // projection distance of U against u.mat (k x m): the upper triangle of U_gt U_gt^T - U U^T
+    std::vector<double> h = read_doubles(dpath + "/u.mat", (size_t)k * m);
+    gt.alloc((size_t)k * m, s, false);
+    ckh(hipMemcpyAsync(gt.p, h.data(), (size_t)k * m * 8, hipMemcpyHostToDevice, s), "h2d");
+    tmp.alloc(m * m, s, false);
+    ck(exc_dgemm(1, 0, (long)m, (long)m, k, 1.0, gt.p, (long)m, gt.p, (long)m, 0.0, tmp.p, (long)m, nullptr, 0, s),
+       "P_gt");
+    ck(exc_dgemm(0, 1, (long)m, (long)m, k, 1.0, D.FinalU.p, k, D.FinalU.p, k, -1.0, tmp.p, (long)m, nullptr, 0, s),
+       "P - P_gt");
+    const double e = reduce(1, (long)m, (long)m, tmp.p, (long)m, nullptr, 0, 0).first / ((double)m * (m + 1) / 2);
+    log("Projection Distance U " + std::to_string(e));
+    add("projection_distance_u", e);
+  }
+  if ((svd_mode == 1 || svd_mode == 3) && compute_v) {
+    // projection distance of V (this peer's columns of vt.mat), by row tiles of the
+    // local_n x local_n difference
+    const size_t gn = global_n;
+    std::vector<double> h = read_doubles(dpath + "/vt.mat", (size_t)k * gn);
+    gt.alloc((size_t)k * local_n, s, false);
+    ckh(hipMemcpy2DAsync(gt.p, local_n * 8, h.data() + n_pos[client_id], gn * 8, local_n * 8, k,
+                         hipMemcpyHostToDevice, s),
+        "h2d");
+    const long ln = (long)local_n, T = std::min<long>(ln, 2048);
+    tmp.alloc((size_t)T * ln, s, false);
+    double err = 0.0;
+    for (long i0 = 0; i0 < ln; i0 += T) {
+      const long rows = std::min(T, ln - i0);
+      ck(exc_dgemm(1, 0, rows, ln, k, 1.0, gt.p + i0, ln, gt.p, ln, 0.0, tmp.p, ln, nullptr, 0, s), "P_gt");
+      ck(exc_dgemm(1, 0, rows, ln, k, 1.0, fvt + i0, ln, fvt, ln, -1.0, tmp.p, ln, nullptr, 0, s), "P - P_gt");
+      err += reduce(1, rows, ln, tmp.p, ln, nullptr, 0, i0).first;
+    }
+    const double e = err / ((double)ln * (ln + 1) / 2);
+    log("Projection Distance VT " + std::to_string(e));
+    add("projection_distance_vt", e);
+  }
+  log("Evaluation Done");
+  printf("EXC_EVAL %s}\n", js.c_str());
+  fflush(stdout);
 }

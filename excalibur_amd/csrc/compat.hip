@@ -367,9 +367,18 @@ int exc_bb_givens_mask(size_t m, size_t n, size_t mask_times, double* x, int mas
 int exc_bb_block_shift_mask(size_t m, size_t n, size_t mask_times, double* x, int mask, int is_left,
                             unsigned int seed) {
   const size_t K = is_left ? m : n;
-  EXC_REQUIRE(K <= 10000, EXC_ERR_ARG,
-              "exc_bb_block_shift_mask: the virtual-matrix mask for more than 10000 lines is not implemented");
-  return exc_bb_givens_mask(m, n, mask_times, x, mask, is_left, seed);
+  if (K <= 10000) return exc_bb_givens_mask(m, n, mask_times, x, mask, is_left, seed);
+  // the virtual-matrix mask (bbtsvd.cpp:548-668) on the device
+  EXC_REQUIRE(x, EXC_ERR_ARG, "exc_bb_block_shift_mask: null matrix");
+  srand(seed);
+  Scope S;
+  double* dx = S.upload(x, m * n);
+  const size_t wb = exc_block_shift_mask_workspace((long)m, (long)n, (long)mask_times, is_left);
+  void* dw = S.alloc<char>(wb);
+  EXC_DEV(dx && dw);
+  int rc = exc_block_shift_mask(dx, (long)m, (long)n, (long)mask_times, mask, is_left, seed, dw, wb, nullptr);
+  if (rc) return rc;
+  return download(x, dx, m * n * sizeof(double));
 }
 
 int exc_bb_find_best_convert(size_t k, size_t* out_m, size_t* out_n, size_t* out_remain) {

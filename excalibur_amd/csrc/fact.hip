@@ -452,6 +452,8 @@ static int potrf_rec(int n, double* A, long lda, int k0, int* info, void* work, 
 
 int potrf_lower(int n, double* A, long lda, int* info, void* work, size_t wb, hipStream_t s) {
   EXC_REQUIRE(n >= 0 && A, EXC_ERR_ARG, "potrf_lower: bad arguments");
+  // *info is 0 unless this call meets a non-positive pivot (the kernels only write failures)
+  if (info) EXC_CHECK_HIP(hipMemsetAsync(info, 0, sizeof(int), s));
   if (n == 0) return EXC_OK;
   if (n > NB && n <= kTileMaxN) return potrf_tiles(n, A, lda, info, work, wb, s);
   int rc = potrf_rec(n, A, lda, 0, info, work, wb, s);

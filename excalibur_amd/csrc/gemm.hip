@@ -7,6 +7,12 @@
 // summed in a fixed order by a second kernel, so results do not depend on scheduling), skipping
 // the zero K range of a triangular operand, and computing only the lower tiles of a symmetric
 // product (mirrored into the upper half by the reduction).
+#include <algorithm>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
 #include "exc_common.hpp"
 #include "gemm.hpp"
 
@@ -122,23 +128,16 @@ struct Stager {
   __device__ inline int b_k(int q) const { return TB ? bk : bk + 2 * q; }
 };
 
+// One output tile [i0, i0+BM) x [j0, j0+BN) over the K range [k0, k1). The result element (gi, gj)
+// goes to Cb[(gi - ci0) * ldc + (gj - cj0)]: scaled by alpha (plus beta times the old value) when
+// direct, else stored as it is (a split-K partial).
 template <typename T, int TA, int TB>
-__global__ __launch_bounds__(NT, 2) void k_gemm(GemmArgs g) {
+__device__ __forceinline__ void gemm_tile(const GemmArgs& g, long i0, long j0, long k0, long k1, T* Cb, long ldc,
+                                          long ci0, long cj0, bool direct, T (*As)[BK][BM + 1],
+                                          T (*Bs)[BK][BN + 1]) {
   using acc_t = typename Mfma<T>::acc_t;
-  __shared__ T As[2][BK][BM + 1];
-  __shared__ T Bs[2][BK][BN + 1];
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
-  long tile_m, tile_n, split;
-  tile_of(g, tile_m, tile_n, split);
-  const long i0 = tile_m * BM, j0 = tile_n * BN;
-  long k0, k1;
-  k_range(g, i0, j0, k0, k1);
-  if (g.kchunk > 0) {
-    const long ks = split * g.kchunk;
-    k0 = max(k0, ks);
-    k1 = min(k1, ks + g.kchunk);
-  }
   const Stager<T, TA, TB> st(g, i0, j0, tid);
   // interior tile: no row or column bounds, chunks aligned to BK. A whole chunk of it then loads
   // without checks unless it crosses the diagonal of a triangular operand (the k range already
@@ -225,9 +224,7 @@ __global__ __launch_bounds__(NT, 2) void k_gemm(GemmArgs g) {
     }
   }
   // epilogue: operands were supplied as A[lane%16][lane/16], B[lane/16][lane%16]
-  T* C = g.kchunk > 0 ? (T*)g.part + (size_t)split * g.M * g.N : (T*)g.C;
-  const long ldc = g.kchunk > 0 ? g.N : g.ldc;
-  const bool direct = g.kchunk == 0;
+  T* C = Cb;
   const T alpha = (T)g.alpha, beta = (T)g.beta;
 #pragma unroll
   for (int a = 0; a < 4; a++)
@@ -239,15 +236,99 @@ __global__ __launch_bounds__(NT, 2) void k_gemm(GemmArgs g) {
         const long gj = j0 + wn + b * 16 + li;
         if (gi < g.M && gj < g.N) {
           const T val = acc[a][b][v];
+          T* cp = C + (gi - ci0) * ldc + (gj - cj0);
           if (direct) {
             T out = alpha * val;
-            if (g.beta != 0.0) out += beta * C[gi * ldc + gj];
-            C[gi * ldc + gj] = out;
+            if (g.beta != 0.0) out += beta * *cp;
+            *cp = out;
           } else {
-            C[gi * ldc + gj] = val;
+            *cp = val;
           }
         }
       }
+}
+
+template <typename T, int TA, int TB>
+__global__ __launch_bounds__(NT, 2) void k_gemm(GemmArgs g) {
+  __shared__ T As[2][BK][BM + 1];
+  __shared__ T Bs[2][BK][BN + 1];
+  long tile_m, tile_n, split;
+  tile_of(g, tile_m, tile_n, split);
+  const long i0 = tile_m * BM, j0 = tile_n * BN;
+  long k0, k1;
+  k_range(g, i0, j0, k0, k1);
+  if (g.kchunk > 0) {
+    const long ks = split * g.kchunk;
+    k0 = max(k0, ks);
+    k1 = min(k1, ks + g.kchunk);
+  }
+  T* Cb = g.kchunk > 0 ? (T*)g.part + (size_t)split * g.M * g.N : (T*)g.C;
+  const long ldc = g.kchunk > 0 ? g.N : g.ldc;
+  gemm_tile<T, TA, TB>(g, i0, j0, k0, k1, Cb, ldc, 0, 0, g.kchunk == 0, As, Bs);
+}
+
+// Grouped products: many independent problems of one transposition pattern in one launch. The
+// host lays out work items (problem, tile, K range, partial slot); a persistent grid strides over
+// them, each item running the same tile code as k_gemm. Items of a tile split over K (or of a
+// mirrored symmetric product) store partial tiles; k_grouped_reduce then sums each tile's
+// partials in slot order (deterministic), applies alpha and beta, and mirrors lower tiles.
+struct GroupProb {
+  const void* A;
+  const void* B;
+  void* C;
+  long lda, ldb, ldc, M, N, K;
+  int tri, lower_only, mirror, pad;
+};
+struct GroupItem {
+  int p, tile_m, tile_n, slot;  // slot < 0: the item owns its whole tile and stores it directly
+  long k0, k1;
+};
+struct GroupRed {
+  int p, tile_m, tile_n, first_slot, nslots, pad;
+};
+
+template <typename T, int TA, int TB>
+__global__ __launch_bounds__(NT, 2) void k_gemm_grouped(const GroupProb* __restrict__ probs,
+                                                        const GroupItem* __restrict__ items, long nitems,
+                                                        T* __restrict__ part, double alpha, double beta) {
+  __shared__ T As[2][BK][BM + 1];
+  __shared__ T Bs[2][BK][BN + 1];
+  for (long it = blockIdx.x; it < nitems; it += gridDim.x) {
+    const GroupItem w = items[it];
+    const GroupProb pr = probs[w.p];
+    GemmArgs g{};
+    g.A = pr.A; g.lda = pr.lda; g.B = pr.B; g.ldb = pr.ldb; g.M = pr.M; g.N = pr.N; g.K = pr.K;
+    g.tri = pr.tri; g.alpha = alpha; g.beta = beta;
+    const long i0 = (long)w.tile_m * BM, j0 = (long)w.tile_n * BN;
+    long k0, k1;
+    k_range(g, i0, j0, k0, k1);
+    k0 = max(k0, w.k0);
+    k1 = min(k1, w.k1);
+    if (w.slot < 0) gemm_tile<T, TA, TB>(g, i0, j0, k0, k1, (T*)pr.C, pr.ldc, 0, 0, true, As, Bs);
+    else gemm_tile<T, TA, TB>(g, i0, j0, k0, k1, part + (size_t)w.slot * BM * BN, BN, i0, j0, false, As, Bs);
+    __syncthreads();  // the next item restages the LDS tiles
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_grouped_reduce(const GroupProb* __restrict__ probs,
+                                                        const GroupRed* __restrict__ reds, const T* __restrict__ part,
+                                                        double alpha, double beta) {
+  const GroupRed r = reds[blockIdx.x];
+  const GroupProb pr = probs[r.p];
+  T* C = (T*)pr.C;
+  for (int e = threadIdx.x; e < BM * BN; e += blockDim.x) {
+    const int li = e / BN, lj = e - li * BN;
+    const long gi = (long)r.tile_m * BM + li, gj = (long)r.tile_n * BN + lj;
+    if (gi >= pr.M || gj >= pr.N) continue;
+    if (pr.lower_only && gj > gi) continue;  // the upper part of a diagonal tile comes by mirroring
+    T s = T(0);
+    for (int q = 0; q < r.nslots; q++) s += part[(size_t)(r.first_slot + q) * BM * BN + e];
+    T out = (T)alpha * s;
+    if (beta != 0.0) out += (T)beta * C[gi * pr.ldc + gj];
+    C[gi * pr.ldc + gj] = out;
+    if (pr.mirror && gj < gi) C[gj * pr.ldc + gi] = out;
+  }
 }
 
 // Sums split-K partials in split order, applies alpha/beta, and mirrors lower tiles when asked.
@@ -361,7 +442,195 @@ int gemm_impl(int ta, int tb, long M, long N, long K, double alpha, const T* A, 
   return EXC_OK;
 }
 
+// ---- grouped products: tables, launch and the autotuner
+struct GroupPlan {
+  std::vector<GroupProb> probs;
+  std::vector<GroupItem> items;
+  std::vector<GroupRed> reds;
+  long slots = 0;
+  double flops = 0.0;
+};
+
+GroupPlan group_plan(int nprob, const ExcGemmProblem* P, long kchunk) {
+  GroupPlan g;
+  if (kchunk > 0) kchunk = ((kchunk + BK - 1) / BK) * BK;
+  for (int p = 0; p < nprob; p++) {
+    const ExcGemmProblem& q = P[p];
+    GroupProb gp{q.A, q.B, q.C, q.lda, q.ldb, q.ldc, q.M, q.N, q.K, q.tri, q.lower_only, q.mirror, 0};
+    g.probs.push_back(gp);
+    if (q.M <= 0 || q.N <= 0) continue;
+    double f = 2.0 * (double)q.M * q.N * q.K;
+    if (q.tri != EXC_TRI_NONE) f *= 0.5;
+    if (q.lower_only) f *= 0.5;
+    g.flops += f;
+    const long tm = (q.M + BM - 1) / BM, tn = (q.N + BN - 1) / BN;
+    const long splits = (kchunk > 0 && q.K > kchunk) ? (q.K + kchunk - 1) / kchunk : 1;
+    for (long b = 0; b < tn; b++)
+      for (long a = 0; a < tm; a++) {
+        if (q.lower_only && a < b) continue;
+        if (splits == 1 && !q.mirror) {
+          g.items.push_back(GroupItem{p, (int)a, (int)b, -1, 0, q.K});
+          continue;
+        }
+        const long first = g.slots;
+        for (long sp = 0; sp < splits; sp++) {
+          const long k0 = splits == 1 ? 0 : sp * kchunk;
+          const long k1 = splits == 1 ? q.K : std::min(q.K, (sp + 1) * kchunk);
+          g.items.push_back(GroupItem{p, (int)a, (int)b, (int)(g.slots++), k0, k1});
+        }
+        g.reds.push_back(GroupRed{p, (int)a, (int)b, (int)first, (int)splits, 0});
+      }
+  }
+  return g;
+}
+
+size_t group_bytes(const GroupPlan& g, size_t tsize) {
+  auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+  return up(g.probs.size() * sizeof(GroupProb)) + up(g.items.size() * sizeof(GroupItem)) +
+         up(std::max<size_t>(1, g.reds.size()) * sizeof(GroupRed)) + (size_t)g.slots * BM * BN * tsize;
+}
+
+template <typename T>
+int group_run(int ta, int tb, const GroupPlan& g, double alpha, double beta, int grid, void* work, size_t wb,
+              hipStream_t s) {
+  EXC_REQUIRE(group_bytes(g, sizeof(T)) <= wb, EXC_ERR_SPACE, "exc_gemm_grouped: workspace too small");
+  if (g.items.empty()) return EXC_OK;
+  auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+  char* w = (char*)work;
+  GroupProb* dp = (GroupProb*)w;
+  w += up(g.probs.size() * sizeof(GroupProb));
+  GroupItem* di = (GroupItem*)w;
+  w += up(g.items.size() * sizeof(GroupItem));
+  GroupRed* dr = (GroupRed*)w;
+  w += up(std::max<size_t>(1, g.reds.size()) * sizeof(GroupRed));
+  T* part = (T*)w;
+  // the tables are small; copy them synchronously so no host buffer must outlive the call
+  EXC_CHECK_HIP(hipMemcpyAsync(dp, g.probs.data(), g.probs.size() * sizeof(GroupProb), hipMemcpyHostToDevice, s));
+  EXC_CHECK_HIP(hipMemcpyAsync(di, g.items.data(), g.items.size() * sizeof(GroupItem), hipMemcpyHostToDevice, s));
+  if (!g.reds.empty())
+    EXC_CHECK_HIP(hipMemcpyAsync(dr, g.reds.data(), g.reds.size() * sizeof(GroupRed), hipMemcpyHostToDevice, s));
+  EXC_CHECK_HIP(hipStreamSynchronize(s));
+  const long nitems = (long)g.items.size();
+  if (grid <= 0) grid = (int)std::min<long>(nitems, resident_slots(sizeof(T) == 8 ? 2 : 3));
+  grid = (int)std::min<long>(grid, nitems);
+  hipEvent_t e0 = exc_prof::mark(s);
+  if (!ta && !tb) hipLaunchKernelGGL((k_gemm_grouped<T, 0, 0>), dim3(grid), dim3(NT), 0, s, dp, di, nitems, part, alpha, beta);
+  else if (!ta && tb) hipLaunchKernelGGL((k_gemm_grouped<T, 0, 1>), dim3(grid), dim3(NT), 0, s, dp, di, nitems, part, alpha, beta);
+  else if (ta && !tb) hipLaunchKernelGGL((k_gemm_grouped<T, 1, 0>), dim3(grid), dim3(NT), 0, s, dp, di, nitems, part, alpha, beta);
+  else hipLaunchKernelGGL((k_gemm_grouped<T, 1, 1>), dim3(grid), dim3(NT), 0, s, dp, di, nitems, part, alpha, beta);
+  EXC_CHECK_HIP(hipGetLastError());
+  if (!g.reds.empty()) {
+    hipLaunchKernelGGL(k_grouped_reduce<T>, dim3((unsigned)g.reds.size()), dim3(256), 0, s, dp, dr, part, alpha,
+                       beta);
+    EXC_CHECK_HIP(hipGetLastError());
+  }
+  if (e0) exc_prof::book(sizeof(T) == 8 ? "gemm_grouped_f64" : "gemm_grouped_f32", e0, exc_prof::mark(s), g.flops, 0.0);
+  return EXC_OK;
+}
+
+// Tuning results per problem set: the K chunk and the persistent grid that ran fastest.
+struct TuneEntry {
+  long key[8];
+  long kchunk;
+  int grid;
+  double ms;
+};
+std::vector<TuneEntry> g_tune;
+std::mutex g_tune_mu;
+
+template <typename T>
+int grouped_impl(int ta, int tb, int nprob, const ExcGemmProblem* P, double alpha, double beta, long kchunk, int grid,
+                 void* work, size_t wb, hipStream_t s) {
+  EXC_REQUIRE(nprob >= 0 && (nprob == 0 || P), EXC_ERR_ARG, "exc_gemm_grouped: bad arguments");
+  if (kchunk >= 0) return group_run<T>(ta, tb, group_plan(nprob, P, kchunk), alpha, beta, grid, work, wb, s);
+  // autotune: one timed run per candidate on the caller's data (each run overwrites the outputs
+  // completely, so the last one leaves the result), the fastest kept per problem-set key
+  long key[8] = {(long)sizeof(T), ta, tb, nprob, 0, 0, 0, 0};
+  for (int p = 0; p < nprob; p++) {
+    key[4] += P[p].K;
+    key[5] = std::max(key[5], P[p].K);
+    key[6] = std::max(key[6], P[p].M * 100003 + P[p].N);
+    key[7] = key[7] * 31 + P[p].lower_only * 2 + P[p].tri * 7 + (P[p].beta_zero_hint ? 1 : 0);
+  }
+  {
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    for (const TuneEntry& e : g_tune)
+      if (std::equal(e.key, e.key + 8, key))
+        return group_run<T>(ta, tb, group_plan(nprob, P, e.kchunk), alpha, beta, e.grid, work, wb, s);
+  }
+  EXC_REQUIRE(beta == 0.0, EXC_ERR_ARG, "exc_gemm_grouped: tuning runs need beta = 0 (each run overwrites C)");
+  const long slots = resident_slots(sizeof(T) == 8 ? 2 : 3);
+  const long chunks[] = {0, 16384, 8192, 4096, 2048};
+  const int mults[] = {1, 2};
+  hipEvent_t a, b;
+  EXC_CHECK_HIP(hipEventCreate(&a));
+  EXC_CHECK_HIP(hipEventCreate(&b));
+  TuneEntry best{};
+  best.ms = -1.0;
+  for (long kc : chunks) {
+    const GroupPlan g = group_plan(nprob, P, kc);
+    if (group_bytes(g, sizeof(T)) > wb) continue;
+    for (int mu : mults) {
+      const int gr = (int)std::min<long>((long)g.items.size(), slots * mu);
+      int rc = group_run<T>(ta, tb, g, alpha, beta, gr, work, wb, s);  // warm (and tables uploaded)
+      if (rc) return rc;
+      EXC_CHECK_HIP(hipEventRecord(a, s));
+      rc = group_run<T>(ta, tb, g, alpha, beta, gr, work, wb, s);
+      if (rc) return rc;
+      EXC_CHECK_HIP(hipEventRecord(b, s));
+      EXC_CHECK_HIP(hipEventSynchronize(b));
+      float ms = 0.f;
+      EXC_CHECK_HIP(hipEventElapsedTime(&ms, a, b));
+      if (best.ms < 0 || ms < best.ms) {
+        best.ms = ms;
+        best.kchunk = kc;
+        best.grid = gr;
+      }
+    }
+  }
+  hipEventDestroy(a);
+  hipEventDestroy(b);
+  EXC_REQUIRE(best.ms >= 0, EXC_ERR_SPACE, "exc_gemm_grouped: workspace too small for any tuning candidate");
+  std::copy(key, key + 8, best.key);
+  {
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    g_tune.push_back(best);
+  }
+  return group_run<T>(ta, tb, group_plan(nprob, P, best.kchunk), alpha, beta, best.grid, work, wb, s);
+}
+
 }  // namespace
+
+int exc_gemm_grouped_opts(int dtype, int ta, int tb, int nprob, const ExcGemmProblem* probs, double alpha, double beta,
+                          long kchunk, int grid, void* work, size_t work_bytes, hipStream_t s) {
+  EXC_REQUIRE(dtype == 4 || dtype == 8, EXC_ERR_ARG, "exc_gemm_grouped: dtype is 4 (float) or 8 (double)");
+  if (dtype == 8) return grouped_impl<double>(ta, tb, nprob, probs, alpha, beta, kchunk, grid, work, work_bytes, s);
+  return grouped_impl<float>(ta, tb, nprob, probs, alpha, beta, kchunk, grid, work, work_bytes, s);
+}
+
+size_t exc_gemm_grouped_workspace(int dtype, int nprob, const ExcGemmProblem* probs, long kchunk) {
+  // the autotuner's smallest chunk needs the most partial slots
+  const long kc = kchunk < 0 ? 2048 : kchunk;
+  return group_bytes(group_plan(nprob, probs, kc), dtype == 8 ? 8 : 4) + 256;
+}
+
+extern "C" int exc_gemm_grouped_tuning(char* buf, size_t cap) {
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  std::string out = "[";
+  for (size_t i = 0; i < g_tune.size(); i++) {
+    char line[256];
+    snprintf(line, sizeof line, "%s{\"dtype\": %ld, \"problems\": %ld, \"sum_k\": %ld, \"kchunk\": %ld, \"grid\": %d, \"ms\": %.4f}",
+             i ? ", " : "", g_tune[i].key[0], g_tune[i].key[3], g_tune[i].key[4], g_tune[i].kchunk, g_tune[i].grid,
+             g_tune[i].ms);
+    out += line;
+  }
+  out += "]";
+  if (buf && cap) {
+    strncpy(buf, out.c_str(), cap - 1);
+    buf[cap - 1] = 0;
+  }
+  return EXC_OK;
+}
 
 int exc_gemm_opts(int ta, int tb, long M, long N, long K, double alpha, const double* A, long lda, const double* B,
                   long ldb, double beta, double* C, long ldc, const ExcGemmOpts& o, hipStream_t s) {

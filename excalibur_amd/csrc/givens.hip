@@ -8,6 +8,8 @@
 // is applied on its own after (mask) or before (unmask) the others, which is the order the
 // reference's sequential loop gives it. Each update is c*x1 + s'*x2 with separately rounded
 // products and sum, so the result is bit-identical to the reference's loop.
+#include <algorithm>
+
 #include "exc_common.hpp"
 
 #pragma clang fp contract(off)
@@ -25,12 +27,9 @@ __device__ __forceinline__ void rot(T* p1, T* p2, T c, T ss, T nss) {
 
 // odd_mode: 0 none, 1 the special pair is pair 0 (applied first), 2 it is the last pair.
 template <typename T, bool LEFT>
-__global__ __launch_bounds__(kThreads) void k_givens(T* __restrict__ x, long m, long n, int W,
-                                                     const int32_t* __restrict__ pairs,
-                                                     const double* __restrict__ cs, int passes, int half, int mask,
-                                                     int odd_mode) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  T* buf = reinterpret_cast<T*>(smem_raw);
+__device__ __forceinline__ void givens_strip(T* __restrict__ x, long m, long n, int W,
+                                             const int32_t* __restrict__ pairs, const double* __restrict__ cs,
+                                             int passes, int half, int mask, int odd_mode, T* buf) {
   const int tid = threadIdx.x;
   const long K = LEFT ? m : n;
   const long base = (long)blockIdx.x * W;
@@ -97,6 +96,27 @@ __global__ __launch_bounds__(kThreads) void k_givens(T* __restrict__ x, long m, 
   }
 }
 
+template <typename T, bool LEFT>
+__global__ __launch_bounds__(kThreads) void k_givens(T* __restrict__ x, long m, long n, int W,
+                                                     const int32_t* __restrict__ pairs,
+                                                     const double* __restrict__ cs, int passes, int half, int mask,
+                                                     int odd_mode) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  givens_strip<T, LEFT>(x, m, n, W, pairs, cs, passes, half, mask, odd_mode, reinterpret_cast<T*>(smem_raw));
+}
+
+// The same sweep over a batch of matrices of one shape, each with its own plan: blockIdx.y picks
+// the matrix.
+template <typename T, bool LEFT>
+__global__ __launch_bounds__(kThreads) void k_givens_batched(T* const* __restrict__ xs, long m, long n, int W,
+                                                             const int32_t* const* __restrict__ pairs,
+                                                             const double* const* __restrict__ cs, int passes,
+                                                             int half, int mask, int odd_mode) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int b = blockIdx.y;
+  givens_strip<T, LEFT>(xs[b], m, n, W, pairs[b], cs[b], passes, half, mask, odd_mode, reinterpret_cast<T*>(smem_raw));
+}
+
 constexpr size_t kLdsSoft = 64 * 1024;   // two workgroups per CU
 constexpr size_t kLdsHard = 160 * 1024;  // one workgroup per CU
 
@@ -144,9 +164,43 @@ int givens_launch(T* x, long m, long n, const int32_t* pairs, const double* cs, 
   return EXC_OK;
 }
 
+template <typename T>
+int givens_batched_launch(int nb, T* const* d_xs, long m, long n, const int32_t* const* d_pairs,
+                          const double* const* d_cs, long passes, int mask, int is_left, hipStream_t s) {
+  EXC_REQUIRE(nb >= 0 && m >= 1 && n >= 1 && passes >= 0 && (nb == 0 || (d_xs && d_pairs && d_cs)), EXC_ERR_ARG,
+              "exc_givens_apply_batched: bad arguments");
+  const long K = is_left ? m : n;
+  const long lines = is_left ? n : m;
+  if (nb == 0 || K <= 1 || passes == 0) return EXC_OK;
+  const int half = (int)((K % 2 == 0 ? K : K + 1) / 2);
+  const int odd_mode = (K % 2 == 1) ? (mask ? 2 : 1) : 0;
+  const size_t line_bytes = (size_t)K * sizeof(T);
+  EXC_REQUIRE(line_bytes <= kLdsHard, EXC_ERR_ARG, "exc_givens_apply_batched: K too large");
+  long w_lds = (long)(kLdsSoft / line_bytes);
+  if (w_lds < 1) w_lds = 1;
+  // the batch already spreads the work: give each workgroup as many lines as fit in LDS
+  long w = std::max<long>(1, std::min<long>({w_lds, 64L, std::max<long>(1, lines * nb / 512)}));
+  const size_t lds = (size_t)w * line_bytes;
+  const dim3 grid(exc_detail::ceil_div(lines, w), nb);
+  if (is_left)
+    hipLaunchKernelGGL((k_givens_batched<T, true>), grid, dim3(kThreads), lds, s, d_xs, m, n, (int)w, d_pairs, d_cs,
+                       (int)passes, half, mask, odd_mode);
+  else
+    hipLaunchKernelGGL((k_givens_batched<T, false>), grid, dim3(kThreads), lds, s, d_xs, m, n, (int)w, d_pairs, d_cs,
+                       (int)passes, half, mask, odd_mode);
+  EXC_CHECK_HIP(hipGetLastError());
+  return EXC_OK;
+}
+
 }  // namespace
 
 extern "C" {
+
+int exc_givens_apply_batched(int nb, double* const* d_xs, long m, long n, const int32_t* const* d_pairs,
+                             const double* const* d_cs, long passes, int mask, int is_left, exc_stream_t stream) {
+  return givens_batched_launch<double>(nb, d_xs, m, n, d_pairs, d_cs, passes, mask, is_left,
+                                       exc_detail::stream(stream));
+}
 
 int exc_givens_apply(double* d_x, long m, long n, const int32_t* d_pairs, const double* d_cs, long passes, int mask,
                      int is_left, exc_stream_t stream) {

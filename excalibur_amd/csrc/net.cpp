@@ -317,6 +317,25 @@ void PeerNet::ring_all_reduce_sum(double* buf, long n) {
   comm_s_ += now_seconds() - t0;
 }
 
+void PeerNet::all_reduce_sum_same(double* buf, long n) {
+  if (n >= (long)P_ * 10000 || P_ == 1) {
+    ring_all_reduce_sum(buf, n);
+    return;
+  }
+  const double t0 = now_seconds();
+  broadcast(buf, n * sizeof(double));
+  std::vector<std::vector<char>> got(P_);
+  for (int j = 0; j < P_; j++)
+    if (j != cid_) got[j] = receive(j);
+  std::vector<double> own(buf, buf + n);
+  for (long q = 0; q < n; q++) {
+    double acc = 0.0;
+    for (int j = 0; j < P_; j++) acc = acc + (j == cid_ ? own[q] : ((const double*)got[j].data())[q]);
+    buf[q] = acc;
+  }
+  comm_s_ += now_seconds() - t0;
+}
+
 void PeerNet::all_gather(std::vector<char>& buf, size_t bytes, const std::vector<int>& grp, int src) {
   const double t0 = now_seconds();
   const int nc = (int)grp.size();

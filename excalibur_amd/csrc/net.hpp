@@ -38,6 +38,10 @@ class PeerNet {
   // client.cpp:313-356: below P*10000 values a broadcast and a sum of the others' buffers in
   // peer order added to this peer's own; above, reduce-scatter then all-gather on the ring.
   void ring_all_reduce_sum(double* buf, long n);
+  // The same sum with bit-identical results on every peer: below P*10000 values every peer adds
+  // all buffers in peer order 0..P-1; above, the ring (whose chunks are reduced once and then
+  // copied) as ring_all_reduce_sum.
+  void all_reduce_sum_same(double* buf, long n);
   // client.cpp:358-411: peer `src` of group `grp` distributes `bytes` bytes (buf is filled on
   // the receivers).
   void all_gather(std::vector<char>& buf, size_t bytes, const std::vector<int>& grp, int src);

@@ -7,6 +7,7 @@
 // reflector from the reduced message in a fixed order, so all workgroups see the same h and tau)
 // and then writes its partial sums of step i+1's message; a second kernel adds the partials in
 // workgroup order. The host all-reduces that message over the peers between steps.
+#include <algorithm>
 #include <atomic>
 #include <cfloat>
 #include <chrono>
@@ -1053,8 +1054,9 @@ const void* loop_kernel_for(const LoopShape& sh) {
   return nullptr;
 }
 
-// True when twice the grid fits on the device at once (this peer's loop and another's).
-int loop_resident(const LoopShape& sh, const void* fn, bool* fits) {
+// True when `need` copies of the grid fit on the device at once: two for peers that share the
+// device (this peer's loop and another's), one for a loop that waits on no other grid.
+int loop_resident(const LoopShape& sh, const void* fn, bool* fits, int need = 2) {
   *fits = false;
   if (!fn || sh.nwg < 1 || sh.nwg > kLoopMaxWg) return EXC_OK;
   int dev = 0, cus = 0, per_cu = 0;
@@ -1063,7 +1065,7 @@ int loop_resident(const LoopShape& sh, const void* fn, bool* fits) {
   if (sh.lds > 64 * 1024)
     EXC_CHECK_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh.lds));
   EXC_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, sh.lds));
-  *fits = (long)per_cu * cus >= 2L * sh.nwg;
+  *fits = (long)per_cu * cus >= (long)need * sh.nwg;
   return EXC_OK;
 }
 
@@ -1142,7 +1144,7 @@ int loop_start(int m, int c, double* d_fc, double* d_uc, double* d_house, double
   const LoopShape sh = loop_shape(m, c, cols_per_wave);
   const void* fn = loop_kernel_for(sh);
   bool fits = false;
-  const int rc = loop_resident(sh, fn, &fits);
+  const int rc = loop_resident(sh, fn, &fits, B.P == 1 ? 1 : 2);
   if (rc) return rc;
   EXC_REQUIRE(fits, EXC_ERR_ARG, "exc_ralha_loop_start: the grid would not be resident (see exc_ralha_loop_fits)");
   if (B.P > 0)
@@ -1341,6 +1343,88 @@ int exc_rot_back(int m, int k, double* d_bu, const double* d_cs, exc_stream_t st
                      d_bu, d_cs);
   EXC_CHECK_HIP(hipGetLastError());
   return EXC_OK;
+}
+
+}  // extern "C"
+
+namespace {
+// Workspace layout of exc_bidiag_local: [scalars 256 B][box][loop or step workspace][2 messages].
+struct LocalWs {
+  double* scal;
+  void* box;
+  char* work;
+  size_t work_bytes;
+  double* msg[2];
+};
+size_t local_ws(int m, int c, void* base, LocalWs* w) {
+  const size_t box = (exc_ralha_box_bytes(m) + 255) / 256 * 256;
+  const size_t lw = std::max(exc_ralha_loop_workspace(m, c), exc_ralha_workspace(m, c));
+  const size_t lwp = (lw + 255) / 256 * 256;
+  const size_t msg = ((size_t)(m + 3) * sizeof(double) + 255) / 256 * 256;
+  if (w) {
+    char* p = (char*)base;
+    w->scal = (double*)p;
+    w->box = p + 256;
+    w->work = p + 256 + box;
+    w->work_bytes = lwp;
+    w->msg[0] = (double*)(p + 256 + box + lwp);
+    w->msg[1] = (double*)(p + 256 + box + lwp + msg);
+  }
+  return 256 + box + lwp + 2 * msg;
+}
+}  // namespace
+
+extern "C" {
+
+size_t exc_bidiag_local_workspace(int m, int c) { return m < 3 || c < 1 ? 0 : local_ws(m, c, nullptr, nullptr); }
+
+int exc_bidiag_local(int m, int c, double* d_fc, double* d_uc, double* d_house, double* d_tau, double* d_alpha,
+                     double* d_beta, double* d_cs, void* h_link, void* d_work, size_t work_bytes, double timeout_s,
+                     exc_stream_t stream) {
+  EXC_REQUIRE(m >= 3 && c >= 1 && d_fc && d_uc && d_house && d_tau && d_alpha && d_beta && d_cs && d_work &&
+                  timeout_s > 0,
+              EXC_ERR_ARG, "exc_bidiag_local: bad arguments");
+  EXC_REQUIRE(work_bytes >= exc_bidiag_local_workspace(m, c), EXC_ERR_SPACE, "exc_bidiag_local: workspace too small");
+  LocalWs w;
+  local_ws(m, c, d_work, &w);
+  hipStream_t s = exc_detail::stream(stream);
+  // the first column (client.cpp:830-833): the all-reduce of one owner is the value itself
+  int rc = exc_ralha_tail_update(-1, m, c, d_fc, d_uc, nullptr, d_beta, w.scal, stream);
+  if (!rc) rc = exc_ralha_tail_scale(0, m, c, d_fc, d_uc, w.scal, d_alpha, stream);
+  if (rc) return rc;
+  const LoopShape sh = loop_shape(m, c, 0);
+  const void* fn = loop_kernel_for(sh);
+  bool fits = false;
+  if (h_link && fn) {
+    rc = loop_resident(sh, fn, &fits, 1);
+    if (rc) return rc;
+  }
+  if (fits) {
+    rc = exc_ralha_box_clear(w.box, stream);
+    if (rc) return rc;
+    void* boxes[1] = {w.box};
+    int nwg[1] = {sh.nwg};
+    rc = exc_ralha_loop_start_boxes(m, c, d_fc, d_uc, d_house, d_tau, d_alpha, d_beta, h_link, w.work, w.work_bytes,
+                                    0, timeout_s, 1, 0, boxes, nwg, stream);
+    if (!rc) rc = exc_ralha_loop_finish(h_link, stream);
+    if (rc) return rc;
+  } else {
+    // one launch per step, back to back on the stream: step i reads message i and writes i + 1
+    rc = exc_ralha_step(-1, m, c, d_fc, d_uc, nullptr, d_house, d_tau, d_alpha, d_beta, w.msg[0], w.work,
+                        w.work_bytes, stream);
+    for (int i = 0; !rc && i + 2 < m; i++)
+      rc = exc_ralha_step(i, m, c, d_fc, d_uc, w.msg[i & 1], d_house, d_tau, d_alpha, d_beta, w.msg[(i + 1) & 1],
+                          w.work, w.work_bytes, stream);
+    if (rc) return rc;
+  }
+  // the last two columns (client.cpp:897-905)
+  for (int i = m - 2; i < m; i++) {
+    rc = exc_ralha_dot(i, m, c, d_fc, d_uc, w.scal, stream);
+    if (!rc) rc = exc_ralha_tail_update(i, m, c, d_fc, d_uc, w.scal, d_beta, w.scal + 1, stream);
+    if (!rc) rc = exc_ralha_tail_scale(i, m, c, d_fc, d_uc, w.scal + 1, d_alpha, stream);
+    if (rc) return rc;
+  }
+  return exc_lower_to_upper(m, d_alpha, d_beta, d_cs, stream);
 }
 
 }  // extern "C"

@@ -104,6 +104,37 @@ SIGNATURES = {
     "exc_reflector_message_len": (_L, [_L, _L, _L]),
     "exc_pack_reflectors": (_I, [_L, _L, _L, _dp, _L, _dp, _dp, _vp]),
     "exc_unpack_reflectors": (_I, [_L, _L, _L, _dp, _dp, _L, _dp, _vp]),
+    "exc_bidiag_local_workspace": (_S, [_I, _I]),
+    "exc_bidiag_local": (_I, [_I, _I, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _vp, _dp, _S, _D, _vp]),
+    "exc_convert_f64_f32": (_I, [_L, _L, _dp, _L, _dp, _L, _vp]),
+    "exc_convert_f32_f64": (_I, [_L, _L, _dp, _L, _dp, _L, _vp]),
+    "exc_trace": (_I, [_I, _dp, _L, _dp, _vp]),
+    "exc_add_diag": (_I, [_I, _dp, _L, _D, _vp]),
+    "exc_dev_identity": (_I, [_I, _dp, _L, _dp, _vp]),
+    "exc_oa_refine_workspace": (_S, [_I]),
+    "exc_oa_refine": (_I, [_I, _dp, _L, _I, _dp, _L, _dp, _L, _dp, _I, _D, _D, _dp, _S, _vp]),
+    "exc_givens_apply_batched": (_I, [_I, _vp, _L, _L, _vp, _vp, _L, _I, _I, _vp]),
+    "exc_block_shift_mask_workspace": (_S, [_L, _L, _L, _I]),
+    "exc_block_shift_mask": (_I, [_dp, _L, _L, _L, _I, _I, _U, _vp, _S, _vp]),
+    "exc_batched_local_qr_workspace": (_S, [_I, _I, _vp]),
+    "exc_batched_local_qr": (_I, [_I, _I, _vp, _vp, _vp, _vp, _vp, _vp, _S, _vp]),
+    "exc_gemm_grouped_tuning": (_I, [ctypes.c_char_p, _S]),
+    "exc_gk_max_n": (_I, []),
+    "exc_gk_bidiagonals": (_I, [_I] + [_vp] * 6 + [_vp]),
+    "exc_bidiag_scale": (_I, [_I] + [_vp] * 6 + [_vp]),
+    "exc_count_features": (_I, [_I, _I] + [_vp] * 6 + [_vp]),
+    "exc_bisect_index": (_I, [_I] + [_vp] * 5 + [_D] +
+                         [_vp] * 9 + [_vp]),
+    "exc_mlp_param_count": (_L, []),
+    "exc_mlp_dims": (_I, [ctypes.POINTER(_I)] * 3),
+    "exc_mlp_acts_floats": (_S, [_L]),
+    "exc_mlp_backward_scratch_floats": (_S, [_L]),
+    "exc_mlp_forward": (_I, [_L] + [_vp] * 5 + [_S, _vp]),
+    "exc_mlp_backward": (_I, [_L] + [_vp] * 9 + [_S, _vp]),
+    "exc_mlp_adam": (_I, [_vp] * 4 + [ctypes.c_float] * 4 + [_I, _vp]),
+    "exc_mlp_to_bf16": (_I, [_vp] * 3),
+    "exc_mlp_infer_bf16": (_I, [_L] + [_vp] * 3 + [_vp]),
+
 }
 
 EXTRA_SIGNATURES = {}
@@ -574,3 +605,58 @@ def rot_back(bu, cs):
     call("exc_rot_back", m, k, du.ptr, dc.ptr, None)
     sync()
     return du.to_host()
+
+
+def oa_refine(L, U, W, iters=1, tau=1e-6, floor_rel=1e-4, lower=True):
+    """FP64 Ogita-Aishima refinement of L ~ U diag(s) W^T on the GPU (exc_oa_refine).
+    Returns (U, W, sigma)."""
+    L = np.ascontiguousarray(L, dtype=np.float64)
+    m = L.shape[0]
+    dL = DeviceArray.from_host(L)
+    dU = DeviceArray.from_host(U)
+    dW = DeviceArray.from_host(W)
+    ds = DeviceArray((m,)).zero()
+    ws = workspace(lib.exc_oa_refine_workspace(m))
+    call("exc_oa_refine", m, dL.ptr, m, 1 if lower else 0, dU.ptr, m, dW.ptr, m, ds.ptr, iters, tau, floor_rel,
+         ws.ptr, ws.nbytes, None)
+    sync()
+    return dU.to_host(), dW.to_host(), ds.to_host()
+
+
+def bidiag_local(F, one_launch=True, timeout_s=30.0):
+    """One-sided bidiagonalization of F (m x c, one owner) and lower to upper (exc_bidiag_local).
+    Returns dict(alpha, beta, house, tau, U (m x c), cs) in the oracle's layouts."""
+    F = np.ascontiguousarray(F, dtype=np.float64)
+    m, c = F.shape
+    dF = DeviceArray.from_host(np.ascontiguousarray(F.T))  # column-major storage of F
+    dU = DeviceArray((c, m)).zero()
+    dh = DeviceArray((m, m)).zero()
+    dt = DeviceArray((m,)).zero()
+    da = DeviceArray((m,)).zero()
+    db = DeviceArray((m,)).zero()
+    dcs = DeviceArray((2 * m,)).zero()
+    ws = workspace(lib.exc_bidiag_local_workspace(m, c))
+    link = ctypes.c_void_p()
+    if one_launch:
+        call("exc_ralha_link_alloc", m, ctypes.byref(link))
+    try:
+        call("exc_bidiag_local", m, c, dF.ptr, dU.ptr, dh.ptr, dt.ptr, da.ptr, db.ptr, dcs.ptr,
+             link if one_launch else None, ws.ptr, ws.nbytes, timeout_s, None)
+        sync()
+    finally:
+        if one_launch:
+            lib.exc_ralha_link_free(link)
+    return dict(alpha=da.to_host(), beta=db.to_host()[: m - 1], house=dh.to_host(), tau=dt.to_host(),
+                U=np.ascontiguousarray(dU.to_host().T), cs=dcs.to_host()[: 2 * (m - 1)])
+
+
+def block_shift_mask(x, passes, mask, is_left, seed):
+    """block_shift_mask (bbtsvd.cpp:533-669) on the GPU, any K (exc_block_shift_mask)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    m, n = x.shape
+    dx = DeviceArray.from_host(x)
+    ws = workspace(lib.exc_block_shift_mask_workspace(m, n, passes, 1 if is_left else 0) + 1)
+    call("exc_block_shift_mask", dx.ptr, m, n, passes, 1 if mask else 0, 1 if is_left else 0, seed & 0xFFFFFFFF,
+         ws.ptr, ws.nbytes, None)
+    sync()
+    return dx.to_host()

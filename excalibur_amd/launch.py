@@ -48,10 +48,19 @@ def write_peer_files(xs, dpath):
         np.ascontiguousarray(x, dtype=np.float64).tofile(os.path.join(dpath, f"Client{i}.mat"))
 
 
+def write_ground_truth(u, vt, dpath):
+    """u.mat (k x m, row i = u_i) and vt.mat (k x global_n, row i = v_i), the ground-truth files
+    generate_data.cpp writes for evaluation (generate_data.cpp:152-186)."""
+    os.makedirs(dpath, exist_ok=True)
+    np.ascontiguousarray(u, dtype=np.float64).tofile(os.path.join(dpath, "u.mat"))
+    np.ascontiguousarray(vt, dtype=np.float64).tofile(os.path.join(dpath, "vt.mat"))
+
+
 def run_peers(xs, seed=0, mode=1, topk=-1, opt=3, form="fp64", dump=True, workdir=None, timeout=600,
-              recv_timeout=300.0, extra=None):
+              recv_timeout=300.0, extra=None, evaluate=False, memmap=False, ground_truth=None):
     """Runs len(xs) peers on the column blocks xs (each m x local_n_i). Returns dict with per-peer
-    timers and, when dump, sigma / FinalU / FinalVT arrays."""
+    timers and, when dump, sigma / FinalU / FinalVT arrays. evaluate: the reference's evaluation
+    report (-e 1; ground_truth = (u, vt) is written next to the data); memmap: -f 1."""
     if not os.path.exists(PEER_PATH):
         raise RuntimeError(f"{PEER_PATH} is not built")
     P = len(xs)
@@ -62,12 +71,14 @@ def run_peers(xs, seed=0, mode=1, topk=-1, opt=3, form="fp64", dump=True, workdi
     logdir = os.path.join(workdir, "logs")
     os.makedirs(logdir, exist_ok=True)
     write_peer_files(xs, dpath)
+    if ground_truth is not None:
+        write_ground_truth(ground_truth[0], ground_truth[1], dpath)
     port = free_port(P + 1)
     procs = []
     for cid in range(P):
         cmd = [PEER_PATH, "-l", "127.0.0.1", "-p", str(port), "-m", str(m), "-n", str(xs[cid].shape[1]),
                "-d", dpath, "-g", logdir, "-k", str(topk), "-o", str(mode), "-t", str(P), "-c", str(cid),
-               "-e", "0", "-f", "0", "-s", str(seed), "-a", str(opt), "--form", form,
+               "-e", "1" if evaluate else "0", "-f", "1" if memmap else "0", "-s", str(seed), "-a", str(opt), "--form", form,
                "--dump", "1" if dump else "0", "--recv-timeout", str(recv_timeout)] + list(extra or [])
         out = open(os.path.join(logdir, f"peer{cid}.out"), "w")
         procs.append((subprocess.Popen(cmd, stdout=out, stderr=subprocess.STDOUT), out))
@@ -91,6 +102,9 @@ def run_peers(xs, seed=0, mode=1, topk=-1, opt=3, form="fp64", dump=True, workdi
             if line.startswith("EXC_RESULT "):
                 info = json.loads(line[len("EXC_RESULT "):])
         peer = {"log": text, "result": info}
+        for line in text.splitlines():
+            if line.startswith("EXC_EVAL "):
+                peer["eval"] = json.loads(line[len("EXC_EVAL "):])
         if dump and info is not None:
             k = info["sigma_count"]
             base = os.path.join(logdir, f"Client{cid}_")

@@ -88,6 +88,7 @@ class DecentralizedClient {
          time_combine_svd = 0, time_communication = 0, time_total = 0;
   double time_ralha_allreduce = 0;  // part of time_bi_diag spent in the per-column all-reduce
   double time_results_d2h = 0;      // copy of the results into the host members, after "Finish"
+  double time_refine = 0;           // mixed form: the FP64 refinement of the triplets
 
   void load_data();
   void remove_input_data();
@@ -118,6 +119,11 @@ class DecentralizedClient {
  private:
   char* receive_frame(int cid);
   void copy_results_to_host(int k, bool compute_u, bool compute_v, bool lqr);
+  void finish_run(int k, bool compute_u, bool compute_v, bool lqr, double t_run);
+  // The reference's evaluation report (client.cpp:1215-1309) when `evaluate` is set.
+  void evaluate_results(int k, bool compute_u, bool compute_v, bool lqr);
+  // The mixed-precision form of run() after the seeds; returns the number of triplets kept.
+  int run_mixed(bool compute_u, bool compute_v);
   std::vector<double> h_sigma_, h_FinalU_, h_FinalVT_, h_alpha_, h_beta_;
   void log(const std::string& line);
   // Once per client: whether all peers share this device (and each runs the one-launch

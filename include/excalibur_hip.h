@@ -114,6 +114,16 @@ int exc_givens_apply(double* d_x, long m, long n, const int32_t* d_pairs, const 
 int exc_givens_apply_f32(float* d_x, long m, long n, const int32_t* d_pairs, const double* d_cs, long passes,
                          int mask, int is_left, exc_stream_t stream);
 
+/* block_shift_mask (bbtsvd.cpp:533-669) on d_x (m x n row-major, in place), K = m (is_left) or n
+ * lines rotated. K <= 10000: the plain sweep of exc_givens_apply with the plan of (K, seed). K >
+ * 10000: the virtual-matrix mask, each line viewed as a vm x vn matrix plus a remainder
+ * (find_best_convert), with the reference's plans of seed + 1000 .. seed + 5000, bit-exact with
+ * the reference's loop. Draws the plans with the C library's generator (srand per plan), like the
+ * reference. Synchronizes the stream. */
+size_t exc_block_shift_mask_workspace(long m, long n, long passes, int is_left);
+int exc_block_shift_mask(double* d_x, long m, long n, long passes, int mask, int is_left, unsigned seed, void* d_work,
+                         size_t work_bytes, exc_stream_t stream);
+
 /* ---------------------------------------------------------------- dense kernels (MFMA) */
 
 /* C = alpha op(A) op(B) + beta C, row-major, op = transpose when trans_* != 0. FP64 on
@@ -285,6 +295,113 @@ int exc_rot_back(int m, int k, double* d_bu, const double* d_cs, exc_stream_t st
 /* d_out2 = {||alpha||^2 + ||beta||^2, 0}: the bisection root bound of client.cpp:966 and the
  * two shifts of its singular-value count. */
 int exc_bisection_bound(int n, const double* d_alpha, const double* d_beta, double* d_out2, exc_stream_t stream);
+
+/* One-sided bidiagonalization of an m x c matrix F that one caller holds whole (no peers): the
+ * steps -1 .. m-1 of exc_ralha_step with every all-reduce the identity (client.cpp:829-910 for
+ * one peer), then lower to upper (client.cpp:947-954) into d_alpha, d_beta, d_cs. The loop kernel
+ * runs in one launch when its grid alone fits on the device and h_link (exc_ralha_link_alloc) is
+ * given, else one launch per step back to back on the stream. Layouts as exc_ralha_step. The
+ * mixed-precision form uses it for the m x m factor every peer shares. */
+size_t exc_bidiag_local_workspace(int m, int c);
+int exc_bidiag_local(int m, int c, double* d_fc, double* d_uc, double* d_house, double* d_tau, double* d_alpha,
+                     double* d_beta, double* d_cs, void* h_link, void* d_work, size_t work_bytes, double timeout_s,
+                     exc_stream_t stream);
+
+/* ---------------------------------------------------------------- mixed precision (FP32 bulk) */
+
+/* Precision conversion of a rows x cols row-major block (leading dimensions in elements). */
+int exc_convert_f64_f32(long rows, long cols, const double* d_src, long lds, float* d_dst, long ldd,
+                        exc_stream_t stream);
+int exc_convert_f32_f64(long rows, long cols, const float* d_src, long lds, double* d_dst, long ldd,
+                        exc_stream_t stream);
+/* d_out[0] = trace(A); A += value * I; d_out[0] = max |A - I| and d_out[1] = ||A - I||_F
+ * (n x n, two doubles out). */
+int exc_trace(int n, const double* d_a, long lda, double* d_out, exc_stream_t stream);
+int exc_add_diag(int n, double* d_a, long lda, double value, exc_stream_t stream);
+int exc_dev_identity(int n, const double* d_a, long lda, double* d_out, exc_stream_t stream);
+/* FP64 Ogita-Aishima refinement of a square SVD L ~ U diag(sigma) W^T (all m x m row-major,
+ * L lower triangular when l_lower): `iters` steps of U <- U (I + F), W <- W (I + G) from the
+ * residual products I - U^T U, I - W^T W and U^T L W on the MFMA GEMM; sigma receives
+ * T_ii / (1 - (R_ii + S_ii) / 2) of the last step. Values within relative distance tau form
+ * clusters that are orthogonalized symmetrically, and the left vectors of cluster members above
+ * floor_rel * max sigma are recomputed as L w / sigma. Ogita and Aishima, J. Comput. Appl. Math.
+ * 369 (2020). No counterpart in the reference (its FP64 path has no refinement). */
+size_t exc_oa_refine_workspace(int m);
+int exc_oa_refine(int m, const double* d_l, long ldl, int l_lower, double* d_u, long ldu, double* d_w, long ldw,
+                  double* d_sigma, int iters, double tau, double floor_rel, void* d_work, size_t work_bytes,
+                  exc_stream_t stream);
+
+/* ---------------------------------------------------------------- batched peers */
+
+/* The same sweep as exc_givens_apply over nb matrices of one shape (m x n), each with its own
+ * plan (device arrays of nb pointers), in one launch. */
+int exc_givens_apply_batched(int nb, double* const* d_xs, long m, long n, const int32_t* const* d_pairs,
+                             const double* const* d_cs, long passes, int mask, int is_left, exc_stream_t stream);
+/* Local QR of nb blocks at once (client.cpp:572-589 for many peers on one device): for every
+ * block X_b (m x cols[b] row-major FP64, cols[b] >= m; host array of device pointers h_x) R_b^T
+ * (m x m lower, positive diagonal) into h_rt[b] and, when h_qt is given, Q_b^T (m x cols[b]) into
+ * h_qt[b]. A shifted CholeskyQR whose first pass runs in FP32 on the MFMA (all blocks' Gram
+ * matrices in one grouped persistent launch, its K chunk and grid chosen by the library's
+ * autotuner), refined by FP64 passes (grouped launches); the m x m factorizations run one
+ * workgroup per block in one launch. h_stats (nullable, 3 nb doubles): per block the FP32 shift,
+ * max |Z Z^T - I| before the last FP64 pass and the number of FP64 passes. Synchronizes. No
+ * counterpart in the reference, which runs one peer per process. */
+size_t exc_batched_local_qr_workspace(int nb, int m, const long* h_cols);
+int exc_batched_local_qr(int nb, int m, const long* h_cols, const double* const* h_x, double* const* h_rt,
+                         double* const* h_qt, double* h_stats, void* d_work, size_t work_bytes, exc_stream_t stream);
+/* The grouped GEMM autotuner's choices so far (JSON list of {dtype, problems, sum_k, kchunk, grid,
+ * ms}). */
+int exc_gemm_grouped_tuning(char* buf, size_t cap);
+
+/* ---------------------------------------------------------------- learned bisection brackets */
+
+/* nb bidiagonals stored back to back: bidiagonal b has n = d_n[b] values at offset d_off[b] of
+ * d_sigma / d_start / d_d / d_e (d_e holds n - 1 values there). exc_gk_bidiagonals runs the
+ * Golub-Kahan (Lanczos) bidiagonalization of diag(sigma) from the start vector (n <=
+ * exc_gk_max_n()): the bidiagonal of A = Q1 diag(sigma) Q2^T for a random orthogonal pair. */
+int exc_gk_max_n(void);
+int exc_gk_bidiagonals(int nb, const int* d_n, const long* d_off, const double* d_sigma, const double* d_start,
+                       double* d_d, double* d_e, exc_stream_t stream);
+/* d_root[b] = ||d||^2 + ||e||^2 as the reference's bisection root (bbtsvd.cpp:936), d_scale[b] =
+ * sqrt(sum d^2 + sum e^2). */
+int exc_bidiag_scale(int nb, const int* d_n, const long* d_off, const double* d_d, const double* d_e, double* d_root,
+                     double* d_scale, exc_stream_t stream);
+/* Network features: d_frac[b * K + k] = nc(scale_b * 10^(-k/2)) / n_b (exact Sturm counts). */
+int exc_count_features(int nb, int K, const int* d_n, const long* d_off, const double* d_d, const double* d_e,
+                       const double* d_scale, float* d_frac, exc_stream_t stream);
+/* One lane per target j: the leaf of the reference's bisection tree (bbtsvd.cpp:19-52, root
+ * [0, d_root[b]], tolerance tol) holding the d_ti[j]-th largest singular value of bidiagonal
+ * d_tb[j]. With brackets (d_blo and d_bhi both given) each end inside the root interval is checked
+ * by one exact Sturm count and becomes a lower or an upper bound of the value (an end the value
+ * does not lie above bounds it from above); the tree nodes whose midpoints lie beyond a bound are
+ * decided without a count. d_iters[j] = Sturm counts evaluated; d_status[j] = 1 bounded on both
+ * sides, 2 on one side, 0 from the root, -1 the root does not hold the value. Targets whose
+ * values share a leaf (closer than its width) get the same leaf; one value per distinct leaf, in
+ * order, is the reference's list. */
+int exc_bisect_index(int ntg, const int* d_n, const long* d_off, const double* d_d, const double* d_e,
+                     const double* d_root, double tol, const int* d_tb, const int* d_ti, const double* d_blo,
+                     const double* d_bhi, double* d_sigma, double* d_leaf_lo, double* d_leaf_hi, int* d_iters,
+                     int* d_status, exc_stream_t stream);
+/* The bracket network: in 32 -> 256 -> 256 -> 2, ReLU, parameters in one FP32 array of
+ * exc_mlp_param_count() floats (W1 32x256, b1, W2 256x256, b2, W3 256x2, b3; row-major, inputs
+ * by rows). Outputs (c, w): the target is c with Laplace scale exp(w). Training in FP32 on the
+ * MFMA GEMM: exc_mlp_forward keeps the activations (exc_mlp_acts_floats(B)), exc_mlp_backward
+ * writes the gradient of the mean Laplace negative log-likelihood and the loss, exc_mlp_adam
+ * updates. Inference in BF16: exc_mlp_to_bf16 converts the parameters (2 bytes each),
+ * exc_mlp_infer_bf16 runs a batch. No counterpart in the reference. */
+long exc_mlp_param_count(void);
+int exc_mlp_dims(int* d_in, int* hidden, int* d_out);
+size_t exc_mlp_acts_floats(long B);
+size_t exc_mlp_backward_scratch_floats(long B);
+int exc_mlp_forward(long B, const float* d_x, const float* d_p, float* d_acts, float* d_out, void* d_work,
+                    size_t work_bytes, exc_stream_t stream);
+int exc_mlp_backward(long B, const float* d_x, const float* d_y, const float* d_p, float* d_acts, const float* d_out,
+                     float* d_grad, float* d_scratch, float* d_loss, void* d_work, size_t work_bytes,
+                     exc_stream_t stream);
+int exc_mlp_adam(float* d_p, const float* d_grad, float* d_m, float* d_v, float lr, float beta1, float beta2,
+                 float eps, int step, exc_stream_t stream);
+int exc_mlp_to_bf16(const float* d_p, void* d_pbf16, exc_stream_t stream);
+int exc_mlp_infer_bf16(long B, const float* d_x, const void* d_pbf16, float* d_out, exc_stream_t stream);
 
 /* ---------------------------------------------------------------- data movement */
 int exc_copy2d(long rows, long cols, const double* d_src, long lds, double* d_dst, long ldd, exc_stream_t stream);

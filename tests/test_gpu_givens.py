@@ -48,3 +48,17 @@ def test_givens_f32_close(hip, oracle):
     got = hip.givens_apply(x, pairs, cs, True, True, f32=True)
     want = oracle.givens_mask(x, 32, True, True, 3)
     assert np.max(np.abs(got - want)) < 2e-5 * np.abs(want).max()
+
+
+@pytest.mark.parametrize("m,n,is_left", [(6, 10007, False), (5, 12100, False), (10501, 4, True)])
+def test_virtual_matrix_mask_bit_exact(hip, oracle, m, n, is_left):
+    """block_shift_mask above 10000 lines (bbtsvd.cpp:548-668): the virtual vm x vn matrix, the
+    cyclic shift and the remainder sweeps, mask and unmask, bit for bit with the oracle."""
+    rng = np.random.default_rng(m + n)
+    x = rng.standard_normal((m, n))
+    got = hip.block_shift_mask(x, 32, True, is_left, 4242)
+    want = oracle.block_shift_mask(x, 32, True, is_left, 4242)
+    assert np.array_equal(got, want)
+    back = hip.block_shift_mask(got, 32, False, is_left, 4242)
+    assert np.array_equal(back, oracle.block_shift_mask(want, 32, False, is_left, 4242))
+    assert np.max(np.abs(back - x)) <= 1e-12
