@@ -323,7 +323,7 @@ def bench_learned(n_train=10_000, steps=2000, seed=10):
     net = Lm.BracketNet(seed=0)
     hist = net.train(X, y, steps=steps, batch=4096, lr=2e-3, seed=1)
     held = Lm.make_set(500, (512, 4096), seed=seed + 5)
-    k_lo, k_hi = net.calibrate(*Lm.training_data(held, per=32, seed=seed + 6))
+    k_lo, k_hi = net.calibrate(held, *Lm.targets(held, per=32, seed=seed + 6))
     test = Lm.make_set(4, (16384, 16384), seed=seed + 10)
     fams = {}
     for b in range(test.nb):

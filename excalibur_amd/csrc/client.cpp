@@ -101,8 +101,8 @@ struct ClientDevice {
   HBuf h_masked, h_repart, h_msg, h_U;
   // the mixed-precision form: FP32 copies of the data, the exchanged Grams, the Cholesky factors
   // of the passes, and the host staging of the Grams and of the shared small SVD
-  DBuf x32, y32, g32, li32, Gm, L1, L2, L3, Lt, Uf, idm;
-  HBuf h_gram, h_svd;
+  DBuf x32, y32, g32, li32, Gm, L1, L2, L3, Lt, Uf, idm, pk;
+  HBuf h_gram, h_tri32;
   // the results as the host members hold them (pinned, so the copies run at the link's rate)
   HBuf r_sigma, r_FinalU, r_FinalVT, r_alpha, r_beta;
   void* link = nullptr;  // message link of the one-launch Ralha loop (coherent pinned memory)
@@ -117,6 +117,7 @@ struct ClientDevice {
   // Bulk exchange among peers on one device: this peer's outbox (device memory, exported by IPC)
   // and every other peer's outbox as last opened here (by handle).
   int colocated = 0;
+  int samedev = 0;  // every peer runs on this peer's device (whether or not they exchange there)
   double* outbox = nullptr;
   size_t outbox_cap = 0;  // doubles
   char outbox_handle[EXC_IPC_HANDLE_BYTES] = {0};
@@ -392,6 +393,7 @@ void DecentralizedClient::decide_device_exchange(int mi, int ci, bool one_launch
   if (one_launch && !D.loop_ok)
     log("the peers' bidiagonalization loops do not fit on the device together: one launch per step");
   const bool same = samedev && usable;
+  D.samedev = samedev ? 1 : 0;
   D.colocated = (same && P > 1) ? 1 : 0;
   D.peer_handle.assign(P, std::string());
   D.peer_outbox.assign(P, nullptr);
@@ -1131,19 +1133,20 @@ void DecentralizedClient::local_qr_reduction(size_t mr, size_t n, double** raw_x
 // The mixed-precision form (ClientOptions::form = "mixed"). The m x m factor R of A (A = Q R,
 // A^T = [X_0 ... X_{P-1}]) comes from a CholeskyQR over all peers' Gram matrices instead of the
 // local QR, the masked exchange and the global QR of client.cpp:572-798, and its SVD is computed
-// once, by peer 0, instead of the per-column all-reduce of client.cpp:829-910:
+// by every peer from the same bits instead of the per-column all-reduce of client.cpp:829-910:
 //   pass 1 (FP32 MFMA): G = sum_i X_i X_i^T, masked with the public Givens plan on both sides
-//     before the exchange, shifted by u32 * trace(G) until its Cholesky factor L1 exists;
+//     before the exchange (only the lower triangle of the symmetric masked matrix travels, as
+//     FP32, summed in FP64), shifted by u32 * trace(G) until its Cholesky factor L1 exists;
 //   pass 2 (FP32 MFMA): Y_i = L1^-1 X_i and the Cholesky factor L2 of sum_i Y_i Y_i^T: R1 = L1 L2
 //     is the R^T of a shifted CholeskyQR2 in single precision;
 //   pass 3 (FP64, the refinement of Q): Z_i = R1^-1 X_i by triangular solve, L3 from
 //     sum_i Z_i Z_i^T, which is I to within FP32 error; Q_i^T = L3^-1 Z_i and R^T = Lt = R1 L3
 //     (repeated, Z_i <- L3^-1 Z_i, while ||sum_i Z_i Z_i^T - I||_F > 1/2: at most 3 passes);
-//   peer 0: one-sided bidiagonalization of Lt^T (one owner, no exchange), bisection + twisted
+//   every peer: one-sided bidiagonalization of Lt^T (one owner, no exchange), bisection + twisted
 //     factorization (FP64, as the plain form), then the FP64 Ogita-Aishima refinement of
-//     Lt = U diag(sigma) W^T (exc_oa_refine); it sends sigma, U and W' = L3^-T W to the others;
-//   every peer: FinalVT_i = W'^T Z_i = W^T Q_i^T.
-// Every exchange is a sum of Gram matrices (bit-identical on every peer) or peer 0's broadcast.
+//     Lt = U diag(sigma) W^T (exc_oa_refine), W' = L3^-T W and FinalVT_i = W'^T Z_i = W^T Q_i^T.
+// Every exchange is a sum of Gram matrices, bit-identical on every peer, so every peer factors the
+// same Lt; a checksum of sigma, U and W' exchanged at the end confirms that they agree.
 int DecentralizedClient::run_mixed(bool compute_u, bool compute_v) {
   hipStream_t s = dev_->s;
   exc::ClientDevice& D = *dev_;
@@ -1191,18 +1194,25 @@ int DecentralizedClient::run_mixed(bool compute_u, bool compute_v) {
   D.scal.alloc(4, s);
   D.h_gram.alloc(mm, false);
 
-  // masked FP32 Gram in g32 -> sum over the peers -> unmasked FP64 Gram in D.Gm
+  // The masked Gram matrix Q G Q^T (the public plan on both sides) is symmetric, so only its lower
+  // triangle travels: n_tri = m (m + 1) / 2 values, packed by rows.
+  const long n_tri = (long)m * ((long)m + 1) / 2;
+  D.pk.alloc((size_t)n_tri, s, false);
+  D.h_tri32.alloc(((size_t)n_tri + 1) / 2, false);
+  // masked FP32 Gram in g32 -> its triangle summed over the peers in FP64 -> unmasked FP64 Gram
+  // in D.Gm
   auto exchange_f32 = [&]() {
     double ta = now_seconds();
     givens_mask_f32(g32, (long)m, (long)m, 1, 1, public_seed, plans, s);
     givens_mask_f32(g32, (long)m, (long)m, 1, 0, public_seed, plans, s);
-    ck(exc_convert_f32_f64((long)m, (long)m, g32, (long)m, D.Gm.p, (long)m, s), "convert");
-    ckh(hipMemcpyAsync(D.h_gram.p, D.Gm.p, mm * 8, hipMemcpyDeviceToHost, s), "d2h");
+    ck(exc_pack_lower_f32(mi, g32, (long)m, (float*)D.pk.p, s), "pack");
+    ckh(hipMemcpyAsync(D.h_tri32.p, D.pk.p, (size_t)n_tri * 4, hipMemcpyDeviceToHost, s), "d2h");
     sync();
     double tb = now_seconds();
-    net.all_reduce_sum_same(D.h_gram.p, (long)mm);
+    net.all_reduce_sum_same_f32((const float*)D.h_tri32.p, D.h_gram.p, n_tri);
     double tc = now_seconds();
-    ckh(hipMemcpyAsync(D.Gm.p, D.h_gram.p, mm * 8, hipMemcpyHostToDevice, s), "h2d");
+    ckh(hipMemcpyAsync(D.pk.p, D.h_gram.p, (size_t)n_tri * 8, hipMemcpyHostToDevice, s), "h2d");
+    ck(exc_unpack_lower_f64(mi, D.pk.p, D.Gm.p, (long)m, s), "unpack");
     givens_mask(D.Gm.p, (long)m, (long)m, 0, 1, public_seed, plans, s);
     givens_mask(D.Gm.p, (long)m, (long)m, 0, 0, public_seed, plans, s);
     sync();
@@ -1275,12 +1285,14 @@ int DecentralizedClient::run_mixed(bool compute_u, bool compute_v) {
     double ta = now_seconds();
     givens_mask(D.Gm.p, (long)m, (long)m, 1, 1, public_seed, plans, s);
     givens_mask(D.Gm.p, (long)m, (long)m, 1, 0, public_seed, plans, s);
-    ckh(hipMemcpyAsync(D.h_gram.p, D.Gm.p, mm * 8, hipMemcpyDeviceToHost, s), "d2h");
+    ck(exc_pack_lower_f64(mi, D.Gm.p, (long)m, D.pk.p, s), "pack");
+    ckh(hipMemcpyAsync(D.h_gram.p, D.pk.p, (size_t)n_tri * 8, hipMemcpyDeviceToHost, s), "d2h");
     sync();
     double tb = now_seconds();
-    net.all_reduce_sum_same(D.h_gram.p, (long)mm);
+    net.all_reduce_sum_same(D.h_gram.p, n_tri);
     double tc = now_seconds();
-    ckh(hipMemcpyAsync(D.Gm.p, D.h_gram.p, mm * 8, hipMemcpyHostToDevice, s), "h2d");
+    ckh(hipMemcpyAsync(D.pk.p, D.h_gram.p, (size_t)n_tri * 8, hipMemcpyHostToDevice, s), "h2d");
+    ck(exc_unpack_lower_f64(mi, D.pk.p, D.Gm.p, (long)m, s), "unpack");
     givens_mask(D.Gm.p, (long)m, (long)m, 0, 1, public_seed, plans, s);
     givens_mask(D.Gm.p, (long)m, (long)m, 0, 0, public_seed, plans, s);
     sync();
@@ -1305,7 +1317,7 @@ int DecentralizedClient::run_mixed(bool compute_u, bool compute_v) {
   for (;;) {
     gram_f64();
     fp64_passes++;
-    ck(exc_dev_identity(mi, D.Gm.p, (long)m, D.scal.p + 2, s), "deviation");
+    ck(exc_dev_identity(mi, D.Gm.p, (long)m, D.scal.p + 2, work, work_bytes, s), "deviation");
     double dev[2] = {0.0, 0.0};
     ckh(hipMemcpyAsync(dev, D.scal.p + 2, 16, hipMemcpyDeviceToHost, s), "d2h");
     sync();
@@ -1324,17 +1336,16 @@ int DecentralizedClient::run_mixed(bool compute_u, bool compute_v) {
   time_global_qr = 0;
   log("Mixed CholeskyQR (FP32 x 2, FP64 x " + std::to_string(fp64_passes) + ") Costs " + std::to_string(tf));
 
-  // ---- the SVD of Lt on peer 0, sent to the others
+  // ---- the SVD of Lt, on every peer: Lt is bit-identical on all of them (every exchange is a sum
+  // in a fixed order that all peers evaluate alike) and every step below is deterministic, so all
+  // peers obtain the same sigma, U and W' without sending them; a checksum confirms it.
   int k = 0;
   D.alpha.alloc(m, s);
   D.beta.alloc(m, s);
-  // message to the other peers: k, then sigma (m), U (m x m), W' (m x m), alpha (m), beta (m)
-  const size_t msg_doubles = 1 + m + 2 * mm + 2 * m;
-  D.h_svd.alloc(msg_doubles, false);
   D.Uf.alloc(mm, s, false);
   D.bu.alloc(mm, s);
   D.sigma.alloc(m, s);
-  if (cid == 0) {
+  {
     t0 = now_seconds();
     D.F.alloc(mm, s, false);
     D.U.alloc(mm, s);
@@ -1349,8 +1360,10 @@ int DecentralizedClient::run_mixed(bool compute_u, bool compute_v) {
       D.link_m = mi;
     }
     D.msgd.alloc((exc_bidiag_local_workspace(mi, mi) + 7) / 8, s, false);
+    // peers that share the device run their loops at the same time
+    const int colocated = D.samedev ? std::max(1, (int)num_client) : 1;
     ck(exc_bidiag_local(mi, mi, D.F.p, D.U.p, D.house.p, D.tau_h.p, D.alpha.p, D.beta.p, D.cs.p,
-                        opt_.ralha_one_launch ? D.link : nullptr, D.msgd.p, D.msgd.n * 8,
+                        opt_.ralha_one_launch ? D.link : nullptr, colocated, D.msgd.p, D.msgd.n * 8,
                         std::max(30.0, opt_.recv_timeout_s), s),
        "bidiagonalization");
     sync();
@@ -1397,31 +1410,35 @@ int DecentralizedClient::run_mixed(bool compute_u, bool compute_v) {
     ck(exc_trsm_left(1, 0, mi, k, D.idm.p, (long)m, D.bu.p, k, D.work.p, D.work.n * 8, s), "W'");
     sync();
     time_refine = now_seconds() - t0;
-    double* h = D.h_svd.p;
-    h[0] = k;
-    ckh(hipMemcpyAsync(h + 1, D.sigma.p, (size_t)k * 8, hipMemcpyDeviceToHost, s), "d2h");
-    ckh(hipMemcpyAsync(h + 1 + m, D.Uf.p, m * (size_t)k * 8, hipMemcpyDeviceToHost, s), "d2h");
-    ckh(hipMemcpyAsync(h + 1 + m + mm, D.bu.p, m * (size_t)k * 8, hipMemcpyDeviceToHost, s), "d2h");
-    ckh(hipMemcpyAsync(h + 1 + m + 2 * mm, D.alpha.p, m * 8, hipMemcpyDeviceToHost, s), "d2h");
-    ckh(hipMemcpyAsync(h + 1 + 2 * m + 2 * mm, D.beta.p, m * 8, hipMemcpyDeviceToHost, s), "d2h");
+  }
+  if (num_client > 1) {
+    // FNV-1a over k, sigma and the first and last rows of U and W'
+    std::vector<double> probe((size_t)1 + 5 * k);
+    probe[0] = k;
+    ckh(hipMemcpyAsync(probe.data() + 1, D.sigma.p, (size_t)k * 8, hipMemcpyDeviceToHost, s), "d2h");
+    ckh(hipMemcpyAsync(probe.data() + 1 + k, D.Uf.p, (size_t)k * 8, hipMemcpyDeviceToHost, s), "d2h");
+    ckh(hipMemcpyAsync(probe.data() + 1 + 2 * k, D.Uf.p + (m - 1) * (size_t)k, (size_t)k * 8,
+                       hipMemcpyDeviceToHost, s),
+        "d2h");
+    ckh(hipMemcpyAsync(probe.data() + 1 + 3 * k, D.bu.p, (size_t)k * 8, hipMemcpyDeviceToHost, s), "d2h");
+    ckh(hipMemcpyAsync(probe.data() + 1 + 4 * k, D.bu.p + (m - 1) * (size_t)k, (size_t)k * 8,
+                       hipMemcpyDeviceToHost, s),
+        "d2h");
     sync();
+    uint64_t h = 1469598103934665603ull;
+    const unsigned char* bytes = (const unsigned char*)probe.data();
+    for (size_t q = 0; q < probe.size() * sizeof(double); q++) h = (h ^ bytes[q]) * 1099511628211ull;
     const double tb = now_seconds();
-    if (num_client > 1) net.broadcast(h, msg_doubles * 8);
+    net.broadcast(&h, sizeof h);
+    for (int j = 0; j < (int)num_client; j++) {
+      if (j == cid) continue;
+      uint64_t o = 0;
+      net.receive_into(j, &o, sizeof o);
+      if (o != h)
+        throw std::runtime_error("mixed form: peer " + std::to_string(j) +
+                                 " computed a different SVD of the shared factor (checksums differ)");
+    }
     t_comm += now_seconds() - tb;
-    net.add_comm_seconds(now_seconds() - tb);
-  } else {
-    t0 = now_seconds();
-    net.receive_into(0, D.h_svd.p, msg_doubles * 8);
-    t_comm += now_seconds() - t0;
-    net.add_comm_seconds(now_seconds() - t0);
-    const double* h = D.h_svd.p;
-    k = (int)h[0];
-    ckh(hipMemcpyAsync(D.sigma.p, h + 1, (size_t)k * 8, hipMemcpyHostToDevice, s), "h2d");
-    ckh(hipMemcpyAsync(D.Uf.p, h + 1 + m, m * (size_t)k * 8, hipMemcpyHostToDevice, s), "h2d");
-    ckh(hipMemcpyAsync(D.bu.p, h + 1 + m + mm, m * (size_t)k * 8, hipMemcpyHostToDevice, s), "h2d");
-    ckh(hipMemcpyAsync(D.alpha.p, h + 1 + m + 2 * mm, m * 8, hipMemcpyHostToDevice, s), "h2d");
-    ckh(hipMemcpyAsync(D.beta.p, h + 1 + 2 * m + 2 * mm, m * 8, hipMemcpyHostToDevice, s), "h2d");
-    sync();
   }
   time_exchange_low_dim = t_comm;
 
@@ -1439,7 +1456,7 @@ int DecentralizedClient::run_mixed(bool compute_u, bool compute_v) {
     ck(exc_dgemm(1, 0, kk, r, (long)m, 1.0, D.bu.p, k, D.X.p, r, 0.0, D.FinalVT.p, r, nullptr, 0, s), "final V");
   }
   sync();
-  time_combine_svd_remove_mask = (cid == 0 ? time_refine : 0.0) + (now_seconds() - t0);
+  time_combine_svd_remove_mask = time_refine + (now_seconds() - t0);
   D.x32.release();
   D.y32.release();
   return kk;

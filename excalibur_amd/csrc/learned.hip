@@ -132,12 +132,14 @@ __global__ __launch_bounds__(256) void k_count_features(int nb, int K, const int
 // only for the sides no bound covers. Outputs the leaf midpoint, the leaf's endpoints, the number
 // of Sturm counts evaluated and a status (1: bounded on both sides, 2: on one side, 0: from the
 // root, -1: the root does not hold the value, as when the reference's root bound is below
-// sigma_max).
+// sigma_max). Fallback points flo / fhi (optional) are counted only for a side the bracket left
+// open: flo for a missing lower bound, fhi for a missing upper one.
 __global__ __launch_bounds__(64) void k_bisect_index(int ntg, const int* __restrict__ ns, const long* __restrict__ off,
                                                      const double* __restrict__ d, const double* __restrict__ e,
                                                      const double* __restrict__ root, double tol,
                                                      const int* __restrict__ tb, const int* __restrict__ ti,
                                                      const double* __restrict__ blo, const double* __restrict__ bhi,
+                                                     const double* __restrict__ flo, const double* __restrict__ fhi,
                                                      double* __restrict__ sigma, double* __restrict__ leaf_lo,
                                                      double* __restrict__ leaf_hi, int* __restrict__ iters,
                                                      int* __restrict__ status) {
@@ -171,6 +173,20 @@ __global__ __launch_bounds__(64) void k_bisect_index(int ntg, const int* __restr
       }
     }
     if (haveL && haveH && !(L < H)) haveL = haveH = false;  // counts that are not monotone: use neither
+  }
+  if (flo && !haveL) {
+    const double p = flo[j];
+    if (p > lo && p < hi && (!haveH || p < H)) {
+      it++;
+      if (sturm_count(n, a, bb, p) <= t) { L = p; haveL = true; }
+    }
+  }
+  if (fhi && !haveH) {
+    const double p = fhi[j];
+    if (p > lo && p < hi && (!haveL || p > L)) {
+      it++;
+      if (t < sturm_count(n, a, bb, p)) { H = p; haveH = true; }
+    }
   }
   if (!haveL || !haveH) {
     bool in_tree = true;
@@ -375,16 +391,16 @@ int exc_count_features(int nb, int K, const int* d_n, const long* d_off, const d
 
 int exc_bisect_index(int ntg, const int* d_n, const long* d_off, const double* d_d, const double* d_e,
                      const double* d_root, double tol, const int* d_tb, const int* d_ti, const double* d_blo,
-                     const double* d_bhi, double* d_sigma, double* d_leaf_lo, double* d_leaf_hi, int* d_iters,
-                     int* d_status, exc_stream_t stream) {
+                     const double* d_bhi, const double* d_flo, const double* d_fhi, double* d_sigma,
+                     double* d_leaf_lo, double* d_leaf_hi, int* d_iters, int* d_status, exc_stream_t stream) {
   EXC_REQUIRE(ntg >= 0 && (ntg == 0 || (d_n && d_off && d_d && d_e && d_root && d_tb && d_ti && d_sigma &&
                                         d_leaf_lo && d_leaf_hi && d_iters && d_status)) &&
-                  (!d_blo) == (!d_bhi),
+                  (!d_blo) == (!d_bhi) && (!d_flo) == (!d_fhi),
               EXC_ERR_ARG, "exc_bisect_index: bad arguments");
   if (ntg == 0) return EXC_OK;
   hipLaunchKernelGGL(k_bisect_index, dim3(grid_for(ntg, 64)), dim3(64), 0, exc_detail::stream(stream), ntg, d_n,
-                     d_off, d_d, d_e, d_root, tol, d_tb, d_ti, d_blo, d_bhi, d_sigma, d_leaf_lo, d_leaf_hi, d_iters,
-                     d_status);
+                     d_off, d_d, d_e, d_root, tol, d_tb, d_ti, d_blo, d_bhi, d_flo, d_fhi, d_sigma, d_leaf_lo,
+                     d_leaf_hi, d_iters, d_status);
   EXC_CHECK_HIP(hipGetLastError());
   return EXC_OK;
 }

@@ -54,12 +54,19 @@ __global__ __launch_bounds__(256) void k_cvt_f64_f32_flat(long n, const double* 
 }
 
 // out[0] = trace(A) (sequential order, one thread: n is small).
-__global__ void k_trace(int n, const double* __restrict__ a, long lda, double* __restrict__ out) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    double t = 0.0;
-    for (int i = 0; i < n; i++) t = t + a[(long)i * lda + i];
-    out[0] = t;
+// out[0] = trace(A): thread t sums the diagonal entries t, t + 256, ... in order, then the 256
+// partial sums are added in a fixed tree order (the same bits on every run and every device).
+__global__ __launch_bounds__(256) void k_trace(int n, const double* __restrict__ a, long lda, double* __restrict__ out) {
+  __shared__ double part[256];
+  double t = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) t = t + a[(long)i * lda + i];
+  part[threadIdx.x] = t;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) part[threadIdx.x] = part[threadIdx.x] + part[threadIdx.x + w];
+    __syncthreads();
   }
+  if (threadIdx.x == 0) out[0] = part[0];
 }
 
 __global__ void k_add_diag(int n, double* __restrict__ a, long lda, double v) {
@@ -67,17 +74,19 @@ __global__ void k_add_diag(int n, double* __restrict__ a, long lda, double v) {
   if (i < n) a[(long)i * lda + i] = a[(long)i * lda + i] + v;
 }
 
-// out[0] = max |A - I| and out[1] = ||A - I||_F over the n x n matrix (one workgroup).
-__global__ __launch_bounds__(256) void k_dev_identity(int n, const double* __restrict__ a, long lda,
-                                                      double* __restrict__ out) {
+// Per-workgroup partials of max |A - I| and sum (A - I)^2 over rows r = blockIdx.x, blockIdx.x +
+// gridDim.x, ...; part[2 b] = max, part[2 b + 1] = sum of squares (fixed order within the block).
+constexpr int kDevBlocks = 256;
+__global__ __launch_bounds__(256) void k_dev_identity_part(int n, const double* __restrict__ a, long lda,
+                                                           double* __restrict__ part) {
   __shared__ double red[4], rsq[4];
   double mx = 0.0, sq = 0.0;
-  for (long e = threadIdx.x; e < (long)n * n; e += blockDim.x) {
-    const long i = e / n, j = e - i * n;
-    const double v = a[i * lda + j] - (i == j ? 1.0 : 0.0);
-    mx = fmax(mx, fabs(v));
-    sq += v * v;
-  }
+  for (long i = blockIdx.x; i < n; i += gridDim.x)
+    for (long j = threadIdx.x; j < n; j += blockDim.x) {
+      const double v = a[i * lda + j] - (i == j ? 1.0 : 0.0);
+      mx = fmax(mx, fabs(v));
+      sq += v * v;
+    }
   mx = wave_max(mx);
   sq = wave_sum(sq);
   if ((threadIdx.x & 63) == 0) {
@@ -86,9 +95,51 @@ __global__ __launch_bounds__(256) void k_dev_identity(int n, const double* __res
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    out[0] = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
-    out[1] = sqrt(rsq[0] + rsq[1] + rsq[2] + rsq[3]);
+    part[2 * blockIdx.x] = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+    part[2 * blockIdx.x + 1] = ((rsq[0] + rsq[1]) + rsq[2]) + rsq[3];
   }
+}
+
+// out[0] = max over the partials, out[1] = sqrt of their sums of squares added in block order.
+__global__ void k_dev_identity_final(int nparts, const double* __restrict__ part, double* __restrict__ out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    double mx = 0.0, sq = 0.0;
+    for (int b = 0; b < nparts; b++) {
+      mx = fmax(mx, part[2 * b]);
+      sq = sq + part[2 * b + 1];
+    }
+    out[0] = mx;
+    out[1] = sqrt(sq);
+  }
+}
+
+// Row-packed lower triangle: packed[i (i + 1) / 2 + j] = a[i * lda + j], j <= i. One thread per
+// packed entry; the row comes from the index by the inverse of the triangular numbers.
+__device__ __forceinline__ void tri_index(long q, long& i, long& j) {
+  i = (long)((sqrt(8.0 * (double)q + 1.0) - 1.0) * 0.5);
+  while (i * (i + 1) / 2 > q) i--;
+  while ((i + 1) * (i + 2) / 2 <= q) i++;
+  j = q - i * (i + 1) / 2;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_pack_lower(int n, const T* __restrict__ a, long lda, T* __restrict__ packed) {
+  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= (long)n * (n + 1) / 2) return;
+  long i, j;
+  tri_index(q, i, j);
+  packed[q] = a[i * lda + j];
+}
+
+__global__ __launch_bounds__(256) void k_unpack_lower(int n, const double* __restrict__ packed, double* __restrict__ a,
+                                                      long lda) {
+  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= (long)n * (n + 1) / 2) return;
+  long i, j;
+  tri_index(q, i, j);
+  const double v = packed[q];
+  a[i * lda + j] = v;
+  a[j * lda + i] = v;
 }
 
 // sigma_i = T_ii / (1 - (R_ii + S_ii) / 2) with R_ii = 1 - (U^T U)_ii, S_ii = 1 - (W^T W)_ii;
@@ -200,7 +251,7 @@ int exc_convert_f32_f64(long rows, long cols, const float* d_src, long lds, doub
 
 int exc_trace(int n, const double* d_a, long lda, double* d_out, exc_stream_t stream) {
   EXC_REQUIRE(n >= 1 && d_a && d_out && lda >= n, EXC_ERR_ARG, "exc_trace: bad arguments");
-  hipLaunchKernelGGL(k_trace, dim3(1), dim3(64), 0, exc_detail::stream(stream), n, d_a, lda, d_out);
+  hipLaunchKernelGGL(k_trace, dim3(1), dim3(256), 0, exc_detail::stream(stream), n, d_a, lda, d_out);
   EXC_CHECK_HIP(hipGetLastError());
   return EXC_OK;
 }
@@ -213,9 +264,46 @@ int exc_add_diag(int n, double* d_a, long lda, double value, exc_stream_t stream
   return EXC_OK;
 }
 
-int exc_dev_identity(int n, const double* d_a, long lda, double* d_out, exc_stream_t stream) {
-  EXC_REQUIRE(n >= 1 && d_a && d_out && lda >= n, EXC_ERR_ARG, "exc_dev_identity: bad arguments");
-  hipLaunchKernelGGL(k_dev_identity, dim3(1), dim3(256), 0, exc_detail::stream(stream), n, d_a, lda, d_out);
+size_t exc_dev_identity_workspace(int n) {
+  (void)n;
+  return (size_t)2 * kDevBlocks * sizeof(double);
+}
+
+int exc_dev_identity(int n, const double* d_a, long lda, double* d_out, void* d_work, size_t work_bytes,
+                     exc_stream_t stream) {
+  EXC_REQUIRE(n >= 1 && d_a && d_out && lda >= n && d_work, EXC_ERR_ARG, "exc_dev_identity: bad arguments");
+  EXC_REQUIRE(work_bytes >= exc_dev_identity_workspace(n), EXC_ERR_SPACE, "exc_dev_identity: workspace too small");
+  const int blocks = std::min(n, kDevBlocks);
+  hipStream_t s = exc_detail::stream(stream);
+  hipLaunchKernelGGL(k_dev_identity_part, dim3(blocks), dim3(256), 0, s, n, d_a, lda, (double*)d_work);
+  hipLaunchKernelGGL(k_dev_identity_final, dim3(1), dim3(64), 0, s, blocks, (const double*)d_work, d_out);
+  EXC_CHECK_HIP(hipGetLastError());
+  return EXC_OK;
+}
+
+int exc_pack_lower_f32(int n, const float* d_a, long lda, float* d_packed, exc_stream_t stream) {
+  EXC_REQUIRE(n >= 1 && d_a && d_packed && lda >= n, EXC_ERR_ARG, "exc_pack_lower_f32: bad arguments");
+  const long t = (long)n * (n + 1) / 2;
+  hipLaunchKernelGGL(k_pack_lower<float>, dim3(exc_detail::ceil_div(t, 256)), dim3(256), 0,
+                     exc_detail::stream(stream), n, d_a, lda, d_packed);
+  EXC_CHECK_HIP(hipGetLastError());
+  return EXC_OK;
+}
+
+int exc_pack_lower_f64(int n, const double* d_a, long lda, double* d_packed, exc_stream_t stream) {
+  EXC_REQUIRE(n >= 1 && d_a && d_packed && lda >= n, EXC_ERR_ARG, "exc_pack_lower_f64: bad arguments");
+  const long t = (long)n * (n + 1) / 2;
+  hipLaunchKernelGGL(k_pack_lower<double>, dim3(exc_detail::ceil_div(t, 256)), dim3(256), 0,
+                     exc_detail::stream(stream), n, d_a, lda, d_packed);
+  EXC_CHECK_HIP(hipGetLastError());
+  return EXC_OK;
+}
+
+int exc_unpack_lower_f64(int n, const double* d_packed, double* d_a, long lda, exc_stream_t stream) {
+  EXC_REQUIRE(n >= 1 && d_a && d_packed && lda >= n, EXC_ERR_ARG, "exc_unpack_lower_f64: bad arguments");
+  const long t = (long)n * (n + 1) / 2;
+  hipLaunchKernelGGL(k_unpack_lower, dim3(exc_detail::ceil_div(t, 256)), dim3(256), 0, exc_detail::stream(stream),
+                     n, d_packed, d_a, lda);
   EXC_CHECK_HIP(hipGetLastError());
   return EXC_OK;
 }

@@ -175,7 +175,15 @@ void PeerNet::reader_loop(int fd, int from) {
   while (!stop_) {
     uint64_t size = 0;
     if (!read_spin(fd, &size, sizeof size, hot_until, stop_)) break;
-    std::vector<char> msg(size);
+    std::vector<char> msg;
+    {
+      std::lock_guard<std::mutex> g(box.mu);
+      if (!box.pool.empty()) {
+        msg = std::move(box.pool.back());
+        box.pool.pop_back();
+      }
+    }
+    msg.resize(size);  // a pooled buffer at least this large only changes its length
     if (size && !read_spin(fd, msg.data(), size, now_seconds() + 1.0, stop_)) break;
     hot_until = now_seconds() + kHotSeconds;
     {
@@ -254,12 +262,20 @@ std::vector<char> PeerNet::receive(int peer) {
   return m;
 }
 
+void PeerNet::recycle(int peer, std::vector<char>&& m) {
+  if (m.capacity() < (size_t(1) << 16)) return;  // small frames are cheap to allocate
+  Inbox& box = *inbox_[peer];
+  std::lock_guard<std::mutex> g(box.mu);
+  if (box.pool.size() < 4) box.pool.push_back(std::move(m));
+}
+
 void PeerNet::receive_into(int peer, void* buf, size_t bytes) {
   std::vector<char> m = receive(peer);
   if (m.size() != bytes)
     throw std::runtime_error("message from peer " + std::to_string(peer) + " has " + std::to_string(m.size()) +
                              " bytes, expected " + std::to_string(bytes));
   if (bytes) memcpy(buf, m.data(), bytes);
+  recycle(peer, std::move(m));
 }
 
 bool PeerNet::any_closed() {
@@ -291,6 +307,7 @@ void PeerNet::ring_all_reduce_sum(double* buf, long n) {
       std::vector<char> m = receive(j);
       const double* x = (const double*)m.data();
       for (long q = 0; q < n; q++) buf[q] = buf[q] + x[q];
+      recycle(j, std::move(m));
     }
   } else {
     std::vector<long> split(P_ + 1, 0);
@@ -305,6 +322,7 @@ void PeerNet::ring_all_reduce_sum(double* buf, long n) {
       di = (di - 1 + P_) % P_;
       const double* x = (const double*)m.data();
       for (long q = 0; q < split[di + 1] - split[di]; q++) buf[split[di] + q] = buf[split[di] + q] + x[q];
+      recycle(front, std::move(m));
     }
     for (int s = 0; s < P_ - 1; s++) {
       int di = (cid_ + 1 - s + P_) % P_;
@@ -312,6 +330,7 @@ void PeerNet::ring_all_reduce_sum(double* buf, long n) {
       std::vector<char> m = receive(front);
       di = (di - 1 + P_) % P_;
       memcpy(buf + split[di], m.data(), (split[di + 1] - split[di]) * sizeof(double));
+      recycle(front, std::move(m));
     }
   }
   comm_s_ += now_seconds() - t0;
@@ -333,6 +352,28 @@ void PeerNet::all_reduce_sum_same(double* buf, long n) {
     for (int j = 0; j < P_; j++) acc = acc + (j == cid_ ? own[q] : ((const double*)got[j].data())[q]);
     buf[q] = acc;
   }
+  for (int j = 0; j < P_; j++)
+    if (j != cid_) recycle(j, std::move(got[j]));
+  comm_s_ += now_seconds() - t0;
+}
+
+void PeerNet::all_reduce_sum_same_f32(const float* in, double* out, long n) {
+  const double t0 = now_seconds();
+  broadcast(in, n * sizeof(float));
+  std::vector<std::vector<char>> got(P_);
+  for (int j = 0; j < P_; j++)
+    if (j != cid_) {
+      got[j] = receive(j);
+      if (got[j].size() != (size_t)n * sizeof(float))
+        throw std::runtime_error("all_reduce_sum_same_f32: frame of the wrong size from peer " + std::to_string(j));
+    }
+  for (long q = 0; q < n; q++) {
+    double acc = 0.0;
+    for (int j = 0; j < P_; j++) acc = acc + (double)(j == cid_ ? in[q] : ((const float*)got[j].data())[q]);
+    out[q] = acc;
+  }
+  for (int j = 0; j < P_; j++)
+    if (j != cid_) recycle(j, std::move(got[j]));
   comm_s_ += now_seconds() - t0;
 }
 

@@ -42,6 +42,9 @@ class PeerNet {
   // all buffers in peer order 0..P-1; above, the ring (whose chunks are reduced once and then
   // copied) as ring_all_reduce_sum.
   void all_reduce_sum_same(double* buf, long n);
+  // out = sum over the peers of their FP32 buffers `in`, each widened to FP64 and added in peer
+  // order 0..P-1, bit-identical on every peer; every peer sends its n floats to every other.
+  void all_reduce_sum_same_f32(const float* in, double* out, long n);
   // client.cpp:358-411: peer `src` of group `grp` distributes `bytes` bytes (buf is filled on
   // the receivers).
   void all_gather(std::vector<char>& buf, size_t bytes, const std::vector<int>& grp, int src);
@@ -62,7 +65,12 @@ class PeerNet {
     std::deque<std::vector<char>> q;
     std::atomic<int> n{0};
     bool closed = false;
+    // frame buffers handed back after use: the reader fills them again without zeroing or
+    // faulting in fresh pages (large frames otherwise cost a memset and a page fault per 4 KB)
+    std::vector<std::vector<char>> pool;
   };
+  // Returns a consumed frame's buffer to the pool of the peer it came from.
+  void recycle(int peer, std::vector<char>&& m);
   void accept_loop();
   void reader_loop(int fd, int from);
 

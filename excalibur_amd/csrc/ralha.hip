@@ -1379,10 +1379,10 @@ extern "C" {
 size_t exc_bidiag_local_workspace(int m, int c) { return m < 3 || c < 1 ? 0 : local_ws(m, c, nullptr, nullptr); }
 
 int exc_bidiag_local(int m, int c, double* d_fc, double* d_uc, double* d_house, double* d_tau, double* d_alpha,
-                     double* d_beta, double* d_cs, void* h_link, void* d_work, size_t work_bytes, double timeout_s,
-                     exc_stream_t stream) {
+                     double* d_beta, double* d_cs, void* h_link, int colocated, void* d_work, size_t work_bytes,
+                     double timeout_s, exc_stream_t stream) {
   EXC_REQUIRE(m >= 3 && c >= 1 && d_fc && d_uc && d_house && d_tau && d_alpha && d_beta && d_cs && d_work &&
-                  timeout_s > 0,
+                  timeout_s > 0 && colocated >= 1,
               EXC_ERR_ARG, "exc_bidiag_local: bad arguments");
   EXC_REQUIRE(work_bytes >= exc_bidiag_local_workspace(m, c), EXC_ERR_SPACE, "exc_bidiag_local: workspace too small");
   LocalWs w;
@@ -1396,7 +1396,9 @@ int exc_bidiag_local(int m, int c, double* d_fc, double* d_uc, double* d_house, 
   const void* fn = loop_kernel_for(sh);
   bool fits = false;
   if (h_link && fn) {
-    rc = loop_resident(sh, fn, &fits, 1);
+    // every colocated caller's grid must be resident at once: a grid that is only partly resident
+    // waits at its barriers for workgroups another caller's waiting grid keeps off the device
+    rc = loop_resident(sh, fn, &fits, colocated);
     if (rc) return rc;
   }
   if (fits) {

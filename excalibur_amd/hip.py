@@ -105,12 +105,16 @@ SIGNATURES = {
     "exc_pack_reflectors": (_I, [_L, _L, _L, _dp, _L, _dp, _dp, _vp]),
     "exc_unpack_reflectors": (_I, [_L, _L, _L, _dp, _dp, _L, _dp, _vp]),
     "exc_bidiag_local_workspace": (_S, [_I, _I]),
-    "exc_bidiag_local": (_I, [_I, _I, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _vp, _dp, _S, _D, _vp]),
+    "exc_bidiag_local": (_I, [_I, _I, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _vp, _I, _dp, _S, _D, _vp]),
+    "exc_pack_lower_f32": (_I, [_I, _vp, _L, _vp, _vp]),
+    "exc_pack_lower_f64": (_I, [_I, _dp, _L, _dp, _vp]),
+    "exc_unpack_lower_f64": (_I, [_I, _dp, _dp, _L, _vp]),
     "exc_convert_f64_f32": (_I, [_L, _L, _dp, _L, _dp, _L, _vp]),
     "exc_convert_f32_f64": (_I, [_L, _L, _dp, _L, _dp, _L, _vp]),
     "exc_trace": (_I, [_I, _dp, _L, _dp, _vp]),
     "exc_add_diag": (_I, [_I, _dp, _L, _D, _vp]),
-    "exc_dev_identity": (_I, [_I, _dp, _L, _dp, _vp]),
+    "exc_dev_identity_workspace": (_S, [_I]),
+    "exc_dev_identity": (_I, [_I, _dp, _L, _dp, _vp, _S, _vp]),
     "exc_oa_refine_workspace": (_S, [_I]),
     "exc_oa_refine": (_I, [_I, _dp, _L, _I, _dp, _L, _dp, _L, _dp, _I, _D, _D, _dp, _S, _vp]),
     "exc_givens_apply_batched": (_I, [_I, _vp, _L, _L, _vp, _vp, _L, _I, _I, _vp]),
@@ -124,7 +128,7 @@ SIGNATURES = {
     "exc_bidiag_scale": (_I, [_I] + [_vp] * 6 + [_vp]),
     "exc_count_features": (_I, [_I, _I] + [_vp] * 6 + [_vp]),
     "exc_bisect_index": (_I, [_I] + [_vp] * 5 + [_D] +
-                         [_vp] * 9 + [_vp]),
+                         [_vp] * 11 + [_vp]),
     "exc_mlp_param_count": (_L, []),
     "exc_mlp_dims": (_I, [ctypes.POINTER(_I)] * 3),
     "exc_mlp_acts_floats": (_S, [_L]),
@@ -641,7 +645,7 @@ def bidiag_local(F, one_launch=True, timeout_s=30.0):
         call("exc_ralha_link_alloc", m, ctypes.byref(link))
     try:
         call("exc_bidiag_local", m, c, dF.ptr, dU.ptr, dh.ptr, dt.ptr, da.ptr, db.ptr, dcs.ptr,
-             link if one_launch else None, ws.ptr, ws.nbytes, timeout_s, None)
+             link if one_launch else None, 1, ws.ptr, ws.nbytes, timeout_s, None)
         sync()
     finally:
         if one_launch:

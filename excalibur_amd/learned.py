@@ -143,20 +143,24 @@ class BidiagSet:
         X[:, K_SHIFTS + 2] = 1.0
         return X
 
-    def bisect(self, tb, ti, blo=None, bhi=None):
-        """exc_bisect_index for the targets; returns dict(sigma, leaf_lo, leaf_hi, iters, status, ms)."""
+    def bisect(self, tb, ti, blo=None, bhi=None, flo=None, fhi=None):
+        """exc_bisect_index for the targets (bracket blo/bhi, fallback points flo/fhi); returns
+        dict(sigma, leaf_lo, leaf_hi, iters, status, ms)."""
         m = len(tb)
         dtb = DA.from_host(np.asarray(tb, np.int32))
         dti = DA.from_host(np.asarray(ti, np.int32))
         dlo = DA.from_host(np.asarray(blo, np.float64)) if blo is not None else None
         dhi = DA.from_host(np.asarray(bhi, np.float64)) if bhi is not None else None
+        dflo = DA.from_host(np.asarray(flo, np.float64)) if flo is not None else None
+        dfhi = DA.from_host(np.asarray(fhi, np.float64)) if fhi is not None else None
         out = {k: DA((m,)) for k in ("sigma", "leaf_lo", "leaf_hi")}
         dit = DA((m,), np.int32)
         dst = DA((m,), np.int32)
         hip.sync()
         t0 = time.perf_counter()
         hip.call("exc_bisect_index", m, self.d_n.ptr, self.d_off.ptr, self.d_d.ptr, self.d_e.ptr, self.d_root.ptr,
-                 TOL, dtb.ptr, dti.ptr, dlo.ptr if dlo else None, dhi.ptr if dhi else None, out["sigma"].ptr,
+                 TOL, dtb.ptr, dti.ptr, dlo.ptr if dlo else None, dhi.ptr if dhi else None,
+                 dflo.ptr if dflo else None, dfhi.ptr if dfhi else None, out["sigma"].ptr,
                  out["leaf_lo"].ptr, out["leaf_hi"].ptr, dit.ptr, dst.ptr, None)
         hip.sync()
         ms = 1000.0 * (time.perf_counter() - t0)
@@ -286,44 +290,71 @@ class BracketNet:
         self.infer_ms = 1000.0 * (time.perf_counter() - t0)
         return out.to_host().reshape(B, 2)
 
-    def calibrate(self, X, y, q_lo=0.5, q_hi=0.99):
-        """Split-conformal bracket widths: on held-out samples (bidiagonals the network was not
-        trained on), the q_lo quantile of the scaled error (c - y) / exp(w) below the prediction and
-        the q_hi quantile of (y - c) / exp(w) above it. A lower end that misses still bounds the
-        value from above, so it is set near the median; the upper end, which spares the most counts
-        when it holds, is set wide."""
-        O = self.infer(X).astype(np.float64)
-        z = (np.asarray(y, np.float64) - O[:, 0]) / np.exp(O[:, 1])
-        self.k_lo = float(max(np.quantile(-z, q_lo), 0.0))
-        self.k_hi = float(max(np.quantile(z, q_hi), 0.0))
+    def predict(self, bs, tb, ti):
+        """(c, b): the BF16 network's prediction of y = ln(sigma / scale) / Y_SCALE and its Laplace
+        scale, per target."""
+        O = self.infer(bs.features(tb, ti)).astype(np.float64)
+        return O[:, 0], np.exp(np.clip(O[:, 1], -30.0, 5.0))
+
+    def _ends(self, bs, tb, c, b, k_lo, k_hi):
+        s = bs.scale[np.asarray(tb)]
+        lo = s * np.exp(np.clip(Y_SCALE * (c - k_lo * b), -700.0, 700.0))
+        hi = s * np.exp(np.clip(Y_SCALE * (c + k_hi * b), -700.0, 700.0))
+        return lo, hi
+
+    def calibrate(self, bs, tb, ti, y, quantiles=(0.3, 0.5, 0.7, 0.9, 0.97, 0.99, 0.999)):
+        """Bracket ends from held-out targets (bidiagonals the network was not trained on): the
+        candidate ends are quantiles of the scaled error (y - c) / b on each side, and the pair that
+        evaluates the fewest Sturm counts on those targets is kept. A lower end that misses still
+        bounds the value from above (and an upper end that misses, from below), so the best ends
+        are not simply the widest."""
+        c, b = self.predict(bs, tb, ti)
+        z = (np.asarray(y, np.float64) - c) / b
+        _, clo, chi = bs.cell(tb, ti)
+        best = None
+        for k_lo in sorted(set(float(np.quantile(-z, q)) for q in quantiles)):
+            for k_hi in sorted(set(float(np.quantile(z, q)) for q in quantiles)):
+                if k_lo + k_hi <= 0:
+                    continue
+                lo, hi = self._clip(*self._ends(bs, tb, c, b, k_lo, k_hi), clo, chi)
+                counts = int(bs.bisect(tb, ti, lo, hi, clo, chi)["iters"].sum())
+                if best is None or counts < best[0]:
+                    best = (counts, k_lo, k_hi)
+        self.k_lo, self.k_hi = best[1], best[2]
         return self.k_lo, self.k_hi
 
-    def brackets(self, bs, tb, ti):
-        """Brackets [lo, hi) for the targets from the BF16 network, clipped to the shifts that
-        enclose each target by the feature counts."""
-        O = self.infer(bs.features(tb, ti)).astype(np.float64)
-        c, b = O[:, 0], np.exp(np.clip(O[:, 1], -30.0, 5.0))
-        s = bs.scale[np.asarray(tb)]
-        lo = s * np.exp(np.clip(Y_SCALE * (c - self.k_lo * b), -700.0, 700.0))
-        hi = s * np.exp(np.clip(Y_SCALE * (c + self.k_hi * b), -700.0, 700.0))
-        _, clo, chi = bs.cell(tb, ti)
+    @staticmethod
+    def _clip(lo, hi, clo, chi):
         lo, hi = np.maximum(lo, clo), np.minimum(hi, chi)
         off = ~(lo < hi)  # a prediction outside the enclosing shifts: keep the shifts
         lo[off], hi[off] = clo[off], chi[off]
         return lo, hi
 
+    def brackets(self, bs, tb, ti):
+        """(lo, hi, flo, fhi): the BF16 network's bracket for each target, clipped to the shifts
+        that enclose it by the feature counts, and those shifts as the fallback points."""
+        c, b = self.predict(bs, tb, ti)
+        _, clo, chi = bs.cell(tb, ti)
+        lo, hi = self._clip(*self._ends(bs, tb, c, b, self.k_lo, self.k_hi), clo, chi)
+        return lo, hi, clo, chi
 
-def training_data(bs, per, seed):
-    """Targets (per random indices of every bidiagonal), their features and labels from the
-    reference's bisection tree."""
+
+def targets(bs, per, seed):
+    """(tb, ti, y): per random indices of every bidiagonal and their labels
+    y = ln(sigma / scale) / Y_SCALE from the reference's bisection tree."""
     rng = np.random.default_rng(seed)
     tb = np.repeat(np.arange(bs.nb), per)
     ti = np.concatenate([rng.integers(0, bs.ns[b], per) for b in range(bs.nb)])
     res = bs.bisect(tb, ti)
     ok = (res["status"] >= 0) & (res["sigma"] > 0)
-    X = bs.features(tb[ok], ti[ok])
     y = (np.log(res["sigma"][ok] / bs.scale[tb[ok]]) / Y_SCALE).astype(np.float32)
-    return X, y
+    return tb[ok], ti[ok], y
+
+
+def training_data(bs, per, seed):
+    """Network inputs and labels for per random indices of every bidiagonal."""
+    tb, ti, y = targets(bs, per, seed)
+    return bs.features(tb, ti), y
 
 
 def evaluate(net, bs, b):
@@ -334,9 +365,8 @@ def evaluate(net, bs, b):
     tb = np.full(n, b)
     ti = np.arange(n)
     plain = bs.bisect(tb, ti)
-    lo, hi = net.brackets(bs, tb, ti)
-    br = bs.bisect(tb, ti, lo, hi)
-    _, clo, chi = bs.cell(tb, ti)
+    lo, hi, clo, chi = net.brackets(bs, tb, ti)
+    br = bs.bisect(tb, ti, lo, hi, clo, chi)
     cell = bs.bisect(tb, ti, clo, chi)
     same = all(np.array_equal(plain[k], r[k]) for r in (br, cell) for k in ("sigma", "leaf_lo", "leaf_hi"))
     cp = plain["iters"].sum()

@@ -299,13 +299,14 @@ int exc_bisection_bound(int n, const double* d_alpha, const double* d_beta, doub
 /* One-sided bidiagonalization of an m x c matrix F that one caller holds whole (no peers): the
  * steps -1 .. m-1 of exc_ralha_step with every all-reduce the identity (client.cpp:829-910 for
  * one peer), then lower to upper (client.cpp:947-954) into d_alpha, d_beta, d_cs. The loop kernel
- * runs in one launch when its grid alone fits on the device and h_link (exc_ralha_link_alloc) is
- * given, else one launch per step back to back on the stream. Layouts as exc_ralha_step. The
- * mixed-precision form uses it for the m x m factor every peer shares. */
+ * runs in one launch when `colocated` copies of its grid (the callers that may run this loop on
+ * the device at the same time, >= 1) fit on the device together and h_link (exc_ralha_link_alloc)
+ * is given, else one launch per step back to back on the stream; both give the same bits. Layouts
+ * as exc_ralha_step. The mixed-precision form uses it for the m x m factor every peer shares. */
 size_t exc_bidiag_local_workspace(int m, int c);
 int exc_bidiag_local(int m, int c, double* d_fc, double* d_uc, double* d_house, double* d_tau, double* d_alpha,
-                     double* d_beta, double* d_cs, void* h_link, void* d_work, size_t work_bytes, double timeout_s,
-                     exc_stream_t stream);
+                     double* d_beta, double* d_cs, void* h_link, int colocated, void* d_work, size_t work_bytes,
+                     double timeout_s, exc_stream_t stream);
 
 /* ---------------------------------------------------------------- mixed precision (FP32 bulk) */
 
@@ -316,9 +317,16 @@ int exc_convert_f32_f64(long rows, long cols, const float* d_src, long lds, doub
                         exc_stream_t stream);
 /* d_out[0] = trace(A); A += value * I; d_out[0] = max |A - I| and d_out[1] = ||A - I||_F
  * (n x n, two doubles out). */
+/* The lower triangle of an n x n row-major matrix packed by rows (n (n + 1) / 2 values: row i
+ * holds columns 0 .. i), and back: unpack writes the triangle and its mirror image. */
+int exc_pack_lower_f32(int n, const float* d_a, long lda, float* d_packed, exc_stream_t stream);
+int exc_pack_lower_f64(int n, const double* d_a, long lda, double* d_packed, exc_stream_t stream);
+int exc_unpack_lower_f64(int n, const double* d_packed, double* d_a, long lda, exc_stream_t stream);
 int exc_trace(int n, const double* d_a, long lda, double* d_out, exc_stream_t stream);
 int exc_add_diag(int n, double* d_a, long lda, double value, exc_stream_t stream);
-int exc_dev_identity(int n, const double* d_a, long lda, double* d_out, exc_stream_t stream);
+size_t exc_dev_identity_workspace(int n);
+int exc_dev_identity(int n, const double* d_a, long lda, double* d_out, void* d_work, size_t work_bytes,
+                     exc_stream_t stream);
 /* FP64 Ogita-Aishima refinement of a square SVD L ~ U diag(sigma) W^T (all m x m row-major,
  * L lower triangular when l_lower): `iters` steps of U <- U (I + F), W <- W (I + G) from the
  * residual products I - U^T U, I - W^T W and U^T L W on the MFMA GEMM; sigma receives
@@ -374,14 +382,15 @@ int exc_count_features(int nb, int K, const int* d_n, const long* d_off, const d
  * d_tb[j]. With brackets (d_blo and d_bhi both given) each end inside the root interval is checked
  * by one exact Sturm count and becomes a lower or an upper bound of the value (an end the value
  * does not lie above bounds it from above); the tree nodes whose midpoints lie beyond a bound are
- * decided without a count. d_iters[j] = Sturm counts evaluated; d_status[j] = 1 bounded on both
- * sides, 2 on one side, 0 from the root, -1 the root does not hold the value. Targets whose
- * values share a leaf (closer than its width) get the same leaf; one value per distinct leaf, in
- * order, is the reference's list. */
+ * decided without a count. Fallback points (d_flo and d_fhi both given, or both NULL) are counted
+ * only for a side the bracket left open. d_iters[j] = Sturm counts evaluated; d_status[j] = 1
+ * bounded on both sides, 2 on one side, 0 from the root, -1 the root does not hold the value.
+ * Targets whose values share a leaf (closer than its width) get the same leaf; one value per
+ * distinct leaf, in order, is the reference's list. */
 int exc_bisect_index(int ntg, const int* d_n, const long* d_off, const double* d_d, const double* d_e,
                      const double* d_root, double tol, const int* d_tb, const int* d_ti, const double* d_blo,
-                     const double* d_bhi, double* d_sigma, double* d_leaf_lo, double* d_leaf_hi, int* d_iters,
-                     int* d_status, exc_stream_t stream);
+                     const double* d_bhi, const double* d_flo, const double* d_fhi, double* d_sigma,
+                     double* d_leaf_lo, double* d_leaf_hi, int* d_iters, int* d_status, exc_stream_t stream);
 /* The bracket network: in 32 -> 256 -> 256 -> 2, ReLU, parameters in one FP32 array of
  * exc_mlp_param_count() floats (W1 32x256, b1, W2 256x256, b2, W3 256x2, b3; row-major, inputs
  * by rows). Outputs (c, w): the target is c with Laplace scale exp(w). Training in FP32 on the

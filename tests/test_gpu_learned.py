@@ -128,6 +128,12 @@ def test_bisect_index_matches_oracle_bisection(L, oracle):
         assert np.all(br["status"][~bad & ~inv] == 1)
         assert br["iters"][br["status"] == 1].mean() < plain["iters"].mean()
         assert br["iters"][br["status"] == 2].mean() < plain["iters"].mean()
+        # the enclosing feature shifts as fallback points close the side a missed bracket leaves open
+        _, clo, chi = bs.cell(tb, ti)
+        fb = bs.bisect(tb, ti, lo, hi, clo, chi)
+        assert np.array_equal(fb["sigma"], s) and np.array_equal(fb["leaf_lo"], plain["leaf_lo"])
+        assert np.mean(fb["status"] == 1) > np.mean(br["status"] == 1)
+        assert fb["iters"].sum() < br["iters"].sum()
 
 
 def test_learned_brackets_save_counts(L, hip, oracle):
@@ -142,7 +148,7 @@ def test_learned_brackets_save_counts(L, hip, oracle):
     hist = net.train(X, y, steps=1500, batch=4096, lr=2e-3, seed=1, log=banner)
     assert hist["loss"][-1][1] < hist["loss"][0][1]
     held = L.make_set(200, (512, 4096), seed=12)
-    k_lo, k_hi = net.calibrate(*L.training_data(held, per=48, seed=13))
+    k_lo, k_hi = net.calibrate(held, *L.targets(held, per=48, seed=13))
     banner(f"calibrated bracket ends: {k_lo:.2f} and {k_hi:.2f} Laplace scales")
     test = L.make_set(4, (16384, 16384), seed=20)
     saved = []

@@ -209,3 +209,44 @@ def test_mixed_config2_full_size(hip):
     banner(f"peers done in {time.time() - t0:.1f} s: {[p['result']['total'] for p in res['peers']]}")
     assert res["peers"][0]["result"]["sigma_count"] == 1000
     assert_bars(measures(res, xs, s))
+
+
+@pytest.mark.parametrize("n", [1, 2, 7, 300, 2001])
+def test_pack_unpack_lower(hip, n):
+    """The triangle packing the mixed form's Gram exchange uses: exc_pack_lower_f32/f64 give the
+    rows of the lower triangle back to back, exc_unpack_lower_f64 the symmetric matrix."""
+    rng = np.random.default_rng(n)
+    A = rng.standard_normal((n, n))
+    want = np.concatenate([A[i, :i + 1] for i in range(n)])
+    dA = hip.DeviceArray.from_host(A)
+    t = n * (n + 1) // 2
+    p64 = hip.DeviceArray((t,))
+    hip.call("exc_pack_lower_f64", n, dA.ptr, n, p64.ptr, None)
+    got = p64.to_host()
+    assert np.array_equal(got, want)
+    A32 = A.astype(np.float32)
+    dA32 = hip.DeviceArray.from_host(A32)
+    p32 = hip.DeviceArray((t,), np.float32)
+    hip.call("exc_pack_lower_f32", n, dA32.ptr, n, p32.ptr, None)
+    assert np.array_equal(p32.to_host(), want.astype(np.float32))
+    out = hip.DeviceArray((n, n)).zero()
+    hip.call("exc_unpack_lower_f64", n, p64.ptr, out.ptr, n, None)
+    L = np.tril(A)
+    assert np.array_equal(out.to_host(), L + np.tril(A, -1).T)
+
+
+@pytest.mark.parametrize("n", [1, 5, 300, 1500])
+def test_trace_and_deviation(hip, n):
+    """exc_trace (diagonal sum) and exc_dev_identity (max |A - I| and ||A - I||_F) against numpy."""
+    rng = np.random.default_rng(n + 1)
+    A = np.eye(n) + 1e-3 * rng.standard_normal((n, n))
+    dA = hip.DeviceArray.from_host(A)
+    out = hip.DeviceArray((2,)).zero()
+    hip.call("exc_trace", n, dA.ptr, n, out.ptr, None)
+    assert np.isclose(out.to_host()[0], np.trace(A), rtol=1e-14, atol=0)
+    ws = hip.workspace(hip.lib.exc_dev_identity_workspace(n))
+    hip.call("exc_dev_identity", n, dA.ptr, n, out.ptr, ws.ptr, ws.nbytes, None)
+    got = out.to_host()
+    E = A - np.eye(n)
+    assert got[0] == np.abs(E).max()
+    assert np.isclose(got[1], np.linalg.norm(E), rtol=1e-13, atol=0)
