@@ -8,6 +8,7 @@
 #include "exc_common.hpp"
 #include "sturm.hpp"
 
+#include <mutex>
 
 #pragma clang fp contract(off)
 
@@ -764,6 +765,52 @@ int exc_bidiag_svd(int k, const double* d_alpha, const double* d_beta, double* d
                      d_sigma, d_u, d_v);
   EXC_CHECK_HIP(hipGetLastError());
   return EXC_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- allocation-free variant
+// Scratch of exc_bidiag_svd_noalloc: one copy per device, alive until the process ends. Host code
+// never passes `scratch` itself to a kernel (in host code the name is only a stand-in): it takes
+// the device address once per device with hipGetSymbolAddress and uses that.
+namespace {
+constexpr size_t kNoAllocScratchBytes = (size_t)48 << 20;
+constexpr int kNoAllocMaxDevices = 64;
+static __device__ __align__(256) char scratch[kNoAllocScratchBytes];
+void* g_scratch_ptr[kNoAllocMaxDevices] = {nullptr};
+std::mutex g_scratch_mu;
+
+int scratch_for_current_device(void** out) {
+  int dev = 0;
+  EXC_CHECK_HIP(hipGetDevice(&dev));
+  EXC_REQUIRE(dev >= 0 && dev < kNoAllocMaxDevices, EXC_ERR_ARG, "exc_bidiag_svd_noalloc: device index too large");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    EXC_CHECK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    EXC_REQUIRE(p != nullptr, EXC_ERR_HIP, "exc_bidiag_svd_noalloc: no device address for the scratch array");
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return EXC_OK;
+}
+}  // namespace
+
+extern "C" {
+
+size_t exc_bidiag_svd_noalloc_capacity(void) { return kNoAllocScratchBytes; }
+
+int exc_bidiag_svd_noalloc(int k, const double* d_alpha, const double* d_beta, double* d_u, double* d_v,
+                           double* d_sigma, int top_k, int* d_twists, exc_stream_t stream) {
+  EXC_REQUIRE(k >= 1 && top_k >= 0 && top_k <= k, EXC_ERR_ARG, "exc_bidiag_svd_noalloc: bad arguments");
+  const size_t need = exc_bidiag_svd_workspace(k, top_k);
+  EXC_REQUIRE(need <= kNoAllocScratchBytes, EXC_ERR_SPACE,
+              "exc_bidiag_svd_noalloc: k and top_k need more scratch than the fixed 48 MiB per device "
+              "(k = top_k = 1024 fits); use exc_bidiag_svd with a workspace");
+  void* p = nullptr;
+  const int rc = scratch_for_current_device(&p);
+  if (rc) return rc;
+  return exc_bidiag_svd(k, d_alpha, d_beta, d_u, d_v, d_sigma, top_k, d_twists, p, kNoAllocScratchBytes, stream);
 }
 
 }  // extern "C"

@@ -104,6 +104,8 @@ SIGNATURES = {
     "exc_reflector_message_len": (_L, [_L, _L, _L]),
     "exc_pack_reflectors": (_I, [_L, _L, _L, _dp, _L, _dp, _dp, _vp]),
     "exc_unpack_reflectors": (_I, [_L, _L, _L, _dp, _dp, _L, _dp, _vp]),
+    "exc_bidiag_svd_noalloc_capacity": (_S, []),
+    "exc_bidiag_svd_noalloc": (_I, [_I, _dp, _dp, _dp, _dp, _dp, _I, _dp, _vp]),
     "exc_bidiag_local_workspace": (_S, [_I, _I]),
     "exc_bidiag_local": (_I, [_I, _I, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _vp, _I, _dp, _S, _D, _vp]),
     "exc_pack_lower_f32": (_I, [_I, _vp, _L, _vp, _vp]),

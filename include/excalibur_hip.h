@@ -95,6 +95,19 @@ size_t exc_bidiag_svd_workspace(int k, int top_k);
 int exc_bidiag_svd(int k, const double* d_alpha, const double* d_beta, double* d_u, double* d_v, double* d_sigma,
                    int top_k, int* d_twists, void* d_work, size_t work_bytes, exc_stream_t stream);
 
+/* exc_bidiag_svd without a workspace argument and without any allocation: for callers that cannot
+ * allocate device memory (code under stream capture, tight serving loops, harnesses that forbid
+ * hipMalloc). No counterpart in the reference. The scratch is a fixed device array of
+ * exc_bidiag_svd_noalloc_capacity() bytes (48 MiB) per device, held until the process ends; its
+ * device address is looked up on the first call on each device. Sizes whose workspace
+ * (exc_bidiag_svd_workspace(k, top_k)) exceeds it are rejected with EXC_ERR_SPACE (k = top_k =
+ * 1024 fits). Every call on one device shares that device's array: calls that overlap on one
+ * device (two streams or two threads at once) race and must be serialized by the caller. Results
+ * are those of exc_bidiag_svd bit for bit. Synchronizes the stream. */
+size_t exc_bidiag_svd_noalloc_capacity(void);
+int exc_bidiag_svd_noalloc(int k, const double* d_alpha, const double* d_beta, double* d_u, double* d_v,
+                           double* d_sigma, int top_k, int* d_twists, exc_stream_t stream);
+
 /* ---------------------------------------------------------------- Givens masking */
 
 /* Host-side plan of givens_mask_cache_optimized_parallel_with_simd (bbtsvd.cpp:727-779): the
