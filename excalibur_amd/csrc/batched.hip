@@ -21,13 +21,15 @@
 #include <vector>
 
 #include "exc_common.hpp"
+#include "fact.hpp"
 #include "gemm.hpp"
 
 namespace {
 
 // Lower Cholesky of A_b + shift_b I for every block, in place (upper triangle zeroed); info_b = 0
 // or 1 + the index of the first non-positive pivot. One workgroup per block: column k is scaled,
-// staged in LDS, and each thread updates whole rows of the trailing lower triangle.
+// staged in LDS, and each thread updates whole rows of the trailing lower triangle (orders above
+// the tile kernels' limit; below it the tile form factors all blocks in one launch).
 __global__ __launch_bounds__(256) void k_potrf_batched(int n, double* const* __restrict__ As, long lda,
                                                        const double* __restrict__ shift, int* __restrict__ info) {
   extern __shared__ double col[];
@@ -66,7 +68,16 @@ __global__ __launch_bounds__(256) void k_potrf_batched(int n, double* const* __r
   }
 }
 
-// X_b = L_b^-1 (L_b lower, n x n) by forward substitution, one thread per column of X.
+// A_b += shift_b I for every block.
+__global__ __launch_bounds__(256) void k_add_shift_batched(int n, double* const* __restrict__ As, long lda,
+                                                           const double* __restrict__ shift) {
+  double* A = As[blockIdx.y];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) A[(long)i * lda + i] += shift[blockIdx.y];
+}
+
+// X_b = L_b^-1 (L_b lower, n x n) by forward substitution, one thread per column of X (orders
+// above the tile kernels' limit).
 __global__ __launch_bounds__(256) void k_trinv_batched(int n, const double* const* __restrict__ Ls, long ldl,
                                                        double* const* __restrict__ Xs, long ldx) {
   const double* L = Ls[blockIdx.y];
@@ -147,6 +158,7 @@ size_t exc_batched_local_qr_workspace(int nb, int m, const long* h_cols) {
   a.take<void*>(8 * (size_t)nb);            // pointer tables
   a.take<double>(2 * (size_t)nb + 2 * nb);  // shifts, stats
   a.take<int>(nb);
+  a.take<char>(fact::tiles_batched_work_bytes(m, nb));  // tile flags
   std::vector<ExcGemmProblem> probs(nb);
   for (int b = 0; b < nb; b++) {
     probs[b].M = m;
@@ -196,6 +208,8 @@ int exc_batched_local_qr(int nb, int m, const long* h_cols, const double* const*
   double* shift = a.take<double>(2 * (size_t)nb);
   double* stats = a.take<double>(2 * (size_t)nb);
   int* info = a.take<int>(nb);
+  const size_t tfb = fact::tiles_batched_work_bytes(m, nb);
+  void* tflags = a.take<char>(tfb);
   const size_t gws = work_bytes - a.used - 256;
   void* gwork = a.take<char>(gws);
   // pointer tables: 0 G, 1 L1, 2 L3, 3 S, 4 g32, 5 Z, 6 rt, 7 qt
@@ -216,6 +230,30 @@ int exc_batched_local_qr(int nb, int m, const long* h_cols, const double* const*
   double* const* tL3 = (double* const*)(tab + 2 * nb);
   double* const* tS = (double* const*)(tab + 3 * nb);
   const float* const* tg32 = (const float* const*)(tab + 4 * nb);
+  const size_t lds = (size_t)m * sizeof(double);
+  // Cholesky of every block (shifted when `sh` is given) and the inverses of lower factors: the
+  // tile forms (all blocks in one launch) up to their order limit, one workgroup per block above
+  auto chol = [&](double* const* tA, const double* sh) -> int {
+    if (m <= fact::kTileMaxN) {
+      if (sh) {
+        hipLaunchKernelGGL(k_add_shift_batched, dim3(exc_detail::ceil_div(m, 256), nb), dim3(256), 0, s, m, tA,
+                           (long)m, sh);
+        EXC_CHECK_HIP(hipGetLastError());
+      }
+      return fact::potrf_tiles_batched(m, tA, nb, (long)m, info, tflags, tfb, s);
+    }
+    hipLaunchKernelGGL(k_potrf_batched, dim3(nb), dim3(256), lds, s, m, tA, (long)m, sh, info);
+    EXC_CHECK_HIP(hipGetLastError());
+    return EXC_OK;
+  };
+  auto trinv = [&](double* const* tL, double* const* tX) -> int {
+    if (m <= fact::kTileMaxN)
+      return fact::trinv_tiles_batched(m, (const double* const*)tL, (long)m, tX, (long)m, nb, tflags, tfb, s);
+    hipLaunchKernelGGL(k_trinv_batched, dim3(exc_detail::ceil_div(m, 256), nb), dim3(256), 0, s, m,
+                       (const double* const*)tL, (long)m, tX, (long)m);
+    EXC_CHECK_HIP(hipGetLastError());
+    return EXC_OK;
+  };
   // ---- pass 1 (FP32)
   for (int b = 0; b < nb; b++) {
     const long n = (long)m * h_cols[b];
@@ -248,13 +286,12 @@ int exc_batched_local_qr(int nb, int m, const long* h_cols, const double* const*
   EXC_CHECK_HIP(hipStreamSynchronize(s));
   const double u32 = std::ldexp(1.0, -24);
   for (int b = 0; b < nb; b++) sh[b] = u32 * hs[2 * b];
-  const size_t lds = (size_t)m * sizeof(double);
   std::vector<int> hi(nb, 1);
   for (int attempt = 0; attempt < 60; attempt++) {
     EXC_CHECK_HIP(hipMemcpyAsync(L1, G, mm * nb * sizeof(double), hipMemcpyDeviceToDevice, s));
     EXC_CHECK_HIP(hipMemcpyAsync(shift, sh.data(), nb * sizeof(double), hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(k_potrf_batched, dim3(nb), dim3(256), lds, s, m, tL1, (long)m, shift, info);
-    EXC_CHECK_HIP(hipGetLastError());
+    rc = chol(tL1, shift);
+    if (rc) return rc;
     EXC_CHECK_HIP(hipMemcpyAsync(hi.data(), info, nb * sizeof(int), hipMemcpyDeviceToHost, s));
     EXC_CHECK_HIP(hipStreamSynchronize(s));
     bool all = true;
@@ -269,9 +306,8 @@ int exc_batched_local_qr(int nb, int m, const long* h_cols, const double* const*
   if (h_stats)
     for (int b = 0; b < nb; b++) h_stats[3 * b] = sh[b];
   // ---- FP64 passes: Z = Linv X (grouped), G3 = Z Z^T (grouped), L3 = chol(G3)
-  hipLaunchKernelGGL(k_trinv_batched, dim3(exc_detail::ceil_div(m, 256), nb), dim3(256), 0, s, m,
-                     (const double* const*)tL1, (long)m, tS, (long)m);
-  EXC_CHECK_HIP(hipGetLastError());
+  rc = trinv(tL1, tS);
+  if (rc) return rc;
   for (int b = 0; b < nb; b++) {
     ExcGemmProblem& q = P[b];
     q = ExcGemmProblem{};
@@ -309,8 +345,8 @@ int exc_batched_local_qr(int nb, int m, const long* h_cols, const double* const*
     if (rc) return rc;
     hipLaunchKernelGGL(k_stats_batched, dim3(nb), dim3(256), 0, s, m, (const double* const*)tG, (long)m, stats);
     EXC_CHECK_HIP(hipMemcpyAsync(L3, G, mm * nb * sizeof(double), hipMemcpyDeviceToDevice, s));
-    hipLaunchKernelGGL(k_potrf_batched, dim3(nb), dim3(256), lds, s, m, tL3, (long)m, (const double*)nullptr, info);
-    EXC_CHECK_HIP(hipGetLastError());
+    rc = chol(tL3, nullptr);
+    if (rc) return rc;
     EXC_CHECK_HIP(hipMemcpyAsync(hs.data(), stats, hs.size() * sizeof(double), hipMemcpyDeviceToHost, s));
     EXC_CHECK_HIP(hipMemcpyAsync(hi.data(), info, nb * sizeof(int), hipMemcpyDeviceToHost, s));
     EXC_CHECK_HIP(hipStreamSynchronize(s));
@@ -322,9 +358,8 @@ int exc_batched_local_qr(int nb, int m, const long* h_cols, const double* const*
     }
     if (!again || pass == 2) break;
     // another pass for all blocks (cheap for the well-conditioned ones): Z <- L3^-1 Z, L1 <- L1 L3
-    hipLaunchKernelGGL(k_trinv_batched, dim3(exc_detail::ceil_div(m, 256), nb), dim3(256), 0, s, m,
-                       (const double* const*)tL3, (long)m, tS, (long)m);
-    EXC_CHECK_HIP(hipGetLastError());
+    rc = trinv(tL3, tS);
+    if (rc) return rc;
     for (int b = 0; b < nb; b++) {
       ExcGemmProblem& q = P[b];
       q = ExcGemmProblem{};
@@ -380,9 +415,8 @@ int exc_batched_local_qr(int nb, int m, const long* h_cols, const double* const*
   rc = exc_gemm_grouped_opts(8, 0, 0, nb, P.data(), 1.0, 0.0, 0, 0, gwork, gws, s);
   if (rc) return rc;
   if (h_qt) {  // Q^T = L3^-1 Z
-    hipLaunchKernelGGL(k_trinv_batched, dim3(exc_detail::ceil_div(m, 256), nb), dim3(256), 0, s, m,
-                       (const double* const*)tL3, (long)m, tS, (long)m);
-    EXC_CHECK_HIP(hipGetLastError());
+    rc = trinv(tL3, tS);
+    if (rc) return rc;
     for (int b = 0; b < nb; b++) {
       ExcGemmProblem& q = P[b];
       q = ExcGemmProblem{};

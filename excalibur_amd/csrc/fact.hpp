@@ -40,5 +40,11 @@ constexpr int kTileMaxN = 4096;
 size_t tiles_work_bytes(int n);
 int potrf_tiles(int n, double* A, long lda, int* info, void* work, size_t work_bytes, hipStream_t s);
 int lu_tiles(int n, double* A, long lda, double* sgn, int square_last, void* work, size_t work_bytes, hipStream_t s);
+// Batched tile forms over device arrays of matrix addresses (n <= kTileMaxN), one launch each.
+size_t tiles_batched_work_bytes(int n, int count);
+int potrf_tiles_batched(int n, double* const* d_As, int count, long lda, int* d_info, void* work, size_t work_bytes,
+                        hipStream_t s);
+int trinv_tiles_batched(int n, const double* const* d_Ls, long ldl, double* const* d_Xs, long ldx, int count,
+                        void* work, size_t work_bytes, hipStream_t s);
 
 }  // namespace fact

@@ -11,6 +11,9 @@
 // on tiles with smaller numbers, and a compute unit's workgroups are dispatched in increasing
 // order, so the lowest unfinished workgroup is always resident and never waits. Every wait is
 // also bounded: after a long spin the kernel records a timeout and all workgroups exit.
+#include <algorithm>
+#include <vector>
+
 #include "exc_common.hpp"
 #include "fact.hpp"
 #include "solve.hpp"
@@ -250,24 +253,28 @@ __device__ __forceinline__ void getf2_reg(double* S, double* xr, int kb, double*
 // Off-diagonal tile solves against the final diagonal tile, read from HBM with wave-uniform
 // addresses (the flag's acquire made it visible): ROWS: lane = row r of S, S[r] <- S[r] op(D)^-1
 // (LOWER_T: op(D) = L^T, Cholesky; else U, LU); else lane = column c, S[:, c] <- L^-1 S[:, c]
-// with L the unit lower part of D (LU).
-template <int MODE>  // 0: rows, L^T; 1: rows, U; 2: columns, unit L
+// with L the unit lower part of D (LU) or (MODE 3) the lower triangle of the tile the caller
+// staged in T (triangular inverse).
+template <int MODE>  // 0: rows, L^T; 1: rows, U; 2: columns, unit L; 3: columns, L
 __device__ __forceinline__ void tile_solve(double* S, double* T, const double* Dg, long lda, int kd) {
   // The diagonal tile was finished by another workgroup. The LU modes copy it into LDS (T) with
   // one batched coherent copy and substitute from there; the Cholesky mode, whose substitution
   // walks rows of L^T, is faster reading it through the caches after one acquire fence.
-  if (MODE != 0) {
+  if (MODE == 1 || MODE == 2) {
     tile_to_lds(T, Dg, lda, 0, 0, kd, kd);
     __syncthreads();
-  } else {
+  } else if (MODE == 0) {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-  }
+  }  // MODE 3: the caller staged the diagonal tile in T
   __asm__ volatile("" ::: "memory");
   const int lane = threadIdx.x;
   double x[TB];
 #pragma unroll
-  for (int k = 0; k < TB; k++) x[k] = MODE == 2 ? S[k * LD + lane] : S[lane * LD + k];
-  if (MODE == 0) {
+  for (int k = 0; k < TB; k++) x[k] = MODE >= 2 ? S[k * LD + lane] : S[lane * LD + k];
+  if (MODE == 3) {
+    if (kd == TB) fact::left_solve<false, false, true>(kd, T, LD, x);
+    else fact::left_solve<false, false, false>(kd, T, LD, x);
+  } else if (MODE == 0) {
     if (kd == TB) fact::right_solve<true, false, true>(kd, Dg, lda, x);
     else fact::right_solve<true, false, false>(kd, Dg, lda, x);
   } else if (MODE == 1) {
@@ -279,7 +286,7 @@ __device__ __forceinline__ void tile_solve(double* S, double* T, const double* D
   }
 #pragma unroll
   for (int k = 0; k < TB; k++) {
-    if (MODE == 2) S[k * LD + lane] = x[k];
+    if (MODE >= 2) S[k * LD + lane] = x[k];
     else S[lane * LD + k] = x[k];
   }
 }
@@ -304,15 +311,17 @@ __device__ void chol_tile(int t, int nb, int* i, int* j) {
 }
 __host__ __device__ inline int chol_index(int i, int j, int nb) { return j * nb - j * (j - 1) / 2 + (i - j); }
 
-__global__ __launch_bounds__(TNT) void k_potrf_tiles(int n, double* __restrict__ A, long lda, int nb,
-                                                     int* __restrict__ done, int* __restrict__ error) {
+// Tile t of the lower triangle of A's Cholesky factor (t in column-major tile order; every tile
+// waits only on tiles of lower index, so in-order dispatch cannot deadlock).
+__device__ __forceinline__ void potrf_tile_run(int n, double* __restrict__ A, long lda, int nb, int* __restrict__ done,
+                                               int* __restrict__ error, int t) {
   extern __shared__ double smem[];  // S, X, Y: three 64 x 65 tiles
   double* S = smem;
   double* X = smem + TB * LD;
   double* Y = smem + 2 * TB * LD;
   __shared__ int sh[1];
   int ti, tj;
-  chol_tile(blockIdx.x, nb, &ti, &tj);
+  chol_tile(t, nb, &ti, &tj);
   const long r0 = (long)ti * TB, c0 = (long)tj * TB;
   const int nr = min(TB, n - (int)r0), nc = min(TB, n - (int)c0);
   const Quad q;
@@ -343,6 +352,66 @@ __global__ __launch_bounds__(TNT) void k_potrf_tiles(int n, double* __restrict__
     __syncthreads();
     lds_to_tile(S, A, lda, r0, c0, nr, nc, false);
   }
+  signal_flag(&done[t]);
+}
+
+__global__ __launch_bounds__(TNT) void k_potrf_tiles(int n, double* __restrict__ A, long lda, int nb,
+                                                     int* __restrict__ done, int* __restrict__ error) {
+  potrf_tile_run(n, A, lda, nb, done, error, blockIdx.x);
+}
+
+// Matrix blockIdx.y of the batch: its own flags (nb * nb) and error pair.
+__global__ __launch_bounds__(TNT) void k_potrf_tiles_batched(int n, double* const* __restrict__ As, long lda, int nb,
+                                                             int* __restrict__ done, int* __restrict__ error) {
+  const int b = blockIdx.y;
+  potrf_tile_run(n, As[b], lda, nb, done + (long)b * nb * nb, error + 2 * b, blockIdx.x);
+}
+
+// X = L^-1 by tiles (both lower, n x n): tile (i, j), i >= j, in the column-major order of the
+// Cholesky tiles, is X(i, j) = L(i, i)^-1 (delta_ij I - sum_{k=j}^{i-1} L(i, k) X(k, j)); it waits
+// only on the tiles X(k, j) above it in its column. The upper triangle of X is not written.
+__global__ __launch_bounds__(TNT) void k_trinv_tiles_batched(int n, const double* const* __restrict__ Ls, long ldl,
+                                                             double* const* __restrict__ Xs, long ldx, int nb,
+                                                             int* __restrict__ done_all, int* __restrict__ error_all) {
+  extern __shared__ double smem[];
+  double* S = smem;
+  double* P = smem + TB * LD;
+  double* Q = smem + 2 * TB * LD;
+  __shared__ int sh[1];
+  const int b = blockIdx.y;
+  const double* L = Ls[b];
+  double* Xo = Xs[b];
+  int* done = done_all + (long)b * nb * nb;
+  int* error = error_all + 2 * b;
+  int ti, tj;
+  chol_tile(blockIdx.x, nb, &ti, &tj);
+  const long r0 = (long)ti * TB, c0 = (long)tj * TB;
+  const int nr = min(TB, n - (int)r0), nc = min(TB, n - (int)c0);
+  const Quad q;
+  d4 acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int bb = 0; bb < 4; bb++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) acc[a][bb][v] = (ti == tj && q.row(a, v) == q.col(bb)) ? 1.0 : 0.0;
+  for (int k = tj; k < ti; k++) {
+    if (!wait_flag(&done[chol_index(k, tj, nb)], error, sh)) return;
+    const int kc = min(TB, n - k * TB);
+    tile_to_lds(P, L, ldl, r0, (long)k * TB, nr, kc);   // L(i, k)
+    tile_to_lds(Q, Xo, ldx, (long)k * TB, c0, kc, nc);  // X(k, j), finished by another workgroup
+    __syncthreads();
+    acc_update<false>(acc, q, P, Q);
+    __syncthreads();
+  }
+  acc_to_lds(acc, q, S);
+  tile_to_lds(P, L, ldl, r0, r0, nr, nr);  // L(i, i)
+  __syncthreads();
+  if (nr < TB) pad_identity(P, nr);
+  __syncthreads();
+  tile_solve<3>(S, P, nullptr, 0, TB);
+  __syncthreads();
+  lds_to_tile(S, Xo, ldx, r0, c0, nr, nc, ti == tj);
   signal_flag(&done[blockIdx.x]);
 }
 
@@ -423,6 +492,20 @@ __global__ __launch_bounds__(256) void k_zero_upper_blocks(int n, double* A, lon
   if (j > i) A[i * lda + j] = 0.0;
 }
 
+__global__ __launch_bounds__(256) void k_zero_upper_batched(int n, double* const* As, long lda) {
+  double* A = As[blockIdx.y];
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < (long)n * n; e += (long)gridDim.x * blockDim.x) {
+    const long i = e / n, j = e - i * n;
+    if (j > i) A[i * lda + j] = 0.0;
+  }
+}
+
+// info[b] = error[2 b + 1] (the first failing pivot of matrix b, 0 when none)
+__global__ void k_tile_infos(int count, const int* __restrict__ error, int* __restrict__ info) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < count) info[b] = error[2 * b + 1];
+}
+
 constexpr size_t kTileLds = 3 * TB * LD * sizeof(double);
 
 }  // namespace
@@ -456,6 +539,75 @@ int potrf_tiles(int n, double* A, long lda, int* info, void* work, size_t wb, hi
   EXC_REQUIRE(herr[0] == 0, EXC_ERR_NUMERIC, "potrf: a tile wait timed out");
   if (info && herr[1]) EXC_CHECK_HIP(hipMemcpyAsync(info, &herr[1], sizeof(int), hipMemcpyHostToDevice, s));
   EXC_CHECK_HIP(hipStreamSynchronize(s));
+  return EXC_OK;
+}
+
+size_t tiles_batched_work_bytes(int n, int count) {
+  const int nb = (n + TB - 1) / TB;
+  return ((size_t)count * nb * nb + 2 * (size_t)count + 8) * sizeof(int);
+}
+
+// The Cholesky factors (lower, in place, upper triangle zeroed) of `count` matrices whose device
+// addresses d_As (a device array) lists, in one launch; d_info[b] = 0 or 1 + the row of the first
+// non-positive pivot of matrix b. Synchronizes the stream to check the bounded waits.
+int potrf_tiles_batched(int n, double* const* d_As, int count, long lda, int* d_info, void* work, size_t wb,
+                        hipStream_t s) {
+  EXC_REQUIRE(n >= 1 && count >= 1 && d_As && d_info, EXC_ERR_ARG, "potrf_tiles_batched: bad arguments");
+  EXC_REQUIRE(n <= kTileMaxN, EXC_ERR_ARG, "potrf_tiles_batched: order above the tile limit");
+  EXC_REQUIRE(wb >= tiles_batched_work_bytes(n, count), EXC_ERR_SPACE, "potrf_tiles_batched: workspace too small");
+  const int nb = (n + TB - 1) / TB;
+  const int T = nb * (nb + 1) / 2;
+  int* done = (int*)work;
+  int* err = done + (long)count * nb * nb;
+  static bool attr = false;
+  if (!attr) {
+    EXC_CHECK_HIP(hipFuncSetAttribute((const void*)k_potrf_tiles_batched, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)kTileLds));
+    attr = true;
+  }
+  const long flags = (long)count * nb * nb + 2L * count;
+  hipLaunchKernelGGL(k_zero_ints, dim3(exc_detail::ceil_div(flags, 256)), dim3(256), 0, s, done, (int)flags);
+  hipLaunchKernelGGL(k_potrf_tiles_batched, dim3(T, count), dim3(TNT), kTileLds, s, n, d_As, lda, nb, done, err);
+  hipLaunchKernelGGL(k_zero_upper_batched, dim3(std::min<unsigned>(1024, exc_detail::ceil_div((long)n * n, 256)), count),
+                     dim3(256), 0, s, n, d_As, lda);
+  hipLaunchKernelGGL(k_tile_infos, dim3(exc_detail::ceil_div(count, 256)), dim3(256), 0, s, count, err, d_info);
+  EXC_CHECK_HIP(hipGetLastError());
+  std::vector<int> herr(2 * (size_t)count);
+  EXC_CHECK_HIP(hipMemcpyAsync(herr.data(), err, herr.size() * sizeof(int), hipMemcpyDeviceToHost, s));
+  EXC_CHECK_HIP(hipStreamSynchronize(s));
+  for (int b = 0; b < count; b++) EXC_REQUIRE(herr[2 * b] == 0, EXC_ERR_NUMERIC, "potrf: a tile wait timed out");
+  return EXC_OK;
+}
+
+// X_b = L_b^-1 (lower; the upper triangle of X_b zeroed) for `count` matrices, in one launch.
+// Synchronizes the stream to check the bounded waits.
+int trinv_tiles_batched(int n, const double* const* d_Ls, long ldl, double* const* d_Xs, long ldx, int count, void* work,
+                        size_t wb, hipStream_t s) {
+  EXC_REQUIRE(n >= 1 && count >= 1 && d_Ls && d_Xs, EXC_ERR_ARG, "trinv_tiles_batched: bad arguments");
+  EXC_REQUIRE(n <= kTileMaxN, EXC_ERR_ARG, "trinv_tiles_batched: order above the tile limit");
+  EXC_REQUIRE(wb >= tiles_batched_work_bytes(n, count), EXC_ERR_SPACE, "trinv_tiles_batched: workspace too small");
+  const int nb = (n + TB - 1) / TB;
+  const int T = nb * (nb + 1) / 2;
+  int* done = (int*)work;
+  int* err = done + (long)count * nb * nb;
+  static bool attr = false;
+  if (!attr) {
+    EXC_CHECK_HIP(hipFuncSetAttribute((const void*)k_trinv_tiles_batched, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)kTileLds));
+    attr = true;
+  }
+  const long flags = (long)count * nb * nb + 2L * count;
+  hipLaunchKernelGGL(k_zero_ints, dim3(exc_detail::ceil_div(flags, 256)), dim3(256), 0, s, done, (int)flags);
+  hipLaunchKernelGGL(k_trinv_tiles_batched, dim3(T, count), dim3(TNT), kTileLds, s, n, d_Ls, ldl, d_Xs, ldx, nb, done,
+                     err);
+  hipLaunchKernelGGL(k_zero_upper_batched, dim3(std::min<unsigned>(1024, exc_detail::ceil_div((long)n * n, 256)), count),
+                     dim3(256), 0, s, n, d_Xs, ldx);
+  EXC_CHECK_HIP(hipGetLastError());
+  std::vector<int> herr(2 * (size_t)count);
+  EXC_CHECK_HIP(hipMemcpyAsync(herr.data(), err, herr.size() * sizeof(int), hipMemcpyDeviceToHost, s));
+  EXC_CHECK_HIP(hipStreamSynchronize(s));
+  for (int b = 0; b < count; b++)
+    EXC_REQUIRE(herr[2 * b] == 0, EXC_ERR_NUMERIC, "trinv: a tile wait timed out");
   return EXC_OK;
 }
 
