@@ -361,6 +361,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the 32 batched peers workload")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-bracket workload")
+    ap.add_argument("--only", choices=["batched", "learned"], default=None,
+                    help="run only that workload and print its JSON (for profiling it alone)")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--workdir", default=None)
     ap.add_argument("--prepare", default=None,
@@ -387,6 +389,9 @@ def main():
     if args.isolated_only:
         P, rows, n, _ = CONFIGS[args.config]
         print(json.dumps(isolated_gemms(n, rows)))
+        return
+    if args.only:
+        print(json.dumps(bench_batched() if args.only == "batched" else bench_learned()))
         return
     from excalibur_amd.launch import write_peer_files
 
