@@ -865,14 +865,154 @@ void orc_gemm(int ta, int tb, long M, long N, long K, const double* A, long lda,
   gemm_rm(ta != 0, tb != 0, M, N, K, A, lda, B, ldb, C, ldc);
 }
 
-// ----- whole run() of P peers, short-and-wide branch (m <= global_n), mode 1 (two-sided,
-// bisection + twisted), opt 3 (Givens masks, overlapped Ralha). x[p]: m x local_n[p] row-major
-// (the Client{p}.mat layout). seeds[p]: each peer's CLI seed. Returns a handle, or null when the
-// configuration is outside this branch. -----
+// Bidiagonal SVD and sign rule of every simulated peer on its own copy of (alpha, beta) of order
+// n (client.cpp:963-996). Returns the count kept, or -1 when the bisection fails.
+static int bsvd_all(RunState* S, int P, long n, std::vector<double*>& al, std::vector<double*>& be, int top_k) {
+  int cnt = 0;
+  for (int p = 0; p < P; p++) {
+    double* a = al[p];
+    double* b = be[p];
+    double na = nrm2(n, a, 1), nb = nrm2(n - 1, b, 1);
+    double ub = na * na + nb * nb;
+    int sc = negative_count(n, a, b, ub) - negative_count(n, a, b, 0);
+    if (top_k > 0) sc = std::min(top_k, sc);
+    vec& bu = S->get("bidiag_u", p);
+    vec& bv = S->get("bidiag_v", p);
+    vec& sg = S->get("sigma", p);
+    bu.assign((size_t)n * sc, 0.0);
+    bv.assign((size_t)n * sc, 0.0);
+    sg.assign(sc > 0 ? sc : 1, 0.0);
+    vec tw(2 * (size_t)std::max(sc, 1), 0.0);
+    std::vector<int> twi(2 * (size_t)std::max(sc, 1), 0);
+    int rc = bidiag_svd((int)n, a, b, bu.data(), bv.data(), sg.data(), sc, twi.data());
+    if (rc < 0) return -1;
+    for (size_t q = 0; q < twi.size(); q++) tw[q] = twi[q];
+    S->get("twists", p) = tw;
+    sign_normalize(n, sc, bu.data(), bv.data());
+    cnt = sc;
+  }
+  return cnt;
+}
+
+// The right vectors of the tall-and-skinny combine (client.cpp:1113-1154, taken when
+// m >= global_n): bidiag_u rotated back and multiplied by the reflectors of order n, then this
+// peer's rows npos .. npos + ln - 1 transposed into FinalVT (cnt x ln) and unmasked on the right
+// with the private plan.
+static void tall_final_v(RunState* S, int p, long n, int cnt, long npos, long ln, unsigned priv) {
+  vec bu = S->get("bidiag_u", p);
+  const vec& cs = S->get("cs", p);
+  for (long i = n - 2; i >= 0; i--)
+    drot(cnt, bu.data() + i * cnt, 1, bu.data() + (i + 1) * cnt, 1, cs[2 * i], -cs[2 * i + 1]);
+  ormqr_house_ln(n, cnt, S->get("house", p).data(), S->get("tau", p).data(), bu.data());
+  vec& fv = S->get("FinalVT", p);
+  fv.assign((size_t)cnt * ln, 0.0);
+  for (long a = 0; a < cnt; a++)
+    for (long l = 0; l < ln; l++) fv[a * ln + l] = bu[(npos + l) * cnt + a];
+  block_shift_mask(cnt, ln, 32, fv.data(), false, false, priv);
+}
+
+// ----- the tall-and-skinny branch (m > global_n) of run(): client.cpp:543-569 (masks), 627-678
+// (re-partition, local QR of m_part x global_n), 800-996 with the bidiagonal of order global_n,
+// 998-1154 (combine). x[p] (m * local_n[p] doubles, the Client{p}.mat bytes) is read as a
+// local_n[p] x m row-major block, as the reference reads it in this branch. FinalU is cnt x m
+// row-major, FinalVT cnt x local_n[p]. -----
+static void* run_tall(RunState* S, int P, long m, const long* local_n, const double* const* x, unsigned pub,
+                      const std::vector<unsigned>& priv, long gn, int top_k, Clock::time_point t0) {
+  std::vector<long> npos(P + 1, 0);
+  for (int p = 0; p < P; p++) npos[p + 1] = npos[p] + local_n[p];
+  S->timers["time_init"] = secs(t0, Clock::now());
+  // masks (client.cpp:543-569): local_k = local_n; private plan on the left, public on the right
+  auto ta0 = Clock::now();
+  std::vector<vec> masked(P);
+  for (int p = 0; p < P; p++) {
+    masked[p].assign(x[p], x[p] + local_n[p] * m);
+    block_shift_mask(local_n[p], m, 32, masked[p].data(), true, true, priv[p]);
+    block_shift_mask(local_n[p], m, 32, masked[p].data(), true, false, pub);
+    S->get("masked_x", p) = masked[p];
+  }
+  S->timers["time_apply_mask"] = secs(ta0, Clock::now());
+  // re-partition (client.cpp:627-659): peer c gets columns mpos[c] .. of every block
+  std::vector<long> mp(P, m / P), mpos(P + 1, 0), bl(P);
+  for (int i = 0; i < (int)(m % P); i++) mp[i] += 1;
+  for (int i = 0; i < P; i++) mpos[i + 1] = mpos[i] + mp[i];
+  std::vector<vec> F(P), ured(P);
+  auto tq0 = Clock::now();
+  for (int c = 0; c < P; c++) {
+    vec rep((size_t)gn * mp[c], 0.0);
+    for (int i = 0; i < P; i++)
+      for (long a = 0; a < local_n[i]; a++)
+        for (long b = 0; b < mp[c]; b++) rep[(npos[i] + a) * mp[c] + b] = masked[i][a * m + mpos[c] + b];
+    S->get("repart", c) = rep;
+    bl[c] = std::min(mp[c], gn);
+    if (mp[c] > gn) {  // local QR of the m_part x global_n column-major block (client.cpp:661-676)
+      F[c].assign((size_t)gn * gn, 0.0);
+      ured[c].assign((size_t)gn * mp[c], 0.0);
+      local_qr(mp[c], gn, rep.data(), F[c].data(), ured[c].data());
+      S->get("u_reduction", c) = ured[c];
+    } else {
+      F[c] = rep;
+    }
+    S->get("ralha_in", c) = F[c];
+  }
+  S->timers["time_local_qr"] = secs(tq0, Clock::now());
+  // Ralha of order global_n (client.cpp:800-910) and lower->upper (947-954)
+  auto tb0 = Clock::now();
+  std::vector<double*> Fp(P), al(P), be(P), ho(P), tau(P), Up(P);
+  for (int p = 0; p < P; p++) {
+    Fp[p] = F[p].data();
+    S->get("alpha", p).assign(gn, 0.0); al[p] = S->get("alpha", p).data();
+    S->get("beta", p).assign(gn - 1, 0.0); be[p] = S->get("beta", p).data();
+    S->get("house", p).assign((size_t)gn * gn, 0.0); ho[p] = S->get("house", p).data();
+    S->get("tau", p).assign(gn, 0.0); tau[p] = S->get("tau", p).data();
+    S->get("ralha_u", p).assign((size_t)gn * bl[p], 0.0); Up[p] = S->get("ralha_u", p).data();
+  }
+  ralha(P, gn, bl.data(), Fp.data(), al.data(), be.data(), ho.data(), tau.data(), Up.data());
+  for (int p = 0; p < P; p++) {
+    S->get("cs", p).assign(2 * (gn - 1), 0.0);
+    lower_to_upper(gn, al[p], be[p], S->get("cs", p).data());
+  }
+  S->timers["time_bi_diag"] = secs(tb0, Clock::now());
+  auto ts0 = Clock::now();
+  const int cnt = bsvd_all(S, P, gn, al, be, top_k);
+  if (cnt < 0) { delete S; return nullptr; }
+  S->count = cnt;
+  S->timers["time_bsvd"] = secs(ts0, Clock::now());
+  // combine (client.cpp:998-1154): FinalU cnt x m, the column blocks m_part of every peer
+  auto tc0 = Clock::now();
+  vec finalU((size_t)cnt * m, 0.0);
+  for (int c = 0; c < P; c++) {
+    vec tmp((size_t)cnt * bl[c], 0.0);
+    gemm_rm(false, false, cnt, bl[c], gn, S->get("bidiag_v", c).data(), gn, Up[c], bl[c], tmp.data(), bl[c]);
+    vec part;
+    if (mp[c] > gn) {
+      part.assign((size_t)cnt * mp[c], 0.0);
+      gemm_rm(false, false, cnt, mp[c], gn, tmp.data(), gn, ured[c].data(), mp[c], part.data(), mp[c]);
+    } else {
+      part = tmp;
+    }
+    for (long i = 0; i < cnt; i++)
+      for (long j = 0; j < mp[c]; j++) finalU[i * m + mpos[c] + j] = part[i * mp[c] + j];
+  }
+  block_shift_mask(cnt, m, 32, finalU.data(), false, false, pub);
+  for (int p = 0; p < P; p++) S->get("FinalU", p) = finalU;
+  for (int p = 0; p < P; p++) tall_final_v(S, p, gn, cnt, npos[p], local_n[p], priv[p]);
+  S->timers["time_combine_svd_remove_mask"] = secs(tc0, Clock::now());
+  S->timers["total"] = secs(t0, Clock::now());
+  S->arrays["seeds"].assign(P, vec());
+  for (int p = 0; p < P; p++) S->arrays["seeds"][p] = vec{(double)pub, (double)priv[p]};
+  return S;
+}
+
+// ----- whole run() of P peers, mode 1 (two-sided, bisection + twisted), opt 3 (Givens masks,
+// overlapped Ralha). x[p]: the Client{p}.mat bytes, m x local_n[p] row-major in the
+// short-and-wide branch (m <= global_n; m == global_n takes the tall-and-skinny combine of the
+// right vectors, as the reference does), local_n[p] x m row-major in the tall-and-skinny branch
+// (m > global_n, run_tall). seeds[p]: each peer's CLI seed. Returns a handle, or null when the
+// bisection fails. -----
 void* orc_run(int P, long m, const long* local_n, const double* const* x, const unsigned* seeds, int top_k) {
   long global_n = 0;
   for (int p = 0; p < P; p++) global_n += local_n[p];
-  if (m > global_n || m < 3 || P < 1) return nullptr;
+  if (m < 3 || P < 1 || global_n < 3) return nullptr;
   RunState* S = new RunState();
   S->P = P; S->m = m;
   auto t0 = Clock::now();
@@ -884,6 +1024,7 @@ void* orc_run(int P, long m, const long* local_n, const double* const* x, const 
     if (p == 0) pub = (unsigned)rand();
     priv[p] = (unsigned)rand();
   }
+  if (m > global_n) return run_tall(S, P, m, local_n, x, pub, priv, global_n, top_k, t0);
   const double thr = 5.0;
   std::vector<long> rl(P), rnp(P + 1, 0);
   std::vector<bool> did_lqr(P);
@@ -979,29 +1120,9 @@ void* orc_run(int P, long m, const long* local_n, const double* const* x, const 
 
   // bidiagonal SVD + sign rule (client.cpp:963-996), every peer on its own copy
   auto ts0 = Clock::now();
-  int cnt = 0;
-  for (int p = 0; p < P; p++) {
-    double* a = al[p];
-    double* b = be[p];
-    double na = nrm2(m, a, 1), nb = nrm2(m - 1, b, 1);
-    double ub = na * na + nb * nb;
-    int sc = negative_count(m, a, b, ub) - negative_count(m, a, b, 0);
-    if (top_k > 0) sc = std::min(top_k, sc);
-    vec& bu = S->get("bidiag_u", p);
-    vec& bv = S->get("bidiag_v", p);
-    vec& sg = S->get("sigma", p);
-    bu.assign((size_t)m * sc, 0.0);
-    bv.assign((size_t)m * sc, 0.0);
-    sg.assign(sc > 0 ? sc : 1, 0.0);
-    vec tw(2 * (size_t)std::max(sc, 1), 0.0);
-    std::vector<int> twi(2 * (size_t)std::max(sc, 1), 0);
-    int rc = bidiag_svd((int)m, a, b, bu.data(), bv.data(), sg.data(), sc, twi.data());
-    if (rc < 0) { delete S; return nullptr; }
-    for (size_t q = 0; q < twi.size(); q++) tw[q] = twi[q];
-    S->get("twists", p) = tw;
-    sign_normalize(m, sc, bu.data(), bv.data());
-    cnt = sc;
-  }
+  std::vector<double*> alv(al.begin(), al.end()), bev(be.begin(), be.end());
+  const int cnt = bsvd_all(S, P, m, alv, bev, top_k);
+  if (cnt < 0) { delete S; return nullptr; }
   S->count = cnt;
   S->timers["time_bsvd"] = secs(ts0, Clock::now());
 
@@ -1016,6 +1137,14 @@ void* orc_run(int P, long m, const long* local_n, const double* const* x, const 
   }
   block_shift_mask(m, cnt, 32, finalU.data(), false, true, pub);
   for (int p = 0; p < P; p++) S->get("FinalU", p) = finalU;
+  if (m == global_n) {  // m >= global_n: the tall-and-skinny combine of the right vectors
+    for (int p = 0; p < P; p++) tall_final_v(S, p, m, cnt, rnp[p], local_n[p], priv[p]);
+    S->timers["time_combine_svd_remove_mask"] = secs(tc0, Clock::now());
+    S->timers["total"] = secs(t0, Clock::now());
+    S->arrays["seeds"].assign(P, vec());
+    for (int p = 0; p < P; p++) S->arrays["seeds"][p] = vec{(double)pub, (double)priv[p]};
+    return S;
+  }
   for (int p = 0; p < P; p++) {
     vec bu = S->get("bidiag_u", p);
     const vec& cs = S->get("cs", p);

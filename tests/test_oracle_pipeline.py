@@ -146,3 +146,73 @@ def test_run_small_matches_numpy_svd(oracle, P, m, widths, seed):
     U_bad[:, 0] *= -1
     X = np.hstack(xs)
     assert np.linalg.norm(X - (U_bad * sg) @ VT) > 1e-3 * np.linalg.norm(X)
+
+
+def _tall_problem(m, parts, seed, decay=6.0):
+    """X = U diag(s) V^T (m x global_n, m >= global_n) split by columns. For m > global_n peer p's
+    file holds X_p^T (local_n x m row-major), the layout the reference reads in the
+    tall-and-skinny branch; for m == global_n it holds X_p (m x local_n), as in the
+    short-and-wide branch whose stages that case takes."""
+    rng = np.random.default_rng(seed)
+    n = sum(parts)
+    U = np.linalg.qr(rng.standard_normal((m, n)))[0]
+    V = np.linalg.qr(rng.standard_normal((n, n)))[0]
+    s = 10.0 ** (-decay * np.arange(n) / (n - 1))
+    X = (U * s) @ V.T
+    raw, o = [], 0
+    for ln in parts:
+        if m > n:
+            B = np.ascontiguousarray(X[:, o:o + ln].T)  # local_n x m
+            raw.append(B.reshape(m, ln))                  # the same bytes, shaped as the file's m x local_n
+        else:
+            raw.append(np.ascontiguousarray(X[:, o:o + ln]))
+        o += ln
+    return X, s, raw
+
+
+@pytest.mark.parametrize("m,parts,decay", [(300, [40, 35], 2.0), (301, [50, 50, 27], 2.0), (120, [70, 50], 2.0),
+                                           (150, [60, 30], 2.0), (300, [40, 35], 6.0), (150, [60, 30], 6.0)])
+def test_oracle_run_tall_and_square(oracle, m, parts, decay):
+    """The tall-and-skinny branch (m > global_n: masks on the local_n x m view, re-partition by
+    columns, local QR of the m_part x global_n blocks when m_part > global_n, bidiagonal of order
+    global_n, FinalU k x m) and the square case m == global_n (the short-and-wide stages with the
+    tall combine of the right vectors), pinned by what the mathematics fixes: the exact singular
+    values, X = FinalU^T diag(sigma) [FinalVT_0 ... FinalVT_{P-1}], and orthonormal vectors. The
+    last holds to 1e-12 for a spectrum over two decades; over six, the vectors built from the
+    one-sided bidiagonalization's v_i = f_i / alpha_i lose orthogonality to ~1e-6 (the
+    reference's algorithm, without reorthogonalization), so only sigma and X are held there."""
+    X, s, raw = _tall_problem(m, parts, seed=m + len(parts), decay=decay)
+    n = sum(parts)
+    run = oracle.Run(raw, [7] * len(parts))
+    k = run.count
+    assert k == n
+    sg = run.get("sigma", 0)[:k]
+    assert np.max(np.abs(sg - s)) <= 1e-11 * s[0]
+    FU = run.get("FinalU", 0)
+    U = FU.reshape(k, m).T if m > n else FU.reshape(m, k)
+    VT = np.hstack([run.get("FinalVT", p).reshape(k, ln) for p, ln in enumerate(parts)])
+    assert np.linalg.norm(X - (U * sg) @ VT) <= 1e-13 * np.linalg.norm(X)
+    if decay <= 2.0:
+        assert np.abs(U.T @ U - np.eye(k)).max() <= 1e-12
+        assert np.abs(VT @ VT.T - np.eye(k)).max() <= 1e-12
+    # every peer holds the gathered FinalU; sigma bits agree for two peers, and within an ulp or
+    # two beyond (each peer adds the others' messages to its own first, client.cpp:318-323)
+    for p in range(1, len(parts)):
+        assert np.array_equal(run.get("FinalU", p), FU)
+        if len(parts) == 2:
+            assert np.array_equal(run.get("sigma", p), run.get("sigma", 0))
+        else:
+            assert np.max(np.abs(run.get("sigma", p) - run.get("sigma", 0))) <= 4 * np.finfo(float).eps * s[0]
+
+
+def test_oracle_run_tall_wrong_unmask_is_caught(oracle):
+    """A planted mistake (the peers' files in the short-and-wide layout while m > global_n) gives
+    the SVD of another matrix, which the reconstruction check rejects."""
+    m, parts = 200, [30, 30]
+    X, s, raw = _tall_problem(m, parts, seed=3)
+    wrong = [np.ascontiguousarray(X[:, o:o + ln]) for o, ln in ((0, 30), (30, 30))]
+    run = oracle.Run(wrong, [7, 7])
+    k = run.count
+    U = run.get("FinalU", 0).reshape(k, m).T
+    VT = np.hstack([run.get("FinalVT", p).reshape(k, 30) for p in range(2)])
+    assert np.linalg.norm(X - (U * run.get("sigma", 0)[:k]) @ VT) > 1e-3 * np.linalg.norm(X)
