@@ -525,6 +525,7 @@ void DecentralizedClient::run() {
     throw std::runtime_error("form '" + opt_.form + "' is not available (fp64 or mixed)");
   const bool compute_u = svd_mode != 3;
   const bool compute_v = svd_mode != 2;
+  bidiag_order = (int)m;  // global_n once the tall-and-skinny branch has run its bidiagonal SVD
 
   // 1) meta and seeds (client.cpp:494-531)
   ClientMeta cm{m, local_n, cid, 0};
@@ -559,10 +560,14 @@ void DecentralizedClient::run() {
   }
   private_seed = (unsigned)rand();
   time_init = now_seconds() - t0;
-  if (m > global_n) throw std::runtime_error("the tall-and-skinny branch (m > global_n) is not supported");
-  if (m >= global_n && compute_v)
-    throw std::runtime_error("m == global_n takes the reference's tall-and-skinny combine branch, not supported");
-  if (m < 3) throw std::runtime_error("m must be at least 3");
+  if (m < 3 || global_n < 3) throw std::runtime_error("m and global_n must be at least 3");
+  if (opt_.form == "mixed" && m > global_n)
+    throw std::runtime_error("the mixed form needs m <= global_n (its m x m Gram matrix is singular otherwise)");
+  if (m > global_n) {
+    const int k = run_tall(compute_u, compute_v);
+    finish_run(k, compute_u, compute_v, true, t_run);
+    return;
+  }
   if (opt_.form == "mixed") {
     const int k = run_mixed(compute_u, compute_v);
     finish_run(k, compute_u, compute_v, true, t_run);
@@ -735,150 +740,11 @@ void DecentralizedClient::run() {
   time_global_qr = now_seconds() - t0;
 
   // 6) Ralha bidiagonalization (client.cpp:800-910) + lower to upper (947-954)
-  t0 = now_seconds();
-  D.U.alloc(m * c, s);
-  D.alpha.alloc(m, s);
-  D.beta.alloc(m, s);
-  D.house.alloc(m * m, s);
-  D.tau_h.alloc(m, s);
-  D.cs.alloc(2 * m, s);
-  D.msg.alloc(m + 3, s);
-  D.g.alloc(m + 3, s);
-  D.scal.alloc(4, s);
-  sync();
-  {
-    HBuf& hb = D.h_msg;
-    hb.alloc(m + 3);
-    double ar = 0.0;
-    auto allreduce_dev = [&](double* dsrc, double* ddst, long n) {
-      ckh(hipMemcpyAsync(hb.p, dsrc, n * sizeof(double), hipMemcpyDeviceToHost, s), "d2h");
-      sync();
-      const double ta = now_seconds();
-      net.ring_all_reduce_sum(hb.p, n);
-      ar += now_seconds() - ta;
-      ckh(hipMemcpyAsync(ddst, hb.p, n * sizeof(double), hipMemcpyHostToDevice, s), "h2d");
-    };
-    const int mi = (int)m, ci = (int)c;
-    ck(exc_ralha_tail_update(-1, mi, ci, D.F.p, D.U.p, nullptr, D.beta.p, D.scal.p, s), "ralha init");
-    allreduce_dev(D.scal.p, D.scal.p + 1, 1);
-    ck(exc_ralha_tail_scale(0, mi, ci, D.F.p, D.U.p, D.scal.p + 1, D.alpha.p, s), "ralha init");
-    const bool one_launch = D.loop_ok == 1;
-    if (one_launch) {
-      // steps -1 .. m-3 in one launch; the messages go through device-memory boxes when every
-      // peer shares this device, else this thread all-reduces them over the network
-      const double tmo = std::max(30.0, opt_.recv_timeout_s);
-      D.work.alloc((exc_ralha_loop_workspace(mi, ci) + 7) / 8, s);
-      if (!D.link || D.link_m != mi) {
-        if (D.link) ck(exc_ralha_link_free(D.link), "ralha link");
-        D.link = nullptr;
-        ck(exc_ralha_link_alloc(mi, &D.link), "ralha link");
-        D.link_m = mi;
-      }
-      if (D.boxes_state == 1) {
-        ck(exc_ralha_box_clear(D.box, s), "ralha box");
-        sync();
-        net.barrier();  // every peer's box counter is zero before any loop starts
-        ck(exc_ralha_loop_start_boxes(mi, ci, D.F.p, D.U.p, D.house.p, D.tau_h.p, D.alpha.p, D.beta.p, D.link,
-                                      D.work.p, D.work.n * 8, 0, tmo, (int)P, (int)cid, D.boxes.data(),
-                                      D.box_nwg.data(), s),
-           "ralha loop");
-      } else {
-        ck(exc_ralha_loop_start(mi, ci, D.F.p, D.U.p, D.house.p, D.tau_h.p, D.alpha.p, D.beta.p, D.link, D.work.p,
-                                D.work.n * 8, 0, tmo, s),
-           "ralha loop");
-      }
-      try {
-        if (D.boxes_state == 1) {
-          const double t0w = now_seconds();
-          while (true) {
-            const int r = exc_ralha_loop_running(s);
-            if (r < 0) ck(r, "ralha loop");
-            if (r == 0) break;
-            if (net.any_closed()) throw std::runtime_error("a peer disconnected during the bidiagonalization");
-            if (now_seconds() - t0w > tmo) throw std::runtime_error("the bidiagonalization loop timed out");
-            usleep(50);
-          }
-        } else {
-          for (int j = 0; j + 2 < mi; j++) {
-            double* msg = nullptr;
-            ck(exc_ralha_loop_wait(D.link, j, tmo, &msg), "ralha loop");
-            const double ta = now_seconds();
-            net.ring_all_reduce_sum(msg, (long)m + 3 - j);
-            ar += now_seconds() - ta;
-            ck(exc_ralha_loop_post(D.link, j), "ralha loop");
-          }
-        }
-      } catch (...) {
-        exc_ralha_loop_abort(D.link);
-        exc_ralha_loop_finish(D.link, s);
-        throw;
-      }
-      ck(exc_ralha_loop_finish(D.link, s), "ralha loop");
-    } else {
-      const size_t rwb = exc_ralha_workspace(mi, ci);
-      D.work.alloc((rwb + 7) / 8, s);
-      // the step kernels write the local message straight into pinned host memory
-      ck(exc_ralha_step(-1, mi, ci, D.F.p, D.U.p, nullptr, D.house.p, D.tau_h.p, D.alpha.p, D.beta.p, hb.p,
-                        D.work.p, D.work.n * 8, s),
-         "ralha step");
-      for (int i = 0; i + 2 < mi; i++) {
-        sync();
-        const double ta = now_seconds();
-        net.ring_all_reduce_sum(hb.p, (long)m + 3 - i);
-        ar += now_seconds() - ta;
-        ckh(hipMemcpyAsync(D.g.p, hb.p, ((long)m + 3 - i) * sizeof(double), hipMemcpyHostToDevice, s), "h2d");
-        ck(exc_ralha_step(i, mi, ci, D.F.p, D.U.p, D.g.p, D.house.p, D.tau_h.p, D.alpha.p, D.beta.p, hb.p,
-                          D.work.p, D.work.n * 8, s),
-           "ralha step");
-      }
-    }
-    for (int i = mi - 2; i < mi; i++) {
-      ck(exc_ralha_dot(i, mi, ci, D.F.p, D.U.p, D.scal.p, s), "ralha dot");
-      allreduce_dev(D.scal.p, D.scal.p + 1, 1);
-      ck(exc_ralha_tail_update(i, mi, ci, D.F.p, D.U.p, D.scal.p + 1, D.beta.p, D.scal.p + 2, s), "ralha tail");
-      allreduce_dev(D.scal.p + 2, D.scal.p + 3, 1);
-      ck(exc_ralha_tail_scale(i, mi, ci, D.F.p, D.U.p, D.scal.p + 3, D.alpha.p, s), "ralha tail");
-    }
-    ck(exc_lower_to_upper((int)m, D.alpha.p, D.beta.p, D.cs.p, s), "lower to upper");
-    sync();
-    time_ralha_allreduce = ar;
-  }
+  ralha_stage((int)m, (int)c, D.loop_ok == 1);
   D.F.release();
-  time_bi_diag = now_seconds() - t0;
-  log("One-side Householder reflector Costs " + std::to_string(time_bi_diag));
 
   // 7) bidiagonal SVD (client.cpp:963-996)
-  t0 = now_seconds();
-  {
-    DBuf& b2 = D.b2;
-    b2.alloc(2, s);
-    DBuf& cnt2 = D.cnt2;
-    cnt2.alloc(1, s);
-    ck(exc_bisection_bound((int)m, D.alpha.p, D.beta.p, b2.p, s), "bound");
-    ck(exc_sturm_count((int)m, D.alpha.p, D.beta.p, b2.p, 2, (int*)cnt2.p, s), "count");
-    int hc[2];
-    ckh(hipMemcpyAsync(hc, cnt2.p, sizeof hc, hipMemcpyDeviceToHost, s), "d2h");
-    sync();
-    sigma_count = hc[0] - hc[1];
-    log("Ori Sigma Count = " + std::to_string(sigma_count));
-    if (top_k > 0) sigma_count = std::min(top_k, sigma_count);
-    log("Adj Sigma Count = " + std::to_string(sigma_count));
-  }
-  const int k = sigma_count;
-  D.bu.alloc(m * (size_t)k, s);
-  D.bv.alloc(m * (size_t)k, s);
-  D.sigma.alloc(std::max(k, 1), s);
-  D.flip.alloc(std::max(k, 1), s);
-  {
-    const size_t wb = exc_bidiag_svd_workspace((int)m, k);
-    D.work.alloc((wb + 7) / 8, s);
-    const int rc = exc_bidiag_svd((int)m, D.alpha.p, D.beta.p, D.bu.p, D.bv.p, D.sigma.p, k, nullptr, D.work.p,
-                                  D.work.n * 8, s);
-    ck(rc, "bidiagonal SVD");
-    ck(exc_sign_normalize((int)m, k, D.bu.p, D.bv.p, D.flip.p, s), "sign rule");
-    sync();
-  }
-  time_bsvd = now_seconds() - t0;
+  const int k = bsvd_stage((int)m);
 
   // 8) combine and unmask (client.cpp:998-1199)
   t0 = now_seconds();
@@ -932,7 +798,17 @@ void DecentralizedClient::run() {
     givens_mask(D.FinalU.p, (long)m, k, 0, 1, public_seed, plans, s);
   }
   D.U.release();
-  if (compute_v) {
+  if (compute_v && m == global_n) {
+    // m >= global_n: the reference's tall-and-skinny combine of the right vectors
+    // (client.cpp:1113-1154): rows n_pos .. of H (rotations back) bu, transposed and unmasked
+    ck(exc_rot_back((int)m, k, D.bu.p, D.cs.p, s), "back rotation");
+    const size_t wb = exc_ormqr_house_workspace((int)m, k);
+    D.work.alloc((wb + 7) / 8, s);
+    ck(exc_ormqr_house_ln((int)m, k, D.house.p, D.tau_h.p, D.bu.p, k, D.work.p, D.work.n * 8, s), "ormqr house");
+    D.tmp.alloc((size_t)k * local_n, s);
+    ck(exc_transpose((long)local_n, k, D.bu.p + n_pos[cid] * (size_t)k, k, D.tmp.p, (long)local_n, s), "FinalVT");
+    givens_mask(D.tmp.p, k, (long)local_n, 0, 0, private_seed, plans, s);
+  } else if (compute_v) {
     ck(exc_rot_back((int)m, k, D.bu.p, D.cs.p, s), "back rotation");
     {
       const size_t wb = exc_ormqr_house_workspace((int)m, k);
@@ -961,6 +837,342 @@ void DecentralizedClient::run() {
   sync();
   time_combine_svd_remove_mask = now_seconds() - t0;
   finish_run(k, compute_u, compute_v, lqr, t_run);
+}
+
+// The tall-and-skinny branch (m > global_n) of run(): client.cpp:543-569 (masks), 627-678
+// (re-partition, local QR), 800-996 with the bidiagonal of order global_n, and the combine of
+// client.cpp:998-1154. The raw block D.X (m * local_n doubles) is read as a local_n x m row-major
+// matrix, as the reference reads it in this branch; FinalU ends k x m row-major (the reference's
+// layout here) and FinalVT k x local_n. Exchanges go over the network.
+int DecentralizedClient::run_tall(bool compute_u, bool compute_v) {
+  hipStream_t s = dev_->s;
+  exc::ClientDevice& D = *dev_;
+  exc::PeerNet& net = *net_;
+  const int P = num_client, cid = client_id;
+  auto sync = [&] { ckh(hipStreamSynchronize(s), "stream sync"); };
+  const size_t gn = global_n, ln = local_n;
+  // the four Givens plans: masks (left K = local_n private, right K = m public) and their removal
+  RunPlans plans;
+  plans.start({{(long)ln, 1, private_seed}, {(long)m, 1, public_seed}, {(long)m, 0, public_seed},
+               {(long)ln, 0, private_seed}});
+
+  // masks (client.cpp:543-569): local_k = local_n
+  double t0 = now_seconds();
+  givens_mask(D.X.p, (long)ln, (long)m, 1, 1, private_seed, plans, s);
+  givens_mask(D.X.p, (long)ln, (long)m, 1, 0, public_seed, plans, s);
+  sync();
+  time_apply_mask = now_seconds() - t0;
+  log("Masking Costs (tall-and-skinny) " + std::to_string(time_apply_mask));
+
+  // re-partition (client.cpp:627-659): peer j gets columns m_part_pos[j] .. of every block, and
+  // this peer's columns of all blocks form Pm = global_n x c (row-major), rows n_pos[i] .. from
+  // peer i
+  std::vector<size_t> m_part(P, m / P), m_part_pos(P + 1, 0);
+  for (int i = 0; i < (int)(m % P); i++) m_part[i] += 1;
+  for (int i = 0; i < P; i++) m_part_pos[i + 1] = m_part_pos[i] + m_part[i];
+  const size_t c = m_part[cid];
+  t0 = now_seconds();
+  {
+    HBuf& hm = D.h_masked;
+    HBuf& hr = D.h_repart;
+    hm.alloc(ln * m);
+    ckh(hipMemcpyAsync(hm.p, D.X.p, ln * m * sizeof(double), hipMemcpyDeviceToHost, s), "d2h");
+    sync();
+    std::vector<double> blk;
+    for (int j = 0; j < P; j++) {
+      if (j == cid) continue;
+      blk.resize(ln * m_part[j]);
+      for (size_t a = 0; a < ln; a++)
+        memcpy(blk.data() + a * m_part[j], hm.p + a * m + m_part_pos[j], m_part[j] * 8);
+      net.send_to(j, blk.data(), blk.size() * 8);
+    }
+    hr.alloc(gn * c);
+    for (int i = 0; i < P; i++) {
+      const size_t li = local_ns[i], off = n_pos[i];
+      if (i == cid) {
+        for (size_t a = 0; a < li; a++) memcpy(hr.p + (off + a) * c, hm.p + a * m + m_part_pos[cid], c * 8);
+      } else {
+        std::vector<char> r = net.receive(i);
+        if (r.size() != li * c * 8) throw std::runtime_error("re-partition block of unexpected size");
+        memcpy(hr.p + off * c, r.data(), li * c * 8);
+      }
+    }
+    D.P.alloc(gn * c, s);
+    ckh(hipMemcpyAsync(D.P.p, hr.p, gn * c * sizeof(double), hipMemcpyHostToDevice, s), "h2d");
+    sync();
+  }
+  D.X.release();
+  time_exchange_low_dim = now_seconds() - t0;
+  net.add_comm_seconds(time_exchange_low_dim);
+
+  // local QR (client.cpp:661-676) when m_part > global_n: Pm^T = Q R (c x global_n), R^T into
+  // D.rt and Q^T in place of Pm (u_reduction); the Ralha block is R (global_n x global_n), else Pm
+  const bool lqr = c > gn;
+  const size_t bl = std::min(c, gn);
+  D.F.alloc(gn * bl, s);
+  t0 = now_seconds();
+  if (lqr) {
+    D.rt.alloc(gn * gn, s);
+    const size_t wb = exc_qr_t_workspace((int)gn, (long)c, 1);
+    D.work.alloc((wb + 7) / 8, s, false);
+    ck(exc_qr_t((int)gn, (long)c, D.P.p, (long)c, D.rt.p, compute_u ? D.P.p : nullptr, (long)c, nullptr, 0, nullptr,
+                nullptr, 0, D.work.p, D.work.n * 8, s),
+       "local QR (tall-and-skinny)");
+    // the Ralha block column-major: element (i, l) = R^T[i][l]
+    ck(exc_transpose((long)gn, (long)gn, D.rt.p, (long)gn, D.F.p, (long)gn, s), "Ralha block");
+    sync();
+    D.rt.release();
+    time_local_qr = now_seconds() - t0;
+    log("Local QR Reduction (tall-and-skinny) Costs " + std::to_string(time_local_qr));
+  } else {
+    ck(exc_transpose((long)gn, (long)c, D.P.p, (long)c, D.F.p, (long)gn, s), "Ralha block");
+    sync();
+    D.P.release();
+  }
+
+  // bidiagonalization of order global_n (one launch per step: the loop's co-residency decision
+  // is taken for the short-and-wide shape) and the bidiagonal SVD
+  ralha_stage((int)gn, (int)bl, false);
+  D.F.release();
+  const int k = bsvd_stage((int)gn);
+
+  // combine (client.cpp:998-1154)
+  t0 = now_seconds();
+  if (compute_u) {
+    // this peer's columns of FinalU (k x m): part = bv U_r (u_reduction), gathered as the rows of
+    // FinalU^T (m x k) and transposed at the end
+    DBuf& Ut = D.tmp;
+    Ut.alloc(m * (size_t)k, s);
+    D.work.alloc((exc_gemm_workspace((long)gn, k, (long)gn) + 7) / 8, s, false);
+    if (lqr) {
+      DBuf& T2 = D.Qg;  // (U_r^T bv^T): global_n x k
+      T2.alloc(gn * (size_t)k, s);
+      ck(exc_dgemm(0, 1, (long)gn, k, (long)gn, 1.0, D.U.p, (long)gn, D.bv.p, (long)gn, 0.0, T2.p, k, D.work.p,
+                   D.work.n * 8, s),
+         "part final U");
+      ck(exc_dgemm(1, 0, (long)c, k, (long)gn, 1.0, D.P.p, (long)c, T2.p, k, 0.0, Ut.p + m_part_pos[cid] * (size_t)k,
+                   k, nullptr, 0, s),
+         "part final U (reduction)");
+      sync();
+      T2.release();
+      D.P.release();
+    } else {
+      ck(exc_dgemm(0, 1, (long)c, k, (long)gn, 1.0, D.U.p, (long)gn, D.bv.p, (long)gn, 0.0,
+                   Ut.p + m_part_pos[cid] * (size_t)k, k, D.work.p, D.work.n * 8, s),
+         "part final U");
+    }
+    // ring all-gather of the column blocks of FinalU (client.cpp:1062-1096); the wire carries
+    // each block as the reference does, k x m_part row-major
+    const double tc = now_seconds();
+    {
+      HBuf& hU = D.h_U;
+      hU.alloc(m * (size_t)k);
+      ckh(hipMemcpyAsync(hU.p + m_part_pos[cid] * k, Ut.p + m_part_pos[cid] * k, c * (size_t)k * 8,
+                         hipMemcpyDeviceToHost, s),
+          "d2h");
+      sync();
+      const int next = (cid + 1) % P, front = (cid - 1 + P) % P;
+      std::vector<double> wire;
+      for (int st = 0; st < P - 1; st++) {
+        int di = (cid - st + P) % P;
+        wire.resize(m_part[di] * (size_t)k);
+        for (size_t r = 0; r < m_part[di]; r++)
+          for (int a = 0; a < k; a++) wire[(size_t)a * m_part[di] + r] = hU.p[(m_part_pos[di] + r) * k + a];
+        net.send_to(next, wire.data(), wire.size() * 8);
+        di = (di - 1 + P) % P;
+        wire.resize(m_part[di] * (size_t)k);
+        net.receive_into(front, wire.data(), wire.size() * 8);
+        for (size_t r = 0; r < m_part[di]; r++)
+          for (int a = 0; a < k; a++) hU.p[(m_part_pos[di] + r) * k + a] = wire[(size_t)a * m_part[di] + r];
+      }
+      ckh(hipMemcpyAsync(Ut.p, hU.p, m * (size_t)k * 8, hipMemcpyHostToDevice, s), "h2d");
+    }
+    net.add_comm_seconds(now_seconds() - tc);
+    D.FinalU.alloc(m * (size_t)k, s, false);
+    ck(exc_transpose((long)m, k, Ut.p, k, D.FinalU.p, (long)m, s), "FinalU");
+    givens_mask(D.FinalU.p, k, (long)m, 0, 0, public_seed, plans, s);
+    sync();
+    Ut.release();
+  }
+  D.U.release();
+  if (compute_v) {
+    // right vectors (client.cpp:1113-1154): the back rotation and the reflectors of order
+    // global_n, then this peer's rows transposed and unmasked with the private plan
+    ck(exc_rot_back((int)gn, k, D.bu.p, D.cs.p, s), "back rotation");
+    const size_t wb = exc_ormqr_house_workspace((int)gn, k);
+    D.work.alloc((wb + 7) / 8, s);
+    ck(exc_ormqr_house_ln((int)gn, k, D.house.p, D.tau_h.p, D.bu.p, k, D.work.p, D.work.n * 8, s), "ormqr house");
+    D.FinalVT.alloc((size_t)k * ln, s, false);
+    ck(exc_transpose((long)ln, k, D.bu.p + n_pos[cid] * (size_t)k, k, D.FinalVT.p, (long)ln, s), "FinalVT");
+    givens_mask(D.FinalVT.p, k, (long)ln, 0, 0, private_seed, plans, s);
+  }
+  sync();
+  time_combine_svd_remove_mask = now_seconds() - t0;
+  return k;
+}
+
+// Ralha bidiagonalization (client.cpp:800-910) and lower to upper (947-954) of the mi x ci
+// block D.F (column-major) this peer holds, every peer's all-reduce over the network or the
+// device boxes; one_launch: the loop kernel in one launch (a decision all peers share).
+void DecentralizedClient::ralha_stage(int mi, int ci, bool one_launch) {
+  hipStream_t s = dev_->s;
+  exc::ClientDevice& D = *dev_;
+  exc::PeerNet& net = *net_;
+  const int P = num_client, cid = client_id;
+  (void)cid;
+  auto sync = [&] { ckh(hipStreamSynchronize(s), "stream sync"); };
+  const double t0 = now_seconds();
+  const size_t mm = (size_t)mi, cc = (size_t)ci;
+  D.U.alloc(mm * cc, s);
+  D.alpha.alloc(mm, s);
+  D.beta.alloc(mm, s);
+  D.house.alloc(mm * mm, s);
+  D.tau_h.alloc(mm, s);
+  D.cs.alloc(2 * mm, s);
+  D.msg.alloc(mm + 3, s);
+  D.g.alloc(mm + 3, s);
+  D.scal.alloc(4, s);
+  sync();
+  {
+    HBuf& hb = D.h_msg;
+    hb.alloc(mm + 3);
+    double ar = 0.0;
+    auto allreduce_dev = [&](double* dsrc, double* ddst, long n) {
+      ckh(hipMemcpyAsync(hb.p, dsrc, n * sizeof(double), hipMemcpyDeviceToHost, s), "d2h");
+      sync();
+      const double ta = now_seconds();
+      net.ring_all_reduce_sum(hb.p, n);
+      ar += now_seconds() - ta;
+      ckh(hipMemcpyAsync(ddst, hb.p, n * sizeof(double), hipMemcpyHostToDevice, s), "h2d");
+    };
+    ck(exc_ralha_tail_update(-1, mi, ci, D.F.p, D.U.p, nullptr, D.beta.p, D.scal.p, s), "ralha init");
+    allreduce_dev(D.scal.p, D.scal.p + 1, 1);
+    ck(exc_ralha_tail_scale(0, mi, ci, D.F.p, D.U.p, D.scal.p + 1, D.alpha.p, s), "ralha init");
+    if (one_launch) {
+      // steps -1 .. m-3 in one launch; the messages go through device-memory boxes when every
+      // peer shares this device, else this thread all-reduces them over the network
+      const double tmo = std::max(30.0, opt_.recv_timeout_s);
+      D.work.alloc((exc_ralha_loop_workspace(mi, ci) + 7) / 8, s);
+      if (!D.link || D.link_m != mi) {
+        if (D.link) ck(exc_ralha_link_free(D.link), "ralha link");
+        D.link = nullptr;
+        ck(exc_ralha_link_alloc(mi, &D.link), "ralha link");
+        D.link_m = mi;
+      }
+      if (D.boxes_state == 1) {
+        ck(exc_ralha_box_clear(D.box, s), "ralha box");
+        sync();
+        net.barrier();  // every peer's box counter is zero before any loop starts
+        ck(exc_ralha_loop_start_boxes(mi, ci, D.F.p, D.U.p, D.house.p, D.tau_h.p, D.alpha.p, D.beta.p, D.link,
+                                      D.work.p, D.work.n * 8, 0, tmo, (int)P, (int)cid, D.boxes.data(),
+                                      D.box_nwg.data(), s),
+           "ralha loop");
+      } else {
+        ck(exc_ralha_loop_start(mi, ci, D.F.p, D.U.p, D.house.p, D.tau_h.p, D.alpha.p, D.beta.p, D.link, D.work.p,
+                                D.work.n * 8, 0, tmo, s),
+           "ralha loop");
+      }
+      try {
+        if (D.boxes_state == 1) {
+          const double t0w = now_seconds();
+          while (true) {
+            const int r = exc_ralha_loop_running(s);
+            if (r < 0) ck(r, "ralha loop");
+            if (r == 0) break;
+            if (net.any_closed()) throw std::runtime_error("a peer disconnected during the bidiagonalization");
+            if (now_seconds() - t0w > tmo) throw std::runtime_error("the bidiagonalization loop timed out");
+            usleep(50);
+          }
+        } else {
+          for (int j = 0; j + 2 < mi; j++) {
+            double* msg = nullptr;
+            ck(exc_ralha_loop_wait(D.link, j, tmo, &msg), "ralha loop");
+            const double ta = now_seconds();
+            net.ring_all_reduce_sum(msg, (long)mi + 3 - j);
+            ar += now_seconds() - ta;
+            ck(exc_ralha_loop_post(D.link, j), "ralha loop");
+          }
+        }
+      } catch (...) {
+        exc_ralha_loop_abort(D.link);
+        exc_ralha_loop_finish(D.link, s);
+        throw;
+      }
+      ck(exc_ralha_loop_finish(D.link, s), "ralha loop");
+    } else {
+      const size_t rwb = exc_ralha_workspace(mi, ci);
+      D.work.alloc((rwb + 7) / 8, s);
+      // the step kernels write the local message straight into pinned host memory
+      ck(exc_ralha_step(-1, mi, ci, D.F.p, D.U.p, nullptr, D.house.p, D.tau_h.p, D.alpha.p, D.beta.p, hb.p,
+                        D.work.p, D.work.n * 8, s),
+         "ralha step");
+      for (int i = 0; i + 2 < mi; i++) {
+        sync();
+        const double ta = now_seconds();
+        net.ring_all_reduce_sum(hb.p, (long)mi + 3 - i);
+        ar += now_seconds() - ta;
+        ckh(hipMemcpyAsync(D.g.p, hb.p, ((long)mi + 3 - i) * sizeof(double), hipMemcpyHostToDevice, s), "h2d");
+        ck(exc_ralha_step(i, mi, ci, D.F.p, D.U.p, D.g.p, D.house.p, D.tau_h.p, D.alpha.p, D.beta.p, hb.p,
+                          D.work.p, D.work.n * 8, s),
+           "ralha step");
+      }
+    }
+    for (int i = mi - 2; i < mi; i++) {
+      ck(exc_ralha_dot(i, mi, ci, D.F.p, D.U.p, D.scal.p, s), "ralha dot");
+      allreduce_dev(D.scal.p, D.scal.p + 1, 1);
+      ck(exc_ralha_tail_update(i, mi, ci, D.F.p, D.U.p, D.scal.p + 1, D.beta.p, D.scal.p + 2, s), "ralha tail");
+      allreduce_dev(D.scal.p + 2, D.scal.p + 3, 1);
+      ck(exc_ralha_tail_scale(i, mi, ci, D.F.p, D.U.p, D.scal.p + 3, D.alpha.p, s), "ralha tail");
+    }
+    ck(exc_lower_to_upper(mi, D.alpha.p, D.beta.p, D.cs.p, s), "lower to upper");
+    sync();
+    time_ralha_allreduce = ar;
+  }
+  time_bi_diag = now_seconds() - t0;
+  log("One-side Householder reflector Costs " + std::to_string(time_bi_diag));
+
+}
+
+// Bidiagonal SVD of the upper bidiagonal (D.alpha, D.beta) of order n (client.cpp:963-996):
+// the count of singular values, bisection + twisted factorization and the peers' sign rule.
+// Returns the number of triplets kept (sigma_count).
+int DecentralizedClient::bsvd_stage(int n) {
+  hipStream_t s = dev_->s;
+  exc::ClientDevice& D = *dev_;
+  auto sync = [&] { ckh(hipStreamSynchronize(s), "stream sync"); };
+  const double t0 = now_seconds();
+  {
+    DBuf& b2 = D.b2;
+    b2.alloc(2, s);
+    DBuf& cnt2 = D.cnt2;
+    cnt2.alloc(1, s);
+    ck(exc_bisection_bound(n, D.alpha.p, D.beta.p, b2.p, s), "bound");
+    ck(exc_sturm_count(n, D.alpha.p, D.beta.p, b2.p, 2, (int*)cnt2.p, s), "count");
+    int hc[2];
+    ckh(hipMemcpyAsync(hc, cnt2.p, sizeof hc, hipMemcpyDeviceToHost, s), "d2h");
+    sync();
+    sigma_count = hc[0] - hc[1];
+    log("Ori Sigma Count = " + std::to_string(sigma_count));
+    if (top_k > 0) sigma_count = std::min(top_k, sigma_count);
+    log("Adj Sigma Count = " + std::to_string(sigma_count));
+  }
+  const int k = sigma_count;
+  bidiag_order = n;
+  D.bu.alloc((size_t)n * k, s);
+  D.bv.alloc((size_t)n * k, s);
+  D.sigma.alloc(std::max(k, 1), s);
+  D.flip.alloc(std::max(k, 1), s);
+  {
+    const size_t wb = exc_bidiag_svd_workspace(n, k);
+    D.work.alloc((wb + 7) / 8, s);
+    const int rc = exc_bidiag_svd(n, D.alpha.p, D.beta.p, D.bu.p, D.bv.p, D.sigma.p, k, nullptr, D.work.p,
+                                  D.work.n * 8, s);
+    ck(rc, "bidiagonal SVD");
+    ck(exc_sign_normalize(n, k, D.bu.p, D.bv.p, D.flip.p, s), "sign rule");
+    sync();
+  }
+  time_bsvd = now_seconds() - t0;
+  return k;
 }
 
 // Logs, host results and the result line of one run() (client.cpp:1200-1213).
@@ -1004,17 +1216,19 @@ void DecentralizedClient::finish_run(int k, bool compute_u, bool compute_v, bool
       if (compute_u) dump("FinalU", FinalU, m * (size_t)k);
       if (compute_v) dump("FinalVT", FinalVT, (size_t)k * local_n);
       // the bidiagonal and its vectors, for stage-by-stage comparison
-      dump("alpha", x_diagonal, m);
-      dump("beta", x_upper_diagonal, m - 1);
+      const size_t nb = bidiag_order > 0 ? (size_t)bidiag_order : m;
+      dump("alpha", x_diagonal, nb);
+      dump("beta", x_upper_diagonal, nb - 1);
       if (opt_.form != "mixed") {
-        std::vector<double> v(m * (size_t)k);
+        std::vector<double> v(nb * (size_t)k);
         ckh(hipMemcpyAsync(v.data(), D.bv.p, v.size() * 8, hipMemcpyDeviceToHost, s), "d2h");
         sync();
         dump("bidiag_v", v.data(), v.size());
       }
     }
   }
-  if (evaluate) evaluate_results(k, compute_u, compute_v, lqr);
+  if (evaluate && m > global_n) log("the evaluation report is computed for the short-and-wide branch only");
+  else if (evaluate) evaluate_results(k, compute_u, compute_v, lqr);
   char json[1024];
   snprintf(json, sizeof json,
            "{\"cid\": %d, \"sigma_count\": %d, \"total\": %.6f, \"communication\": %.6f, \"time_init\": %.6f, "
@@ -1039,10 +1253,11 @@ void DecentralizedClient::copy_results_to_host(int k, bool compute_u, bool compu
   D.r_sigma.alloc(std::max(k, 1), false);
   ckh(hipMemcpyAsync(D.r_sigma.p, D.sigma.p, k * 8, hipMemcpyDeviceToHost, s), "d2h");
   sigma = D.r_sigma.p;
-  D.r_alpha.alloc(m, false);
-  D.r_beta.alloc(m, false);
-  ckh(hipMemcpyAsync(D.r_alpha.p, D.alpha.p, m * 8, hipMemcpyDeviceToHost, s), "d2h");
-  ckh(hipMemcpyAsync(D.r_beta.p, D.beta.p, (m - 1) * 8, hipMemcpyDeviceToHost, s), "d2h");
+  const size_t nb = bidiag_order > 0 ? (size_t)bidiag_order : m;  // global_n in the tall branch
+  D.r_alpha.alloc(nb, false);
+  D.r_beta.alloc(nb, false);
+  ckh(hipMemcpyAsync(D.r_alpha.p, D.alpha.p, nb * 8, hipMemcpyDeviceToHost, s), "d2h");
+  ckh(hipMemcpyAsync(D.r_beta.p, D.beta.p, (nb - 1) * 8, hipMemcpyDeviceToHost, s), "d2h");
   x_diagonal = D.r_alpha.p;
   x_upper_diagonal = D.r_beta.p;
   if (compute_u) {

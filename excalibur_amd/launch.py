@@ -110,7 +110,9 @@ def run_peers(xs, seed=0, mode=1, topk=-1, opt=3, form="fp64", dump=True, workdi
             base = os.path.join(logdir, f"Client{cid}_")
             peer["sigma"] = np.fromfile(base + "sigma.bin")
             if os.path.exists(base + "FinalU.bin"):
-                peer["FinalU"] = np.fromfile(base + "FinalU.bin").reshape(m, k)
+                # k x m in the tall-and-skinny branch (m > global_n), as the reference leaves it
+                tall = m > sum(x.shape[1] for x in xs)
+                peer["FinalU"] = np.fromfile(base + "FinalU.bin").reshape((k, m) if tall else (m, k))
             if os.path.exists(base + "FinalVT.bin"):
                 peer["FinalVT"] = np.fromfile(base + "FinalVT.bin").reshape(k, xs[cid].shape[1])
             for name in ("alpha", "beta", "bidiag_v"):

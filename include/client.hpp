@@ -2,10 +2,11 @@
 // parameters and stage order of run() (client.cpp:447-1213), with every matrix stage executed on
 // the GPU through libexcalibur_amd's C ABI and the peer exchanges kept on host TCP.
 //
-// Supported: the short-and-wide branch (m <= global_n, the branch every benchmark configuration
-// takes), modes 1 (two-sided), 2 (left) and 3 (right) with bisection + twisted factorization,
-// opt 3 / 1 (Givens masks). Mode 0 (full SVD) is routed through the same bisection + twisted
-// path instead of LAPACKE_dbdsdc. The tall-and-skinny branch, mode 4 (linear regression) and the
+// Supported: the short-and-wide branch (m <= global_n) and the tall-and-skinny branch
+// (m > global_n, where the input block is read as local_n x m and FinalU is k x m, as the
+// reference has them), modes 1 (two-sided), 2 (left) and 3 (right) with bisection + twisted
+// factorization, opt 3 / 1 (Givens masks). Mode 0 (full SVD) is routed through the same
+// bisection + twisted path instead of LAPACKE_dbdsdc. Mode 4 (linear regression) and the
 // dense-mask ablation (opt 0 / 2) are rejected with an error message.
 #pragma once
 
@@ -124,6 +125,13 @@ class DecentralizedClient {
   void evaluate_results(int k, bool compute_u, bool compute_v, bool lqr);
   // The mixed-precision form of run() after the seeds; returns the number of triplets kept.
   int run_mixed(bool compute_u, bool compute_v);
+  // The tall-and-skinny branch (m > global_n) after the seeds; returns the triplets kept.
+  int run_tall(bool compute_u, bool compute_v);
+  // Stages shared by the branches: the bidiagonalization of this peer's mi x ci block D.F, and
+  // the bidiagonal SVD of order n (returns sigma_count).
+  void ralha_stage(int mi, int ci, bool one_launch);
+  int bsvd_stage(int n);
+  int bidiag_order = 0;  // order of the bidiagonal of the last run (m, or global_n when tall)
   std::vector<double> h_sigma_, h_FinalU_, h_FinalVT_, h_alpha_, h_beta_;
   void log(const std::string& line);
   // Once per client: whether all peers share this device (and each runs the one-launch
