@@ -311,14 +311,14 @@ def bench_batched(nb=32, m=512, lo=2000, hi=60000, seed=4):
     return out
 
 
-def bench_learned(n_train=10_000, steps=2000, seed=10):
+def bench_learned(n_train=10_000, steps=2000, seed=10, per=32):
     """Learned bisection brackets: 10,000 generated bidiagonals with n in [512, 4096] (four spectrum
     families), the network trained by the library's FP32 kernels, then every singular value of
     n = 16,384 bidiagonals of the four families bisected from its BF16 brackets and plainly."""
     from excalibur_amd import learned as Lm
     t0 = time.time()
     train = Lm.make_set(n_train, (512, 4096), seed=seed)
-    X, y = Lm.training_data(train, per=32, seed=seed + 1)
+    X, y = Lm.training_data(train, per=per, seed=seed + 1)
     t_data = time.time() - t0
     net = Lm.BracketNet(seed=0)
     hist = net.train(X, y, steps=steps, batch=4096, lr=2e-3, seed=1)
@@ -363,6 +363,8 @@ def main():
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-bracket workload")
     ap.add_argument("--only", choices=["batched", "learned"], default=None,
                     help="run only that workload and print its JSON (for profiling it alone)")
+    ap.add_argument("--learned-steps", type=int, default=2000, help="Adam steps of the bracket network")
+    ap.add_argument("--learned-per", type=int, default=32, help="training targets per generated bidiagonal")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--workdir", default=None)
     ap.add_argument("--prepare", default=None,
@@ -391,7 +393,8 @@ def main():
         print(json.dumps(isolated_gemms(n, rows)))
         return
     if args.only:
-        print(json.dumps(bench_batched() if args.only == "batched" else bench_learned()))
+        print(json.dumps(bench_batched() if args.only == "batched"
+                         else bench_learned(steps=args.learned_steps, per=args.learned_per)))
         return
     from excalibur_amd.launch import write_peer_files
 
@@ -444,7 +447,7 @@ def main():
     learned = None
     if not args.no_variants and not args.no_learned:
         try:
-            learned = bench_learned()
+            learned = bench_learned(steps=args.learned_steps, per=args.learned_per)
         except Exception as e:
             log(f"learned brackets failed: {e}")
             learned = {"error": str(e)}

@@ -25,11 +25,21 @@ __device__ __forceinline__ void rot(T* p1, T* p2, T c, T ss, T nss) {
   *p2 = a * nss + b * c;
 }
 
+// Plans of at most this many pairs per pass are staged in LDS (two pass buffers): every
+// workgroup reads each pass's pairs and rotations from HBM or L2 once, one pass ahead of their
+// use, instead of once per line it owns and in the shadow of nothing.
+constexpr int kStageMaxHalf = 1024;
+constexpr int kStagePerThread = kStageMaxHalf / 256;
+size_t stage_bytes(int half) { return half <= kStageMaxHalf ? (size_t)half * 48 : 0; }
+
 // odd_mode: 0 none, 1 the special pair is pair 0 (applied first), 2 it is the last pair.
-template <typename T, bool LEFT>
+// STAGE: the plan of pass p + 1 is loaded into registers before the rotations of pass p and
+// stored into the other LDS plan buffer after them (stage: 2 x half x {2 int32, 2 double}).
+template <typename T, bool LEFT, bool STAGE>
 __device__ __forceinline__ void givens_strip(T* __restrict__ x, long m, long n, int W,
                                              const int32_t* __restrict__ pairs, const double* __restrict__ cs,
-                                             int passes, int half, int mask, int odd_mode, T* buf) {
+                                             int passes, int half, int mask, int odd_mode, T* buf,
+                                             unsigned char* stage) {
   const int tid = threadIdx.x;
   const long K = LEFT ? m : n;
   const long base = (long)blockIdx.x * W;
@@ -66,9 +76,42 @@ __device__ __forceinline__ void givens_strip(T* __restrict__ x, long m, long n, 
       else rot(&buf[(long)cc * n + r1], &buf[(long)cc * n + r2], c, ss, nss);
     }
   };
+  // staged plan buffers: b holds half int2 pairs then half double2 rotations
+  int32_t* spp[2] = {nullptr, nullptr};
+  double* scp[2] = {nullptr, nullptr};
+  // registers of the next pass: pair (r1, r2) and rotation (c, s) of up to kStagePerThread pairs;
+  // scalar loads, so no alignment beyond the elements' own is assumed of the plan
+  int32_t ra[kStagePerThread], rb[kStagePerThread];
+  double rc[kStagePerThread], rs[kStagePerThread];
+  if (STAGE) {
+    for (int b = 0; b < 2; b++) {
+      unsigned char* base_b = stage + (size_t)b * half * 24;
+      spp[b] = reinterpret_cast<int32_t*>(base_b);
+      scp[b] = reinterpret_cast<double*>(base_b + (size_t)half * 8);
+    }
+    for (int idx = tid; idx < 2 * half; idx += kThreads) {
+      spp[0][idx] = pairs[idx];
+      scp[0][idx] = cs[idx];
+    }
+    __syncthreads();
+  }
   for (int p = 0; p < passes; p++) {
-    const int32_t* pp = pairs + (long)p * half * 2;
-    const double* cp = cs + (long)p * half * 2;
+    const int32_t* pp = STAGE ? spp[p & 1] : pairs + (long)p * half * 2;
+    const double* cp = STAGE ? scp[p & 1] : cs + (long)p * half * 2;
+    if (STAGE && p + 1 < passes) {  // next pass's plan in flight while this pass rotates
+      const int32_t* gp = pairs + (long)(p + 1) * half * 2;
+      const double* gc = cs + (long)(p + 1) * half * 2;
+#pragma unroll
+      for (int q = 0; q < kStagePerThread; q++) {
+        const int idx = tid + q * kThreads;
+        if (idx < half) {
+          ra[q] = gp[2 * idx];
+          rb[q] = gp[2 * idx + 1];
+          rc[q] = gc[2 * idx];
+          rs[q] = gc[2 * idx + 1];
+        }
+      }
+    }
     if (odd_mode == 1) {
       apply_range(pp, cp, 0, 1);
       __syncthreads();
@@ -79,6 +122,22 @@ __device__ __forceinline__ void givens_strip(T* __restrict__ x, long m, long n, 
       apply_range(pp, cp, half - 1, half);
     } else {
       apply_range(pp, cp, 0, half);
+    }
+    if (STAGE && p + 1 < passes) {
+      // the other buffer was last read in pass p - 1, which every thread finished at the
+      // barrier that ended it
+      int32_t* np = spp[(p + 1) & 1];
+      double* nc = scp[(p + 1) & 1];
+#pragma unroll
+      for (int q = 0; q < kStagePerThread; q++) {
+        const int idx = tid + q * kThreads;
+        if (idx < half) {
+          np[2 * idx] = ra[q];
+          np[2 * idx + 1] = rb[q];
+          nc[2 * idx] = rc[q];
+          nc[2 * idx + 1] = rs[q];
+        }
+      }
     }
     __syncthreads();
   }
@@ -96,25 +155,30 @@ __device__ __forceinline__ void givens_strip(T* __restrict__ x, long m, long n, 
   }
 }
 
-template <typename T, bool LEFT>
+// LDS: the strip (W lines, rounded up to 256 bytes), then the staged plan when STAGE.
+template <typename T, bool LEFT, bool STAGE>
 __global__ __launch_bounds__(kThreads) void k_givens(T* __restrict__ x, long m, long n, int W,
                                                      const int32_t* __restrict__ pairs,
                                                      const double* __restrict__ cs, int passes, int half, int mask,
                                                      int odd_mode) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  givens_strip<T, LEFT>(x, m, n, W, pairs, cs, passes, half, mask, odd_mode, reinterpret_cast<T*>(smem_raw));
+  const size_t strip = ((size_t)W * (LEFT ? m : n) * sizeof(T) + 255) / 256 * 256;
+  givens_strip<T, LEFT, STAGE>(x, m, n, W, pairs, cs, passes, half, mask, odd_mode, reinterpret_cast<T*>(smem_raw),
+                               smem_raw + strip);
 }
 
 // The same sweep over a batch of matrices of one shape, each with its own plan: blockIdx.y picks
 // the matrix.
-template <typename T, bool LEFT>
+template <typename T, bool LEFT, bool STAGE>
 __global__ __launch_bounds__(kThreads) void k_givens_batched(T* const* __restrict__ xs, long m, long n, int W,
                                                              const int32_t* const* __restrict__ pairs,
                                                              const double* const* __restrict__ cs, int passes,
                                                              int half, int mask, int odd_mode) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int b = blockIdx.y;
-  givens_strip<T, LEFT>(xs[b], m, n, W, pairs[b], cs[b], passes, half, mask, odd_mode, reinterpret_cast<T*>(smem_raw));
+  const size_t strip = ((size_t)W * (LEFT ? m : n) * sizeof(T) + 255) / 256 * 256;
+  givens_strip<T, LEFT, STAGE>(xs[b], m, n, W, pairs[b], cs[b], passes, half, mask, odd_mode,
+                               reinterpret_cast<T*>(smem_raw), smem_raw + strip);
 }
 
 constexpr size_t kLdsSoft = 64 * 1024;   // two workgroups per CU
@@ -131,28 +195,32 @@ int givens_launch(T* x, long m, long n, const int32_t* pairs, const double* cs, 
   const int odd_mode = (K % 2 == 1) ? (mask ? 2 : 1) : 0;
   const size_t line_bytes = (size_t)K * sizeof(T);
   EXC_REQUIRE(line_bytes <= kLdsHard, EXC_ERR_ARG, "exc_givens_apply: a line does not fit in LDS (K too large)");
-  long w_lds = (long)(kLdsSoft / line_bytes);
+  // the staged plan needs its LDS beside one line at least
+  const size_t stage = stage_bytes(half) + 256 <= kLdsHard - line_bytes ? stage_bytes(half) : 0;
+  const size_t budget = kLdsSoft > stage + line_bytes ? kLdsSoft - stage : line_bytes;
+  long w_lds = (long)(budget / line_bytes);
   if (w_lds < 1) w_lds = 1;
   // enough workgroups to spread over the 256 CUs, as long as the strip fits
   long w = lines / 256;
   if (w < 1) w = 1;
   if (w > w_lds) w = w_lds;
   if (w > 64) w = 64;
-  const size_t lds = (size_t)w * line_bytes;
+  const size_t lds = ((size_t)w * line_bytes + 255) / 256 * 256 + stage;
   const unsigned grid = exc_detail::ceil_div(lines, w);
   hipEvent_t e0 = exc_prof::mark(s);
+  auto launch = [&](auto kernel) -> int {
+    if (lds > 64 * 1024)
+      EXC_CHECK_HIP(hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(kernel, dim3(grid), dim3(kThreads), lds, s, x, m, n, (int)w, pairs, cs, (int)passes, half,
+                       mask, odd_mode);
+    return EXC_OK;
+  };
   if (is_left) {
-    if (lds > 64 * 1024)
-      EXC_CHECK_HIP(hipFuncSetAttribute((const void*)k_givens<T, true>,
-                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL((k_givens<T, true>), dim3(grid), dim3(kThreads), lds, s, x, m, n, (int)w, pairs, cs,
-                       (int)passes, half, mask, odd_mode);
+    if (stage) launch(k_givens<T, true, true>);
+    else launch(k_givens<T, true, false>);
   } else {
-    if (lds > 64 * 1024)
-      EXC_CHECK_HIP(hipFuncSetAttribute((const void*)k_givens<T, false>,
-                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL((k_givens<T, false>), dim3(grid), dim3(kThreads), lds, s, x, m, n, (int)w, pairs, cs,
-                       (int)passes, half, mask, odd_mode);
+    if (stage) launch(k_givens<T, false, true>);
+    else launch(k_givens<T, false, false>);
   }
   EXC_CHECK_HIP(hipGetLastError());
   if (e0) {
@@ -176,18 +244,28 @@ int givens_batched_launch(int nb, T* const* d_xs, long m, long n, const int32_t*
   const int odd_mode = (K % 2 == 1) ? (mask ? 2 : 1) : 0;
   const size_t line_bytes = (size_t)K * sizeof(T);
   EXC_REQUIRE(line_bytes <= kLdsHard, EXC_ERR_ARG, "exc_givens_apply_batched: K too large");
-  long w_lds = (long)(kLdsSoft / line_bytes);
+  const size_t stage = stage_bytes(half) + 256 <= kLdsHard - line_bytes ? stage_bytes(half) : 0;
+  const size_t budget = kLdsSoft > stage + line_bytes ? kLdsSoft - stage : line_bytes;
+  long w_lds = (long)(budget / line_bytes);
   if (w_lds < 1) w_lds = 1;
   // the batch already spreads the work: give each workgroup as many lines as fit in LDS
   long w = std::max<long>(1, std::min<long>({w_lds, 64L, std::max<long>(1, lines * nb / 512)}));
-  const size_t lds = (size_t)w * line_bytes;
+  const size_t lds = ((size_t)w * line_bytes + 255) / 256 * 256 + stage;
   const dim3 grid(exc_detail::ceil_div(lines, w), nb);
-  if (is_left)
-    hipLaunchKernelGGL((k_givens_batched<T, true>), grid, dim3(kThreads), lds, s, d_xs, m, n, (int)w, d_pairs, d_cs,
-                       (int)passes, half, mask, odd_mode);
-  else
-    hipLaunchKernelGGL((k_givens_batched<T, false>), grid, dim3(kThreads), lds, s, d_xs, m, n, (int)w, d_pairs, d_cs,
-                       (int)passes, half, mask, odd_mode);
+  auto launch = [&](auto kernel) {
+    if (lds > 64 * 1024)
+      EXC_CHECK_HIP(hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(kernel, grid, dim3(kThreads), lds, s, d_xs, m, n, (int)w, d_pairs, d_cs, (int)passes, half,
+                       mask, odd_mode);
+    return EXC_OK;
+  };
+  if (is_left) {
+    if (stage) launch(k_givens_batched<T, true, true>);
+    else launch(k_givens_batched<T, true, false>);
+  } else {
+    if (stage) launch(k_givens_batched<T, false, true>);
+    else launch(k_givens_batched<T, false, false>);
+  }
   EXC_CHECK_HIP(hipGetLastError());
   return EXC_OK;
 }

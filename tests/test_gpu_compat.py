@@ -93,8 +93,13 @@ def test_masks_bit_exact(bb, oracle, m, n, is_left):
     assert np.abs(z - x).max() < 1e-13 * max(1.0, np.abs(x).max())
 
 
-def test_virtual_mask_branch_rejected(bb):
-    from excalibur_amd.hip import HipError
-    x = np.zeros((1, 10001))
-    with pytest.raises(HipError):
-        bb.block_shift_mask(1, 10001, 32, x, True, False, 1)
+@pytest.mark.parametrize("mask", [True, False])
+def test_virtual_mask_branch_bit_exact(bb, oracle, mask):
+    """block_shift_mask above 10000 lines (the virtual-matrix branch, bbtsvd.cpp:548-668) through
+    the host-pointer drop-in, bit for bit against the oracle."""
+    rng = np.random.default_rng(9)
+    x = rng.standard_normal((3, 10001))
+    want = oracle.block_shift_mask(x.copy(), 32, mask, False, 123)
+    got = x.copy()
+    bb.block_shift_mask(3, 10001, 32, got, mask, False, 123)
+    assert np.array_equal(got, want)
