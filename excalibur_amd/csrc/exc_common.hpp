@@ -45,6 +45,32 @@ __device__ inline float wave_sum(float v) {
   return v;
 }
 
+// One DPP move of a double (both halves), lanes without a source or outside ROW_MASK reading 0.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_move_f64(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, ROW_MASK, 0xf, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROW_MASK, 0xf, true);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// The sum over the 64 lanes as a wave-uniform value, in a fixed order: prefix sums within each
+// row of 16 lanes (row_shr 1, 2, 4, 8), then the row totals carried into lane 63 by row_bcast 15
+// (rows 1 and 3) and row_bcast 31 (rows 2 and 3), and lane 63 read back. All register moves, no
+// LDS traffic; several times shorter than the shuffle butterfly of wave_sum.
+__device__ __forceinline__ double wave_total(double v) {
+  v += dpp_move_f64<0x111, 0xf>(v);
+  v += dpp_move_f64<0x112, 0xf>(v);
+  v += dpp_move_f64<0x114, 0xf>(v);
+  v += dpp_move_f64<0x118, 0xf>(v);
+  v += dpp_move_f64<0x142, 0xa>(v);
+  v += dpp_move_f64<0x143, 0xc>(v);
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, 63);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
 __device__ inline double wave_max(double v) {
   for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, kWave));
   return v;

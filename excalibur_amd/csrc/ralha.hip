@@ -14,8 +14,11 @@
 #include <cstring>
 
 #include "exc_common.hpp"
+#include "solve.hpp"
 
 namespace {
+
+using fact::readlane_f64;
 
 constexpr int kColsPerWg = 4;  // columns per step workgroup (one wavefront each)
 
@@ -27,39 +30,47 @@ __device__ inline double dlapy2_dev(double x, double y) {
   return w * sqrt(1.0 + q * q);
 }
 
-// Block-wide sum with a fixed order (same result in every workgroup for the same input).
+// Block-wide sum with a fixed order (same result in every workgroup for the same input): the
+// wavefront totals, then every thread adds them in wavefront order. The leading barrier keeps a
+// second call from overwriting red before every thread has read the first call's totals.
 __device__ inline double block_sum(double v, double* red) {
-  v = wave_sum(v);
+  v = wave_total(v);
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   __syncthreads();
   if (lane == 0) red[w] = v;
   __syncthreads();
   double t = 0.0;
-  if (threadIdx.x == 0)
-    for (int q = 0; q < (int)(blockDim.x >> 6); q++) t += red[q];
-  __syncthreads();
-  if (threadIdx.x == 0) red[0] = t;
-  __syncthreads();
-  const double out = red[0];
-  __syncthreads();
-  return out;
+  for (int q = 0; q < (int)(blockDim.x >> 6); q++) t += red[q];
+  return t;
 }
 
 // The reflector of step i from the reduced message g (LAPACK dlarfg on n = m-1-i entries, the
 // same fixed-order reduction in every workgroup): hbuf[0..n) = h (h_0 = 1), returns tau; also
-// beta_{i-1} and alpha_i from the four reduced dots (client.cpp:858-884). Workgroup 0 stores
-// column i+1 of house, tau[i+1], beta[i-1] and alpha[i]. Ends with a barrier.
+// beta_{i-1}, alpha_i and 1 / alpha_i from the four reduced dots (client.cpp:858-884). Workgroup 0
+// stores column i+1 of house, tau[i+1], beta[i-1] and alpha[i]. Ends with a barrier.
+//
+// Every dependent FP64 divide or square root is a chain of about ten operations, so the chain is
+// kept short: alpha_i and its reciprocal are formed before the reflector's branch (they are
+// independent of it), and beta = -sign(alpha) sqrt(alpha^2 + ||x||^2) is taken directly while
+// both terms are far from overflow and underflow; outside that range dlapy2's scaled form and
+// dlarfg's rescaling are used as in LAPACK.
 __device__ double ralha_reflector(int i, int m, const double* __restrict__ g, double* hbuf, double* red,
                                   double* __restrict__ house, double* __restrict__ tau, double* __restrict__ alpha,
-                                  double* __restrict__ beta, double* out_b, double* out_a) {
+                                  double* __restrict__ beta, double* out_b, double* out_a, double* out_ia) {
   const int tid = threadIdx.x;
   const int n = m - 1 - i;
   double ss = 0.0;
   for (int r = 1 + tid; r < n; r += blockDim.x) ss += g[r] * g[r];
-  double xnorm = sqrt(block_sum(ss, red));
+  const double x2 = block_sum(ss, red);
+  const double b = i > 0 ? g[m - i - 1] : 0.0;
+  const double a = i > 0 ? sqrt(g[m - i] * (b * b) + g[m - i + 1] - 2 * b * g[m - i + 2]) : 0.0;
+  *out_ia = i > 0 ? 1.0 / a : 0.0;
   double alph = g[0], t = 0.0, scale = 0.0;
-  if (n > 1 && xnorm != 0.0) {
-    double bet = -copysign(dlapy2_dev(alph, xnorm), alph);
+  if (n > 1 && x2 != 0.0) {
+    const double a2 = alph * alph;
+    const double big = fmax(a2, x2);
+    double bet = (big > 1e-280 && big < 1e280) ? -copysign(sqrt(a2 + x2), alph)
+                                               : -copysign(dlapy2_dev(alph, sqrt(x2)), alph);
     double xs = 1.0;
     const double safmin = DBL_MIN / (DBL_EPSILON * 0.5);
     if (fabs(bet) < safmin) {
@@ -73,15 +84,13 @@ __device__ double ralha_reflector(int i, int m, const double* __restrict__ g, do
       } while (fabs(bet) < safmin && knt < 20);
       double s2 = 0.0;
       for (int r = 1 + tid; r < n; r += blockDim.x) s2 += (g[r] * xs) * (g[r] * xs);
-      xnorm = sqrt(block_sum(s2, red));
+      const double xnorm = sqrt(block_sum(s2, red));
       bet = -copysign(dlapy2_dev(alph, xnorm), alph);
     }
     t = (bet - alph) / bet;
     scale = xs * (1.0 / (alph - bet));
   }
   for (int r = tid; r < n; r += blockDim.x) hbuf[r] = r == 0 ? 1.0 : g[r] * scale;
-  const double b = i > 0 ? g[m - i - 1] : 0.0;
-  const double a = i > 0 ? sqrt(g[m - i] * (b * b) + g[m - i + 1] - 2 * b * g[m - i + 2]) : 0.0;
   if (blockIdx.x == 0) {
     for (int r = tid; r < n; r += blockDim.x) house[(long)(i + 1 + r) * m + (i + 1)] = r == 0 ? 1.0 : g[r] * scale;
     if (tid == 0) {
@@ -100,8 +109,8 @@ __device__ double ralha_reflector(int i, int m, const double* __restrict__ g, do
 
 // Sum over the workgroups' partial rows of entry e (partial[v * ld + e], v < nwg) in a fixed
 // order: sixteen groups of 16 threads (group g = threadIdx.x / 16, entry lane threadIdx.x % 16)
-// each add the partials v = g, g + 16, g + 32, ... in turn, four loads in flight; then the
-// sixteen group sums are added in group order. A 256-thread workgroup handles 16 entries per
+// each add the partials v = g, g + 16, g + 32, ... in turn, up to eight loads in flight at
+// once; then the sixteen group sums are added in group order. A 256-thread workgroup handles 16 entries per
 // call; the result is returned to threads 0..15 (entry lane threadIdx.x). With COHERENT the
 // partials are read with device-scope atomics (written by other workgroups of a running grid).
 template <bool COHERENT>
@@ -114,15 +123,14 @@ __device__ __forceinline__ double partials_sum(const double* __restrict__ partia
       const double* p = partial + (long)v * ld + e;
       return COHERENT ? __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : *p;
     };
-    int v = g;
-    for (; v + 48 < nwg; v += 64) {
-      double x[4];
+    for (int v0 = g; v0 < nwg; v0 += 128) {
+      double x[8];
 #pragma unroll
-      for (int q = 0; q < 4; q++) x[q] = ld1(v + 16 * q);
+      for (int q = 0; q < 8; q++) x[q] = v0 + 16 * q < nwg ? ld1(v0 + 16 * q) : 0.0;
 #pragma unroll
-      for (int q = 0; q < 4; q++) s += x[q];
+      for (int q = 0; q < 8; q++)
+        if (v0 + 16 * q < nwg) s += x[q];
     }
-    for (; v < nwg; v += 16) s += ld1(v);
   }
   acc[g][l] = s;
   __syncthreads();
@@ -169,8 +177,8 @@ __device__ __forceinline__ void reg_load(RegCols<KM, CPW>& R, int i, int m, int 
 // Step i >= 0 on the registers (client.cpp:858-891): the reflector from the reduced message g,
 // F[i+1:, col] -= tau h (h . F[i+1:, col]), then f_i -= beta u_{i-1} and u_i = f_i / alpha_i
 // (stored to U); for i = 0, u_0 comes from the tail kernels. Leaves u_i in R.ui on every lane.
-// The CPW columns' dot products run side by side (four partial sums each, one butterfly over
-// all of them), so their latencies overlap.
+// The CPW columns' dot products run side by side (four fused partial sums each, then a DPP wave
+// total per column), so their latencies overlap.
 template <int KM, int CPW>
 __device__ __forceinline__ void reg_apply(RegCols<KM, CPW>& R, int i, int m, int c, const double* __restrict__ g,
                                           double* hbuf, double* red, double* __restrict__ Uc,
@@ -178,7 +186,8 @@ __device__ __forceinline__ void reg_apply(RegCols<KM, CPW>& R, int i, int m, int
                                           double* __restrict__ alpha, double* __restrict__ beta) {
   const int lane = threadIdx.x & 63;
   double b, a;
-  const double t = ralha_reflector(i, m, g, hbuf, red, house, tau, alpha, beta, &b, &a);
+  double ia;
+  const double t = ralha_reflector(i, m, g, hbuf, red, house, tau, alpha, beta, &b, &a, &ia);
   if (t != 0.0) {
     // h from LDS (zero outside rows i+1 .. m-1); read twice rather than held in KM registers
     auto h = [&](int q) {
@@ -191,33 +200,27 @@ __device__ __forceinline__ void reg_apply(RegCols<KM, CPW>& R, int i, int m, int
       double p0 = 0.0, p1 = 0.0, p2 = 0.0, p3 = 0.0;
 #pragma unroll
       for (int q = 0; q < KM; q += 4) {
-        p0 += R.fr[cc][q] * h(q);
-        if (q + 1 < KM) p1 += R.fr[cc][q + 1] * h(q + 1);
-        if (q + 2 < KM) p2 += R.fr[cc][q + 2] * h(q + 2);
-        if (q + 3 < KM) p3 += R.fr[cc][q + 3] * h(q + 3);
+        p0 = fma(R.fr[cc][q], h(q), p0);
+        if (q + 1 < KM) p1 = fma(R.fr[cc][q + 1], h(q + 1), p1);
+        if (q + 2 < KM) p2 = fma(R.fr[cc][q + 2], h(q + 2), p2);
+        if (q + 3 < KM) p3 = fma(R.fr[cc][q + 3], h(q + 3), p3);
       }
       s[cc] = (p0 + p1) + (p2 + p3);
     }
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-      double o[CPW];
-#pragma unroll
-      for (int cc = 0; cc < CPW; cc++) o[cc] = __shfl_xor(s[cc], off, kWave);
-#pragma unroll
-      for (int cc = 0; cc < CPW; cc++) s[cc] += o[cc];
-    }
+    for (int cc = 0; cc < CPW; cc++) s[cc] = wave_total(s[cc]);
 #pragma unroll
     for (int q = 0; q < KM; q++) {
       const int row = lane + 64 * q;
       const double th = -t * h(q);
 #pragma unroll
       for (int cc = 0; cc < CPW; cc++)
-        if (R.cols[cc] < c && row > i && row < m) R.fr[cc][q] += s[cc] * th;
+        if (R.cols[cc] < c && row > i && row < m) R.fr[cc][q] = fma(s[cc], th, R.fr[cc][q]);
     }
   }
   const int qi = i >> 6, li = i & 63;
   if (i > 0) {
-    const double inv_a = 1.0 / a;
+    const double inv_a = ia;
 #pragma unroll
     for (int q = 0; q < KM; q++) {
       if (q == qi) {  // uniform; row i sits in lane i % 64 of slot i / 64
@@ -229,7 +232,7 @@ __device__ __forceinline__ void reg_apply(RegCols<KM, CPW>& R, int i, int m, int
             R.fr[cc][q] = fi;
             Uc[(long)R.cols[cc] * m + i] = ui;
           }
-          const double u = __shfl(ui, li, kWave);
+          const double u = readlane_f64(ui, li);
           R.ui[cc] = R.cols[cc] < c ? u : 0.0;
         }
       }
@@ -277,9 +280,9 @@ __device__ __forceinline__ void reg_share(const RegCols<KM, CPW>& R, int j, int 
 #pragma unroll
     for (int q = 0; q < KM; q++)
       if (lane + 64 * q == j) fj = R.fr[cc][q];
-    fj = __shfl(fj, j & 63, kWave);
+    fj = readlane_f64(fj, j & 63);
 #pragma unroll
-    for (int q = 0; q < KM; q++) pr[q] += R.fr[cc][q] * fj;
+    for (int q = 0; q < KM; q++) pr[q] = fma(R.fr[cc][q], fj, pr[q]);
     if (j > 0) {
       const double u = R.ui[cc];
       d0 += fj * u;
@@ -354,6 +357,20 @@ __global__ __launch_bounds__(256) void k_ralha_step_reg(int i, int m, int c, dou
 // store. A flag is read with the same relaxed loads, and the data after it with them too, so no
 // cache can hold a stale copy.
 //
+// The grid barrier before the slice reduction is a flag per workgroup (its step number), polled
+// by one wavefront with one load per 64 flags: no atomic on a shared counter, whose updates from
+// every workgroup would be serialized at one address.
+//
+// One owner (boxes with P = 1, the bidiagonalization of a matrix a single peer holds) needs no
+// flags at all: its partial rows and message slots start filled with a sentinel (a signaling NaN
+// bit pattern that no arithmetic produces), a reader polls the values themselves until none is the
+// sentinel, and the reader of a partial or the owner of a message slice puts the sentinel back
+// once every workgroup is known to have read it. That removes the barrier and the wait for each
+// flag's data to be acknowledged before the flag is raised. Ordering of the resets: a workgroup
+// waits for its own stores (vmcnt(0)) at the end of each receive, before it stores its next
+// partial row; every later write to a reset location, and every read that must see the reset,
+// happens only after some workgroup has read that partial row.
+//
 // Every wait is bounded by a wall-clock limit and also ends on dsync[kStop] (set by any
 // workgroup that gave up) or link.abort (set by the host), so the grid always drains. The grid is
 // small (one workgroup per 4 x CPW columns) and the host launches it only when twice that many
@@ -390,7 +407,10 @@ struct PeerBoxes {  // kernel argument; P = 0 selects the host link
   char* box[kMaxPeers];
 };
 
-enum { kGin = 0, kBarrier = 1, kStop = 2 };  // device words
+enum { kGin = 0, kStop = 2 };  // device words; the barrier flags follow at kFlagsOffset bytes
+constexpr int kFlagsOffset = 256;      // workgroup w's barrier flag: ((int*)(dsync + 64))[w]
+constexpr int kSyncBytes = 256 + 4 * kLoopMaxWg;
+constexpr unsigned long long kSentinel = 0x7FF0DEAD0000BEEFull;  // a signaling NaN: no arithmetic yields it
 
 __device__ __forceinline__ double ld_dev(const double* p) { return coh_ld(p); }
 __device__ __forceinline__ void st_dev(double* p, double v) { coh_st(p, v); }
@@ -415,7 +435,8 @@ __device__ __forceinline__ bool loop_wait_dev(int* word, int target, int* dsync,
   if (threadIdx.x == 0) {
     int ok = 1;
     const long long t0 = wall_clock64();
-    while (__hip_atomic_load(word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+    for (unsigned k = 0; __hip_atomic_load(word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target; k++) {
+      if ((k & 15) != 15) continue;  // the stop flag and the clock every 16 polls: one load in flight per poll
       if (__hip_atomic_load(&dsync[kStop], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
         ok = 0;
         break;
@@ -425,9 +446,47 @@ __device__ __forceinline__ bool loop_wait_dev(int* word, int target, int* dsync,
         ok = 0;
         break;
       }
-      __builtin_amdgcn_s_sleep(1);
     }
     sh[0] = ok;
+  }
+  __syncthreads();
+  const bool ok = sh[0] != 0;
+  __syncthreads();
+  return ok;
+}
+
+__device__ __forceinline__ int* barrier_flags(int* dsync) { return (int*)((char*)dsync + kFlagsOffset); }
+
+// Wavefront 0 waits until flags[0 .. G) (G <= 256) all reach target, one load per 64 flags per
+// poll, all in flight; the verdict reaches every thread through LDS.
+__device__ __forceinline__ bool wait_all(const int* flags, int G, int target, int* dsync, LoopLink* link,
+                                         long long limit, int* sh) {
+  if (threadIdx.x < 64) {
+    const int lane = threadIdx.x;
+    int ok = 1;
+    const long long t0 = wall_clock64();
+    for (unsigned k = 0;; k++) {
+      int f[4];
+#pragma unroll
+      for (int q = 0; q < 4; q++)
+        f[q] = lane + 64 * q < G ? __hip_atomic_load(&flags[lane + 64 * q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                 : target;
+      if (__all(f[0] >= target && f[1] >= target && f[2] >= target && f[3] >= target)) break;
+      if ((k & 15) == 15) {
+        const int stop = __builtin_amdgcn_readfirstlane(
+            __hip_atomic_load(&dsync[kStop], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        if (stop != 0) {
+          ok = 0;
+          break;
+        }
+        if (wall_clock64() - t0 > limit) {
+          if (lane == 0) loop_fail(dsync, link, 1);
+          ok = 0;
+          break;
+        }
+      }
+    }
+    if (lane == 0) sh[0] = ok;
   }
   __syncthreads();
   const bool ok = sh[0] != 0;
@@ -476,8 +535,9 @@ __device__ __forceinline__ bool loop_publish(int j, int m, const double* __restr
   const int G = gridDim.x, w = blockIdx.x;
   stores_done();
   __syncthreads();
-  if (threadIdx.x == 0) __hip_atomic_fetch_add(&dsync[kBarrier], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  if (!loop_wait_dev(&dsync[kBarrier], G * (j + 1), dsync, link, limit, sh)) return false;
+  if (threadIdx.x == 0)
+    __hip_atomic_store(&barrier_flags(dsync)[w], j + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (!wait_all(barrier_flags(dsync), G, j + 1, dsync, link, limit, sh)) return false;
   const int len = m + 3 - j;
   const int valid = (m - 1 - j) + (j > 0 ? 4 : 0);
   const int per = (len + G - 1) / G;
@@ -546,8 +606,9 @@ __device__ __forceinline__ bool box_publish(int j, int m, const double* __restri
   const int G = gridDim.x, w = blockIdx.x;
   stores_done();
   __syncthreads();
-  if (threadIdx.x == 0) __hip_atomic_fetch_add(&dsync[kBarrier], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  if (!loop_wait_dev(&dsync[kBarrier], G * (j + 1), dsync, link, limit, sh)) return false;
+  if (threadIdx.x == 0)
+    __hip_atomic_store(&barrier_flags(dsync)[w], j + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (!wait_all(barrier_flags(dsync), G, j + 1, dsync, link, limit, sh)) return false;
   const int len = m + 3 - j;
   const int valid = (m - 1 - j) + (j > 0 ? 4 : 0);
   const int per = (len + G - 1) / G;
@@ -631,12 +692,136 @@ __device__ __forceinline__ bool box_receive(int i, int m, const PeerBoxes& B, Lo
   return true;
 }
 
+// ---- One owner: partial rows and message slots polled for the sentinel ----
+//
+// Workspace (exc_ralha_loop_workspace): solo[(2 G + 3) L], L = loop_ld(m): partial row v of step
+// j at solo + ((j & 1) G + v) L, message slot s (= j % 3) at solo + (2 G + s) L.
+__host__ __device__ inline int loop_ld(int m) { return (m + 3 + 31) / 32 * 32; }
+
+__device__ __forceinline__ bool is_sentinel(double v) {
+  return (unsigned long long)__double_as_longlong(v) == kSentinel;
+}
+
+// Every 16th poll: false when another workgroup gave up or the wall-clock limit passed.
+__device__ __forceinline__ bool poll_ok(unsigned k, long long t0, long long limit, int* dsync, LoopLink* link) {
+  if ((k & 15) != 15) return true;
+  if (__hip_atomic_load(&dsync[kStop], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return false;
+  if (wall_clock64() - t0 > limit) {
+    loop_fail(dsync, link, 1);
+    return false;
+  }
+  return true;
+}
+
+__global__ void k_solo_fill(double* __restrict__ p, long count) {
+  const double sv = __longlong_as_double((long long)kSentinel);
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < count; e += (long)gridDim.x * blockDim.x) p[e] = sv;
+}
+
+// Message j of one owner: this workgroup's slice of the G partial rows of step j, polled until no
+// entry is the sentinel and summed in the order of partials_sum, goes to slot j % 3. The partial
+// entries read go back to the sentinel, and so does this slice of message j - 1's slot (all G
+// partials of step j have arrived, so every workgroup has read message j - 1); the last slice
+// also resets entry len, which message j - 1 had and message j lacks.
+__device__ __forceinline__ bool solo_publish(int j, int m, double* __restrict__ solo, int* dsync, LoopLink* link,
+                                             long long limit, double (*acc)[16]) {
+  const int G = gridDim.x, w = blockIdx.x, L = loop_ld(m);
+  const double sv = __longlong_as_double((long long)kSentinel);
+  const int len = m + 3 - j;
+  const int valid = (m - 1 - j) + (j > 0 ? 4 : 0);
+  const int per = (len + G - 1) / G;
+  const int r0 = w * per, r1 = min(len, r0 + per);
+  double* P = solo + (long)(j & 1) * G * L;
+  double* M = solo + (2L * G + j % 3) * L;
+  const int gq = threadIdx.x >> 4, l = threadIdx.x & 15;
+  int ok = 1;
+  for (int base = r0; base < r1; base += 16) {
+    const int e = base + l;
+    const bool active = e < r1 && e < valid;
+    double s = 0.0;
+    for (int v0 = gq; v0 < G; v0 += 128) {
+      double x[8];
+#pragma unroll
+      for (int q = 0; q < 8; q++) x[q] = (active && v0 + 16 * q < G) ? ld_dev(&P[(long)(v0 + 16 * q) * L + e]) : 0.0;
+      const long long t0 = wall_clock64();
+      for (unsigned k = 0; ok; k++) {
+        bool pend = false;
+#pragma unroll
+        for (int q = 0; q < 8; q++) pend |= is_sentinel(x[q]);
+        if (!pend) break;
+#pragma unroll
+        for (int q = 0; q < 8; q++)
+          if (is_sentinel(x[q])) x[q] = ld_dev(&P[(long)(v0 + 16 * q) * L + e]);
+        if (!poll_ok(k, t0, limit, dsync, link)) ok = 0;
+      }
+#pragma unroll
+      for (int q = 0; q < 8; q++)
+        if (active && v0 + 16 * q < G) s += x[q];
+#pragma unroll
+      for (int q = 0; q < 8; q++)
+        if (active && v0 + 16 * q < G) st_dev(&P[(long)(v0 + 16 * q) * L + e], sv);
+    }
+    acc[gq][l] = s;
+    __syncthreads();
+    if (threadIdx.x < 16 && e < r1) {
+      double t = acc[0][l];
+#pragma unroll
+      for (int q = 1; q < 16; q++) t += acc[q][l];
+      st_dev(&M[e], t);
+    }
+    __syncthreads();
+  }
+  if (j >= 1 && r0 < r1) {
+    double* Mp = solo + (2L * G + (j - 1) % 3) * L;
+    const int p1 = r1 == len ? len + 1 : r1;
+    for (int e = r0 + threadIdx.x; e < p1; e += blockDim.x) st_dev(&Mp[e], sv);
+  }
+  return __syncthreads_and(ok) != 0;
+}
+
+// The whole message i (m + 3 - i doubles) from slot i % 3 into gl, polled until complete; then
+// this workgroup's stores (the resets) are waited for, before its next partial row.
+template <int KM>
+__device__ __forceinline__ bool solo_receive(int i, int m, const double* __restrict__ solo, double* gl, int* dsync,
+                                             LoopLink* link, long long limit) {
+  const int G = gridDim.x, L = loop_ld(m);
+  const int len = m + 3 - i;
+  const double* M = solo + (2L * G + i % 3) * L;
+  constexpr int kPer = (64 * KM + 3 + 255) / 256;
+  double v[kPer];
+#pragma unroll
+  for (int q = 0; q < kPer; q++) {
+    const int r = threadIdx.x + 256 * q;
+    v[q] = r < len ? ld_dev(&M[r]) : 0.0;
+  }
+  int ok = 1;
+  const long long t0 = wall_clock64();
+  for (unsigned k = 0; ok; k++) {
+    bool pend = false;
+#pragma unroll
+    for (int q = 0; q < kPer; q++) pend |= is_sentinel(v[q]);
+    if (!pend) break;
+#pragma unroll
+    for (int q = 0; q < kPer; q++)
+      if (is_sentinel(v[q])) v[q] = ld_dev(&M[threadIdx.x + 256 * q]);
+    if (!poll_ok(k, t0, limit, dsync, link)) ok = 0;
+  }
+#pragma unroll
+  for (int q = 0; q < kPer; q++) {
+    const int r = threadIdx.x + 256 * q;
+    if (r < len) gl[r] = v[q];
+  }
+  stores_done();
+  return __syncthreads_and(ok) != 0;
+}
+
 template <int KM, int CPW>
 __global__ __launch_bounds__(256, KM * CPW <= 32 ? 2 : 1) void k_ralha_loop(int m, int c, double* __restrict__ Fc, double* __restrict__ Uc,
                                                     double* __restrict__ house, double* __restrict__ tau,
                                                     double* __restrict__ alpha, double* __restrict__ beta,
                                                     char* link_bytes, int* dsync, double* __restrict__ gdev,
-                                                    double* __restrict__ partial, long long limit, PeerBoxes B) {
+                                                    double* __restrict__ partial, double* __restrict__ solo,
+                                                    long long limit, PeerBoxes B) {
   extern __shared__ double lds[];  // hbuf: m, mp: 4 x (m + 4), gl: m + 3
   double* hbuf = lds;
   double* mp = lds + m;
@@ -649,30 +834,37 @@ __global__ __launch_bounds__(256, KM * CPW <= 32 ? 2 : 1) void k_ralha_loop(int 
   double* mine = partial + (long)blockIdx.x * (m + 3);
   RegCols<KM, CPW> R;
   reg_load(R, -1, m, c, Fc, Uc);
-  const bool boxes = B.P > 0;  // uniform
-  reg_share<KM, CPW, true>(R, 0, m, c, mp, mine);
-  if (!(boxes ? box_publish(0, m, partial, B, link, dsync, limit, sh, acc)
-              : loop_publish(0, m, partial, link, hmsg, dsync, limit, sh, acc)))
-    return;
+  const bool boxes = B.P > 0;   // uniform
+  const bool owner = B.P == 1;  // one owner: the sentinel-polled buffers
+  // this workgroup's partial row of message j
+  auto row = [&](int j) { return owner ? solo + ((long)(j & 1) * gridDim.x + blockIdx.x) * loop_ld(m) : mine; };
+  auto publish = [&](int j) {
+    return owner ? solo_publish(j, m, solo, dsync, link, limit, acc)
+                 : boxes ? box_publish(j, m, partial, B, link, dsync, limit, sh, acc)
+                         : loop_publish(j, m, partial, link, hmsg, dsync, limit, sh, acc);
+  };
+  reg_share<KM, CPW, true>(R, 0, m, c, mp, row(0));
+  if (!publish(0)) return;
   for (int i = 0; i <= m - 3; i++) {
-    if (!(boxes ? box_receive<KM>(i, m, B, link, gl, dsync, limit, sh)
-                : loop_receive<KM>(i, m, link, hmsg, gdev, gl, dsync, limit, sh)))
+    if (!(owner ? solo_receive<KM>(i, m, solo, gl, dsync, link, limit)
+                : boxes ? box_receive<KM>(i, m, B, link, gl, dsync, limit, sh)
+                        : loop_receive<KM>(i, m, link, hmsg, gdev, gl, dsync, limit, sh)))
       return;
     reg_apply(R, i, m, c, gl, hbuf, red, Uc, house, tau, alpha, beta);
 #pragma unroll
     for (int cc = 0; cc < CPW; cc++) R.um1[cc] = R.ui[cc];
     if (i + 1 <= m - 3) {
-      reg_share<KM, CPW, true>(R, i + 1, m, c, mp, mine);
-      if (!(boxes ? box_publish(i + 1, m, partial, B, link, dsync, limit, sh, acc)
-                  : loop_publish(i + 1, m, partial, link, hmsg, dsync, limit, sh, acc)))
-        return;
+      reg_share<KM, CPW, true>(R, i + 1, m, c, mp, row(i + 1));
+      if (!publish(i + 1)) return;
     }
   }
   reg_store(R, 0, m, c, Fc);
 }
 
+// The device words and every workgroup's barrier flag to zero (256 threads).
 __global__ void k_loop_clear(int* dsync) {
-  if (threadIdx.x < 4) dsync[threadIdx.x] = 0;
+  if (threadIdx.x < 64) dsync[threadIdx.x] = 0;
+  barrier_flags(dsync)[threadIdx.x] = 0;
 }
 
 // One column step of the overlapped Ralha loop on column-major F and U (Fc[col*m + row]):
@@ -694,7 +886,8 @@ __global__ __launch_bounds__(256) void k_ralha_step(int i, int m, int c, double*
   if (i >= 0) {
     const int n = m - 1 - i;
     double b, a;
-    const double t = ralha_reflector(i, m, g, hbuf, red, house, tau, alpha, beta, &b, &a);
+    double ia;
+  const double t = ralha_reflector(i, m, g, hbuf, red, house, tau, alpha, beta, &b, &a, &ia);
     for (int q = wave; q < kColsPerWg; q += (int)(blockDim.x >> 6)) {
       const int col = c0 + q;
       if (col >= c) break;
@@ -982,10 +1175,14 @@ int exc_ralha_step_generic(int i, int m, int c, double* d_fc, double* d_uc, cons
                     stream);
 }
 
+// [device words and barrier flags][gdev: m + 3][partial rows: G x (m + 3)][one owner's sentinel
+// buffers: (2 G + 3) x loop_ld(m)], G the largest grid (one column per wavefront).
 size_t exc_ralha_loop_workspace(int m, int c) {
   if (m < 3 || c < 1) return 0;
   const size_t g = ((size_t)(m + 3) * sizeof(double) + 255) / 256 * 256;
-  return 256 + g + (size_t)exc_detail::ceil_div(c, 4) * (m + 3) * sizeof(double);
+  const size_t G = exc_detail::ceil_div(c, 4);
+  const size_t part = (G * (m + 3) * sizeof(double) + 255) / 256 * 256;
+  return kSyncBytes + g + part + (2 * G + 3) * (size_t)loop_ld(m) * sizeof(double);
 }
 
 int exc_ralha_link_alloc(int m, void** h_link) {
@@ -1164,12 +1361,19 @@ int loop_start(int m, int c, double* d_fc, double* d_uc, double* d_house, double
   const long long limit = (long long)(timeout_s * 1e3 * khz);
   char* w = (char*)d_work;
   int* dsync = (int*)w;
-  double* gdev = (double*)(w + 256);
-  double* partial = (double*)(w + 256 + ((size_t)(m + 3) * sizeof(double) + 255) / 256 * 256);
-  hipLaunchKernelGGL(k_loop_clear, dim3(1), dim3(64), 0, s, dsync);
+  const size_t g = ((size_t)(m + 3) * sizeof(double) + 255) / 256 * 256;
+  const size_t part = ((size_t)exc_detail::ceil_div(c, 4) * (m + 3) * sizeof(double) + 255) / 256 * 256;
+  double* gdev = (double*)(w + kSyncBytes);
+  double* partial = (double*)(w + kSyncBytes + g);
+  double* solo = (double*)(w + kSyncBytes + g + part);
+  hipLaunchKernelGGL(k_loop_clear, dim3(1), dim3(kLoopMaxWg), 0, s, dsync);
+  if (B.P == 1) {
+    const long count = (2L * sh.nwg + 3) * loop_ld(m);
+    hipLaunchKernelGGL(k_solo_fill, dim3(std::min<long>(1024, (count + 255) / 256)), dim3(256), 0, s, solo, count);
+  }
   PeerBoxes b = B;
   void* args[] = {&m, &c, &d_fc, &d_uc, &d_house, &d_tau, &d_alpha, &d_beta, &d_link, &dsync, &gdev, &partial,
-                  (void*)&limit, &b};
+                  &solo, (void*)&limit, &b};
   EXC_CHECK_HIP(hipLaunchKernel(fn, dim3(sh.nwg), dim3(256), args, sh.lds, s));
   return EXC_OK;
 }
