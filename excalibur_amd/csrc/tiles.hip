@@ -171,50 +171,63 @@ __device__ inline void wave_sync() {
 
 // Cholesky of the diagonal tile in S (padded rows/cols identity), lane i = row i. Returns the
 // first non-positive pivot (1-based) or 0; such a pivot is replaced by 1 like the blocked path.
-// Panels of PW columns: inside a panel the pivot and the multipliers L[k][j] reach the other lanes
-// by v_readlane (no LDS, no barrier); after the panel its columns go through LDS once for the
-// trailing update. Every element receives the same fused multiply-adds in the same order as a
-// column-by-column right-looking factorization.
+// Every element receives the same fused multiply-adds in the same order as a column-by-column
+// right-looking factorization.
+//
+// The routine runs once per diagonal tile on one wavefront, so every instruction is fetched cold:
+// a fully unrolled form (tens of KB of straight-line code) was bound by instruction fetch, not
+// arithmetic. So only a panel of PW columns is unrolled (inside it the pivot and the multipliers
+// L[k][j] reach the other lanes by v_readlane, no LDS, no barrier); the panel loop is not, and the
+// panel's rank-PW update of the later columns runs as a loop over blocks of 8 columns with eight
+// independent chains, the panel's multipliers read back from LDS (xc).
 constexpr int PW = 8;
-__device__ __forceinline__ int potf2_reg(double* S, double* xc, int kb) {
+__device__ __forceinline__ int potf2_loop(double* S, double* xc, int kb) {
   const int lane = threadIdx.x;
-  __asm__ volatile("" ::: "memory");  // keep this routine's loads inside its branch
-  double x[TB];
-#pragma unroll
-  for (int k = 0; k < TB; k++) x[k] = k <= lane ? S[lane * LD + k] : 0.0;
+  __asm__ volatile("" ::: "memory");
   int bad = 0;
-#pragma unroll
+#pragma unroll 1
   for (int j0 = 0; j0 < TB; j0 += PW) {
-    double lown[PW];  // L[lane][j] for the panel (zero on and above the diagonal)
+    double x[PW], lown[PW];
 #pragma unroll
-    for (int j = j0; j < j0 + PW; j++) {
-      double d = readlane_f64(x[j], j);
+    for (int jj = 0; jj < PW; jj++) x[jj] = S[lane * LD + j0 + jj];
+#pragma unroll
+    for (int jj = 0; jj < PW; jj++) {
+      const int j = j0 + jj;
+      double d = readlane_f64(x[jj], j);
       if (!(d > 0.0)) {
         if (!bad && j < kb) bad = j + 1;
         d = 1.0;
       }
       double rp, piv;
       rsqrt_pair(d, &rp, &piv);
-      const double l = lane > j ? x[j] * rp : 0.0;
-      x[j] = lane == j ? piv : (lane > j ? l : x[j]);
-      lown[j - j0] = l;
+      const double l = lane > j ? x[jj] * rp : 0.0;
+      x[jj] = lane == j ? piv : (lane > j ? l : x[jj]);
+      lown[jj] = l;
 #pragma unroll
-      for (int k = j + 1; k < j0 + PW; k++) x[k] = fma(-l, readlane_f64(l, k), x[k]);
+      for (int kk = jj + 1; kk < PW; kk++) x[kk] = fma(-l, readlane_f64(l, j0 + kk), x[kk]);
     }
-    if (j0 + PW < TB) {
 #pragma unroll
-      for (int jj = 0; jj < PW; jj++) xc[lane * (PW + 1) + jj] = lown[jj];
-      wave_sync();
-#pragma unroll
-      for (int k = j0 + PW; k < TB; k++) {
-#pragma unroll
-        for (int jj = 0; jj < PW; jj++) x[k] = fma(-lown[jj], xc[k * (PW + 1) + jj], x[k]);
-      }
-      wave_sync();
+    for (int jj = 0; jj < PW; jj++) {
+      S[lane * LD + j0 + jj] = x[jj];
+      xc[lane * (PW + 1) + jj] = lown[jj];
     }
+    wave_sync();
+#pragma unroll 1
+    for (int k0 = j0 + PW; k0 < TB; k0 += 8) {
+      double a[8];
+#pragma unroll
+      for (int kk = 0; kk < 8; kk++) a[kk] = S[lane * LD + k0 + kk];
+#pragma unroll
+      for (int jj = 0; jj < PW; jj++)
+#pragma unroll
+        for (int kk = 0; kk < 8; kk++) a[kk] = fma(-lown[jj], xc[(k0 + kk) * (PW + 1) + jj], a[kk]);
+#pragma unroll
+      for (int kk = 0; kk < 8; kk++) S[lane * LD + k0 + kk] = a[kk];
+    }
+    wave_sync();
   }
-#pragma unroll
-  for (int k = 0; k < TB; k++) S[lane * LD + k] = k <= lane ? x[k] : 0.0;
+#pragma unroll 1
+  for (int k = lane + 1; k < TB; k++) S[lane * LD + k] = 0.0;
   return bad;
 }
 
@@ -342,7 +355,7 @@ __device__ __forceinline__ void potrf_tile_run(int n, double* __restrict__ A, lo
   if (ti == tj) {
     pad_identity(S, nr);
     __syncthreads();
-    const int bad = potf2_reg(S, X, nr);
+    const int bad = potf2_loop(S, X, nr);
     if (threadIdx.x == 0 && bad) atomicCAS(&error[1], 0, (int)r0 + bad);
     __syncthreads();
     lds_to_tile(S, A, lda, r0, c0, nr, nc, true);
