@@ -68,6 +68,36 @@ def test_trsm_left(hip, upper, unit, n, nrhs):
     assert np.max(np.abs(got - ref)) <= 1e-12 * np.abs(ref).max()
 
 
+@pytest.mark.parametrize("upper", [False, True])
+@pytest.mark.parametrize("unit", [False, True])
+@pytest.mark.parametrize("n,nrhs", [(100, 16384), (128, 20000), (300, 16400), (1000, 16384)])
+def test_trsm_left_wide(hip, upper, unit, n, nrhs):
+    """Wide right-hand sides (the leaves' many-wavefront form): the solution matches LAPACK's, and
+    on a graded triangle (an R factor of singular values falling over six decades, the triangles
+    the Cholesky QR passes solve with) the solve stays backward stable: ||T X - B|| is at rounding
+    level relative to ||T|| ||X|| (relative to ||B|| it can reach u cond(T), for LAPACK too)."""
+    import scipy.linalg as sl
+    banner(f"trsm_left wide n={n} nrhs={nrhs} upper={upper} unit={unit}")
+    rng = np.random.default_rng(n + nrhs + 2 * upper + unit)
+    T = rng.standard_normal((n, n)) * 0.1 + np.eye(n) * 2
+    T = np.triu(T) if upper else np.tril(T)
+    X = rng.standard_normal((n, nrhs))
+    got = hip.trsm_left(T, X, upper, unit)
+    ref = sl.solve_triangular(T, X, lower=not upper, unit_diagonal=unit)
+    assert np.max(np.abs(got - ref)) <= 1e-12 * np.abs(ref).max()
+    if unit:
+        return
+    # graded: R factor of a matrix with singular values 1 .. 1e-6
+    q1, _ = np.linalg.qr(rng.standard_normal((2 * n, n)))
+    q2, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    R = np.linalg.qr((q1 * np.geomspace(1.0, 1e-6, n)) @ q2.T, mode="r")
+    G = R if upper else np.ascontiguousarray(R.T)
+    got = hip.trsm_left(G, X, upper, False)
+    # normwise backward error (substitution: 1e-17 and below here; LAPACK's dtrtrs alike)
+    back = np.linalg.norm(G @ got - X) / (np.linalg.norm(G) * np.linalg.norm(got))
+    assert back <= 1e-15, back
+
+
 @pytest.mark.parametrize("lower_t", [False, True])
 @pytest.mark.parametrize("n,rows", [(3, 5), (128, 64), (300, 1000), (700, 130), (200, 5000), (64, 4100)])
 def test_trsm_right(hip, lower_t, n, rows):
