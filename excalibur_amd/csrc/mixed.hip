@@ -308,12 +308,17 @@ int exc_unpack_lower_f64(int n, const double* d_packed, double* d_a, long lda, e
   return EXC_OK;
 }
 
+// A first step that starts from vectors this close to orthonormal (max |I - U^T U|, |I - W^T W|)
+// is the last one: the step's error is about the square of the starting one, below rounding.
+constexpr double kOaOneStepDev = 1e-10;
+
 size_t exc_oa_refine_workspace(int m) {
   if (m < 1) return 0;
   const size_t mm = (size_t)m * m * sizeof(double);
   const size_t mat = (mm + 255) / 256 * 256;
   return 8 * mat + 3 * 256 + ((size_t)m * sizeof(double) + 255) / 256 * 256 +
-         ((size_t)m * sizeof(int) + 255) / 256 * 256 + exc_gemm_workspace(m, m, m) + 256;
+         ((size_t)m * sizeof(int) + 255) / 256 * 256 + exc_gemm_workspace(m, m, m) + 256 +
+         (exc_dev_identity_workspace(m) + 255) / 256 * 256 + 256;
 }
 
 int exc_oa_refine(int m, const double* d_l, long ldl, int l_lower, double* d_u, long ldu, double* d_w, long ldw,
@@ -346,6 +351,8 @@ int exc_oa_refine(int m, const double* d_l, long ldl, int l_lower, double* d_u, 
   ExcGemmOpts go;
   go.work = take(exc_gemm_workspace(m, m, m));
   go.work_bytes = exc_gemm_workspace(m, m, m);
+  void* dev_work = take(exc_dev_identity_workspace(m));
+  double* dev = (double*)take(4 * sizeof(double));
   ExcGemmOpts gl = go;
   gl.tri = l_lower ? EXC_TRI_A_LOWER : EXC_TRI_NONE;
   ExcGemmOpts gs = go;
@@ -358,6 +365,16 @@ int exc_oa_refine(int m, const double* d_l, long ldl, int l_lower, double* d_u, 
     if (!rc) rc = exc_gemm_opts(1, 0, m, m, m, 1.0, d_u, ldu, d_u, ldu, 0.0, UtU, m, gs, s);
     if (!rc) rc = exc_gemm_opts(1, 0, m, m, m, 1.0, d_w, ldw, d_w, ldw, 0.0, WtW, m, gs, s);
     if (rc) return rc;
+    if (it == 0 && iters > 1) {
+      // how far the starting vectors are from orthonormal decides whether later steps run
+      rc = exc_dev_identity(m, UtU, m, dev, dev_work, exc_dev_identity_workspace(m), stream);
+      if (!rc) rc = exc_dev_identity(m, WtW, m, dev + 2, dev_work, exc_dev_identity_workspace(m), stream);
+      if (rc) return rc;
+      double h[4];
+      EXC_CHECK_HIP(hipMemcpyAsync(h, dev, sizeof h, hipMemcpyDeviceToHost, s));
+      EXC_CHECK_HIP(hipStreamSynchronize(s));
+      if (std::max(h[0], h[2]) <= kOaOneStepDev) iters = 1;
+    }
     hipLaunchKernelGGL(k_oa_sigma, dim3(1), dim3(1024), 0, s, m, T, UtU, WtW, sig, smax);
     hipLaunchKernelGGL(k_oa_coeffs, dim3(ge), dim3(256), 0, s, m, T, UtU, WtW, sig, tau, F, G);
     hipLaunchKernelGGL(k_oa_members, dim3(exc_detail::ceil_div(m, 256)), dim3(256), 0, s, m, sig, smax, tau,

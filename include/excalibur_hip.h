@@ -341,9 +341,12 @@ size_t exc_dev_identity_workspace(int n);
 int exc_dev_identity(int n, const double* d_a, long lda, double* d_out, void* d_work, size_t work_bytes,
                      exc_stream_t stream);
 /* FP64 Ogita-Aishima refinement of a square SVD L ~ U diag(sigma) W^T (all m x m row-major,
- * L lower triangular when l_lower): `iters` steps of U <- U (I + F), W <- W (I + G) from the
+ * L lower triangular when l_lower): up to `iters` steps of U <- U (I + F), W <- W (I + G) from the
  * residual products I - U^T U, I - W^T W and U^T L W on the MFMA GEMM; sigma receives
- * T_ii / (1 - (R_ii + S_ii) / 2) of the last step. Values within relative distance tau form
+ * T_ii / (1 - (R_ii + S_ii) / 2) of the last step. When the starting vectors are within 1e-10 of
+ * orthonormal (max-entry norm of both deviations), the first step is the last: its error is about
+ * the square of the starting one. Deciding that reads two numbers back (a stream synchronization)
+ * when iters > 1. Values within relative distance tau form
  * clusters that are orthogonalized symmetrically, and the left vectors of cluster members above
  * floor_rel * max sigma are recomputed as L w / sigma. Ogita and Aishima, J. Comput. Appl. Math.
  * 369 (2020). No counterpart in the reference (its FP64 path has no refinement). */

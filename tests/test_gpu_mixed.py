@@ -111,6 +111,29 @@ def test_oa_refine_step_matches_restatement(hip):
     assert np.linalg.norm(L - (U2 * s2) @ W2.T) <= 1e-13 * np.linalg.norm(L)
 
 
+def test_oa_refine_stops_after_one_step_near_orthonormal(hip):
+    """Starting vectors within 1e-10 of orthonormal: the first step is the last (its error is
+    about the square of the starting one), so asking for two steps returns the one-step result
+    bit for bit, at rounding level; 1e-7 starting errors still take both steps (the test above)."""
+    rng = np.random.default_rng(5)
+    m = 200
+    s = np.geomspace(1.0, 1e-5, m)
+    Q1 = np.linalg.qr(rng.standard_normal((m, m)))[0]
+    Q2 = np.linalg.qr(rng.standard_normal((m, m)))[0]
+    L = np.linalg.cholesky(((Q1 * s) @ Q2.T) @ ((Q1 * s) @ Q2.T).T + 1e-30 * np.eye(m))
+    U, sv, Wt = np.linalg.svd(L)
+    U0 = U + U @ (1e-13 * rng.standard_normal((m, m)))
+    W0 = Wt.T + Wt.T @ (1e-13 * rng.standard_normal((m, m)))
+    one = hip.oa_refine(L, U0, W0, iters=1)
+    two = hip.oa_refine(L, U0, W0, iters=2)
+    for a, b in zip(one, two):
+        assert np.array_equal(a, b)
+    Ug, Wg, sg = two
+    for Q in (Ug, Wg):
+        assert np.linalg.norm(Q.T @ Q - np.eye(m), 2) <= 1e-13
+    assert np.linalg.norm(L - (Ug * sg) @ Wg.T) <= 1e-13 * np.linalg.norm(L)
+
+
 def test_oa_refine_cluster(hip):
     """A cluster of 8 values within 1e-10 relative: the step orthogonalizes the cluster's vectors
     and keeps the residual at rounding level, where the unclustered formula would divide by ~0."""
