@@ -155,6 +155,36 @@ def isolated_gemms(n, rows, reps=5):
     return out
 
 
+def isolated_givens(n=1000, passes=32, reps=6, seed=5):
+    """The FP64 Givens mask sweep alone on an n x n matrix (32 passes of disjoint rotations, rows
+    then columns), timed with HIP events through the library's profiler, for PMC runs. Algorithmic
+    bytes per sweep: the matrix read and written once, 2 * 8 * n * n."""
+    import ctypes
+    from excalibur_amd import hip
+    rng = np.random.default_rng(seed)
+    x = hip.DeviceArray.from_host(rng.standard_normal((n, n)))
+    pairs, cs = hip.givens_plan(n, passes, True, seed)
+    dp = hip.DeviceArray.from_host(pairs)
+    dc = hip.DeviceArray.from_host(cs)
+    buf = ctypes.create_string_buffer(1 << 16)
+    out = {}
+    for name, left in (("rows", 1), ("columns", 0)):
+        def launch():
+            hip.call("exc_givens_apply", x.ptr, n, n, dp.ptr, dc.ptr, passes, 1, left, None)
+        launch()
+        hip.sync()
+        hip.call("exc_prof_enable", 1)
+        for _ in range(reps):
+            launch()
+        hip.sync()
+        hip.call("exc_prof_enable", 0)
+        hip.call("exc_prof_report", buf, len(buf))
+        rec = json.loads(buf.value.decode())["givens_f64"]
+        ms = rec["ms"] / rec["launches"]
+        out[name] = {"avg_launch_us": round(1000.0 * ms, 2), "gb_s": round(2.0 * 8 * n * n / (ms * 1e-3) / 1e9, 1)}
+    return out
+
+
 def cpu_info():
     model = "unknown"
     try:
@@ -371,6 +401,8 @@ def main():
                     help="only write the peers' Client{i}.mat files into this directory and exit")
     ap.add_argument("--isolated-only", action="store_true",
                     help="only time the largest products of the configuration alone on the GPU (for PMC runs)")
+    ap.add_argument("--isolated-givens", action="store_true",
+                    help="only time the FP64 Givens mask sweep (1000 x 1000, rows then columns) alone (for PMC runs)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy MIB mebibytes device to device once (HBM traffic calibration for PMC runs)")
     args = ap.parse_args()
@@ -391,6 +423,9 @@ def main():
     if args.isolated_only:
         P, rows, n, _ = CONFIGS[args.config]
         print(json.dumps(isolated_gemms(n, rows)))
+        return
+    if args.isolated_givens:
+        print(json.dumps(isolated_givens()))
         return
     if args.only:
         print(json.dumps(bench_batched() if args.only == "batched"
