@@ -367,9 +367,12 @@ __global__ __launch_bounds__(256) void k_ralha_step_reg(int i, int m, int c, dou
 // sentinel, and the reader of a partial or the owner of a message slice puts the sentinel back
 // once every workgroup is known to have read it. That removes the barrier and the wait for each
 // flag's data to be acknowledged before the flag is raised. Ordering of the resets: a workgroup
-// waits for its own stores (vmcnt(0)) at the end of each receive, before it stores its next
-// partial row; every later write to a reset location, and every read that must see the reset,
-// happens only after some workgroup has read that partial row.
+// waits for its own stores (vmcnt(0)) before each block of its slice is stored, and again at the
+// end of each receive, before it stores its next partial row; every later write to a reset
+// location, and every read that must see the reset, happens only after some workgroup has read a
+// store issued after that wait. (The first wait alone suffices: a workgroup whose slice holds an
+// entry that is written again two steps later still owns a slice at the next step, as slices
+// shrink with the message; the second is a margin.)
 //
 // Every wait is bounded by a wall-clock limit and also ends on dsync[kStop] (set by any
 // workgroup that gave up) or link.abort (set by the host), so the grid always drains. The grid is
