@@ -1,6 +1,6 @@
 // Peer executable: the reference's main.cpp command line (main.cpp:16-33, 58-104) in front of the
-// GPU DecentralizedClient, plus a few long options of this build (--form, --dump,
-// --recv-timeout).
+// GPU DecentralizedClient, plus the long options of this build (--form, --refine-iters, --dump,
+// --recv-timeout, --repeat, --profile, --ralha-loop, --device-exchange).
 #include <getopt.h>
 #include <unistd.h>
 
@@ -64,8 +64,9 @@ int main(int argc, char** argv) {
         printf(" -l --listen base server IP\n -p --port base server port\n -m --drow rows\n -n --dcol local columns\n"
                " -d --dpath data dir\n -g --logdir log dir\n -k --topk top-k (-1 all)\n -o --mode 0..3\n"
                " -t --nclient peers\n -c --cid my id\n -e --evaluate 1 dumps results\n -s --seed\n -a --opt 1|3\n"
-               " --form fp64|mixed  --dump 0|1  --recv-timeout seconds  --ralha-loop 0|1\n"
-               " --device-exchange 0|1\n");
+               " -f --ismemmap 1 maps the input file\n"
+               " --form fp64|mixed  --refine-iters N  --dump 0|1  --recv-timeout seconds\n"
+               " --repeat N  --profile 0|1  --ralha-loop 0|1  --device-exchange 0|1\n");
         return 0;
       default: return 2;
     }

@@ -59,8 +59,8 @@ int exc_bb_bidiagonal_svd(int k, const double* alpha, const double* beta, double
  * in place; reseeds the C library's rand() with seed like the reference. */
 int exc_bb_givens_mask(size_t m, size_t n, size_t mask_times, double* x, int mask, int is_left, unsigned int seed);
 
-/* block_shift_mask, bbtsvd.hpp:47 / bbtsvd.cpp:533-669 for K <= 10000 lines (every
- * configuration the client runs); EXC_ERR_ARG for the virtual-matrix branch K > 10000. */
+/* block_shift_mask, bbtsvd.hpp:47 / bbtsvd.cpp:533-669: K <= 10000 rotated lines take the plain
+ * sweep (bbtsvd.cpp:533-546), K > 10000 the virtual-matrix mask (bbtsvd.cpp:548-668). */
 int exc_bb_block_shift_mask(size_t m, size_t n, size_t mask_times, double* x, int mask, int is_left,
                             unsigned int seed);
 
