@@ -101,8 +101,8 @@ struct ClientDevice {
   HBuf h_masked, h_repart, h_msg, h_U;
   // the mixed-precision form: FP32 copies of the data, the exchanged Grams, the Cholesky factors
   // of the passes, and the host staging of the Grams and of the shared small SVD
-  DBuf x32, y32, g32, li32, Gm, L1, L2, L3, Lt, Uf, idm, pk;
-  HBuf h_gram, h_tri32;
+  DBuf x32, y32, g32, li32, Gm, L1, L2, L3, Lt, Uf, idm, pk, pacc, pstage;
+  HBuf h_gram, h_tri32, h_pstage;
   // the results as the host members hold them (pinned, so the copies run at the link's rate)
   HBuf r_sigma, r_FinalU, r_FinalVT, r_alpha, r_beta;
   void* link = nullptr;  // message link of the one-launch Ralha loop (coherent pinned memory)
@@ -1414,6 +1414,57 @@ int DecentralizedClient::run_mixed(bool compute_u, bool compute_v) {
   const long n_tri = (long)m * ((long)m + 1) / 2;
   D.pk.alloc((size_t)n_tri, s, false);
   D.h_tri32.alloc(((size_t)n_tri + 1) / 2, false);
+  D.pacc.alloc((size_t)n_tri, s, false);
+  D.pstage.alloc((size_t)n_tri * 2, s, false);
+  D.h_pstage.alloc((size_t)n_tri * 2, false);
+  // Every peer's packed triangle (this peer's in D.pk, FP32 or FP64) summed in FP64 on the device
+  // in peer order 0..P-1 (exc_acc_packed), so every peer holds the same bits in D.pacc. The own
+  // triangle, already in pinned memory at h_own, goes to every peer over TCP; each other peer's
+  // frame is copied into a pinned staging slot and uploaded in four pieces, the host copy of a
+  // piece overlapping the upload of the one before. The two slots alternate over the senders; a
+  // slot is refilled only after its last upload has completed.
+  hipEvent_t slot_free[2] = {nullptr, nullptr};
+  for (auto& e : slot_free) ckh(hipEventCreateWithFlags(&e, hipEventDisableTiming), "event");
+  struct EventGuard {
+    hipEvent_t* e;
+    ~EventGuard() {
+      for (int i = 0; i < 2; i++)
+        if (e[i]) hipEventDestroy(e[i]);
+    }
+  } slot_guard{slot_free};
+  auto sum_over_peers = [&](bool f32, const void* h_own) {
+    const size_t bytes = (size_t)n_tri * (f32 ? 4 : 8);
+    const double tb = now_seconds();
+    net.broadcast(h_own, bytes);
+    int used = 0;
+    for (int j = 0; j < (int)num_client; j++) {
+      const void* part = D.pk.p;
+      if (j != cid) {
+        std::vector<char> frame = net.receive(j);
+        if (frame.size() != bytes)
+          throw std::runtime_error("Gram exchange: frame of " + std::to_string(frame.size()) + " bytes from peer " +
+                                   std::to_string(j) + ", expected " + std::to_string(bytes));
+        const int slot = used & 1;
+        if (used >= 2) ckh(hipEventSynchronize(slot_free[slot]), "staging slot");
+        char* hs = (char*)(D.h_pstage.p + (size_t)slot * n_tri);
+        char* ds = (char*)(D.pstage.p + (size_t)slot * n_tri);
+        const size_t piece = (bytes + 3) / 4;
+        for (size_t o = 0; o < bytes; o += piece) {
+          const size_t len = std::min(piece, bytes - o);
+          memcpy(hs + o, frame.data() + o, len);
+          ckh(hipMemcpyAsync(ds + o, hs + o, len, hipMemcpyHostToDevice, s), "h2d");
+        }
+        ckh(hipEventRecord(slot_free[slot], s), "event");
+        net.recycle(j, std::move(frame));
+        part = ds;
+        used++;
+      }
+      ck(exc_acc_packed(n_tri, part, f32 ? 1 : 0, j == 0 ? 1 : 0, D.pacc.p, s), "Gram sum");
+    }
+    const double dt = now_seconds() - tb;
+    net.add_comm_seconds(dt);
+    return dt;
+  };
   // masked FP32 Gram in g32 -> its triangle summed over the peers in FP64 -> unmasked FP64 Gram
   // in D.Gm
   auto exchange_f32 = [&]() {
@@ -1424,10 +1475,9 @@ int DecentralizedClient::run_mixed(bool compute_u, bool compute_v) {
     ckh(hipMemcpyAsync(D.h_tri32.p, D.pk.p, (size_t)n_tri * 4, hipMemcpyDeviceToHost, s), "d2h");
     sync();
     double tb = now_seconds();
-    net.all_reduce_sum_same_f32((const float*)D.h_tri32.p, D.h_gram.p, n_tri);
-    double tc = now_seconds();
-    ckh(hipMemcpyAsync(D.pk.p, D.h_gram.p, (size_t)n_tri * 8, hipMemcpyHostToDevice, s), "h2d");
-    ck(exc_unpack_lower_f64(mi, D.pk.p, D.Gm.p, (long)m, s), "unpack");
+    const double tx = sum_over_peers(true, D.h_tri32.p);
+    double tc = tb + tx;
+    ck(exc_unpack_lower_f64(mi, D.pacc.p, D.Gm.p, (long)m, s), "unpack");
     givens_mask(D.Gm.p, (long)m, (long)m, 0, 1, public_seed, plans, s);
     givens_mask(D.Gm.p, (long)m, (long)m, 0, 0, public_seed, plans, s);
     sync();
@@ -1504,10 +1554,9 @@ int DecentralizedClient::run_mixed(bool compute_u, bool compute_v) {
     ckh(hipMemcpyAsync(D.h_gram.p, D.pk.p, (size_t)n_tri * 8, hipMemcpyDeviceToHost, s), "d2h");
     sync();
     double tb = now_seconds();
-    net.all_reduce_sum_same(D.h_gram.p, n_tri);
-    double tc = now_seconds();
-    ckh(hipMemcpyAsync(D.pk.p, D.h_gram.p, (size_t)n_tri * 8, hipMemcpyHostToDevice, s), "h2d");
-    ck(exc_unpack_lower_f64(mi, D.pk.p, D.Gm.p, (long)m, s), "unpack");
+    const double tx = sum_over_peers(false, D.h_gram.p);
+    double tc = tb + tx;
+    ck(exc_unpack_lower_f64(mi, D.pacc.p, D.Gm.p, (long)m, s), "unpack");
     givens_mask(D.Gm.p, (long)m, (long)m, 0, 1, public_seed, plans, s);
     givens_mask(D.Gm.p, (long)m, (long)m, 0, 0, public_seed, plans, s);
     sync();

@@ -142,6 +142,17 @@ __global__ __launch_bounds__(256) void k_unpack_lower(int n, const double* __res
   a[j * lda + i] = v;
 }
 
+// acc[q] = (first ? 0 : acc[q]) + (double)part[q]: one peer's term of a sum taken in peer order,
+// the same FP64 additions in the same order as a host loop `acc = 0; acc = acc + x_j`.
+template <typename T>
+__global__ __launch_bounds__(256) void k_acc_packed(long n, const T* __restrict__ part, int first,
+                                                    double* __restrict__ acc) {
+  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const double base = first ? 0.0 : acc[q];
+  acc[q] = base + (double)part[q];
+}
+
 // sigma_i = T_ii / (1 - (R_ii + S_ii) / 2) with R_ii = 1 - (U^T U)_ii, S_ii = 1 - (W^T W)_ii;
 // smax[0] = max |sigma_i|. One workgroup of 1024 threads.
 __global__ __launch_bounds__(1024) void k_oa_sigma(int m, const double* __restrict__ T, const double* __restrict__ utu,
@@ -304,6 +315,20 @@ int exc_unpack_lower_f64(int n, const double* d_packed, double* d_a, long lda, e
   const long t = (long)n * (n + 1) / 2;
   hipLaunchKernelGGL(k_unpack_lower, dim3(exc_detail::ceil_div(t, 256)), dim3(256), 0, exc_detail::stream(stream),
                      n, d_packed, d_a, lda);
+  EXC_CHECK_HIP(hipGetLastError());
+  return EXC_OK;
+}
+
+int exc_acc_packed(long n, const void* d_part, int part_is_f32, int first, double* d_acc, exc_stream_t stream) {
+  EXC_REQUIRE(n >= 0 && (n == 0 || (d_part && d_acc)), EXC_ERR_ARG, "exc_acc_packed: bad arguments");
+  if (n == 0) return EXC_OK;
+  hipStream_t s = exc_detail::stream(stream);
+  if (part_is_f32)
+    hipLaunchKernelGGL(k_acc_packed<float>, dim3(exc_detail::ceil_div(n, 256)), dim3(256), 0, s, n,
+                       (const float*)d_part, first, d_acc);
+  else
+    hipLaunchKernelGGL(k_acc_packed<double>, dim3(exc_detail::ceil_div(n, 256)), dim3(256), 0, s, n,
+                       (const double*)d_part, first, d_acc);
   EXC_CHECK_HIP(hipGetLastError());
   return EXC_OK;
 }

@@ -33,6 +33,8 @@ class PeerNet {
   std::vector<char> receive(int peer);
   // receive() into a caller buffer of exactly `bytes` bytes.
   void receive_into(int peer, void* buf, size_t bytes);
+  // Returns a consumed frame's buffer (from receive) to the pool of the peer it came from.
+  void recycle(int peer, std::vector<char>&& m);
   void broadcast(const void* buf, size_t bytes);
 
   // client.cpp:313-356: below P*10000 values a broadcast and a sum of the others' buffers in
@@ -69,8 +71,6 @@ class PeerNet {
     // faulting in fresh pages (large frames otherwise cost a memset and a page fault per 4 KB)
     std::vector<std::vector<char>> pool;
   };
-  // Returns a consumed frame's buffer to the pool of the peer it came from.
-  void recycle(int peer, std::vector<char>&& m);
   void accept_loop();
   void reader_loop(int fd, int from);
 

@@ -111,6 +111,7 @@ SIGNATURES = {
     "exc_pack_lower_f32": (_I, [_I, _vp, _L, _vp, _vp]),
     "exc_pack_lower_f64": (_I, [_I, _dp, _L, _dp, _vp]),
     "exc_unpack_lower_f64": (_I, [_I, _dp, _dp, _L, _vp]),
+    "exc_acc_packed": (_I, [_L, _vp, _I, _I, _dp, _vp]),
     "exc_convert_f64_f32": (_I, [_L, _L, _dp, _L, _dp, _L, _vp]),
     "exc_convert_f32_f64": (_I, [_L, _L, _dp, _L, _dp, _L, _vp]),
     "exc_trace": (_I, [_I, _dp, _L, _dp, _vp]),

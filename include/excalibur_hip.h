@@ -335,6 +335,12 @@ int exc_convert_f32_f64(long rows, long cols, const float* d_src, long lds, doub
 int exc_pack_lower_f32(int n, const float* d_a, long lda, float* d_packed, exc_stream_t stream);
 int exc_pack_lower_f64(int n, const double* d_a, long lda, double* d_packed, exc_stream_t stream);
 int exc_unpack_lower_f64(int n, const double* d_packed, double* d_a, long lda, exc_stream_t stream);
+/* One peer's term of the mixed form's Gram sum, added on the device in peer order:
+ * d_acc[q] = (first ? 0 : d_acc[q]) + d_part[q] for q < n, d_part FP32 (part_is_f32) or FP64,
+ * the sum in FP64. Called for peers 0 .. P-1 in turn it gives the bits of the host loop
+ * acc = 0; acc = acc + x_j, on every peer alike. No counterpart in the reference, which sums
+ * its exchanged buffers on the host (client.cpp:313-327). */
+int exc_acc_packed(long n, const void* d_part, int part_is_f32, int first, double* d_acc, exc_stream_t stream);
 int exc_trace(int n, const double* d_a, long lda, double* d_out, exc_stream_t stream);
 int exc_add_diag(int n, double* d_a, long lda, double value, exc_stream_t stream);
 size_t exc_dev_identity_workspace(int n);

@@ -258,6 +258,27 @@ def test_pack_unpack_lower(hip, n):
     assert np.array_equal(out.to_host(), L + np.tril(A, -1).T)
 
 
+@pytest.mark.parametrize("P,n", [(1, 1), (2, 500500), (3, 7), (5, 1000)])
+def test_acc_packed_peer_order(hip, P, n):
+    """exc_acc_packed called for peers 0..P-1 in turn gives the host loop acc = 0; acc = acc + x_j
+    bit for bit, for FP32 terms widened to FP64 and for FP64 terms; a peer's own term read in place
+    and the others from staging give the same bits."""
+    rng = np.random.default_rng(P * 1000 + n)
+    for f32 in (True, False):
+        dt = np.float32 if f32 else np.float64
+        parts = [(rng.standard_normal(n) * 10.0 ** rng.integers(-8, 8, n)).astype(dt) for _ in range(P)]
+        want = np.zeros(n)
+        for x in parts:
+            want = want + x.astype(np.float64)
+        acc = hip.DeviceArray((n,)).zero()
+        dparts = [hip.DeviceArray.from_host(x) for x in parts]
+        for j, d in enumerate(dparts):
+            hip.call("exc_acc_packed", n, d.ptr, int(f32), int(j == 0), acc.ptr, None)
+        assert np.array_equal(acc.to_host(), want)
+    with pytest.raises(hip.HipError):
+        hip.call("exc_acc_packed", -1, None, 0, 1, None, None)
+
+
 @pytest.mark.parametrize("n", [1, 5, 300, 1500])
 def test_trace_and_deviation(hip, n):
     """exc_trace (diagonal sum) and exc_dev_identity (max |A - I| and ||A - I||_F) against numpy."""
