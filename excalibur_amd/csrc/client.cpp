@@ -62,6 +62,11 @@ struct DBuf {
     if (zero) ckh(hipMemsetAsync(p, 0, std::max<size_t>(count, 1) * sizeof(double), s), "memset");
   }
   void release() { n = 0; }  // keeps the memory for the next run
+  void swap(DBuf& o) {
+    std::swap(p, o.p);
+    std::swap(n, o.n);
+    std::swap(cap, o.cap);
+  }
   void free_all() {
     if (p) exc_free(p);
     p = nullptr;
@@ -101,8 +106,9 @@ struct ClientDevice {
   HBuf h_masked, h_repart, h_msg, h_U;
   // the mixed-precision form: FP32 copies of the data, the exchanged Grams, the Cholesky factors
   // of the passes, and the host staging of the Grams and of the shared small SVD
-  DBuf x32, y32, g32, li32, Gm, L1, L2, L3, Lt, Uf, idm, pk, pacc, pstage;
-  HBuf h_gram, h_tri32, h_pstage;
+  DBuf x32, y32, g32, li32, Gm, L1, L2, L3, Lt, Uf, idm, pk, pacc, pstage, L3i, work2, probe;
+  HBuf h_gram, h_tri32, h_pstage, h_probe;
+  hipStream_t s2 = nullptr;  // side stream of the mixed form (work that overlaps the main chain)
   // the results as the host members hold them (pinned, so the copies run at the link's rate)
   HBuf r_sigma, r_FinalU, r_FinalVT, r_alpha, r_beta;
   void* link = nullptr;  // message link of the one-launch Ralha loop (coherent pinned memory)
@@ -131,6 +137,7 @@ struct ClientDevice {
       if (boxes[q] && boxes[q] != box) exc_ipc_close(boxes[q]);
     if (box) exc_free(box);
     if (link) exc_ralha_link_free(link);
+    if (s2) hipStreamDestroy(s2);
     if (s) hipStreamDestroy(s);
   }
 };
@@ -1578,6 +1585,7 @@ int DecentralizedClient::run_mixed(bool compute_u, bool compute_v) {
   // number near the reach of single precision), Z is first replaced by L3^-1 Z and the pass
   // repeated. The Gram matrix is bit-identical on every peer, so every peer takes the same branch.
   int fp64_passes = 0;
+  double last_dev = 0.0;
   for (;;) {
     gram_f64();
     fp64_passes++;
@@ -1589,8 +1597,38 @@ int DecentralizedClient::run_mixed(bool compute_u, bool compute_v) {
         ", ||Z Z^T - I||_F " + std::to_string(dev[1]));
     shifted_chol(D.L3, 0.0, "pass 3 Cholesky");
     lt_times_l3();
+    last_dev = dev[1];
     if (dev[1] <= 0.5 || fp64_passes == 3) break;
     ck(exc_trsm_left(0, 0, mi, r, D.L3.p, (long)m, D.X.p, r, work, work_bytes, s), "extra pass solve");
+  }
+  // L3^-1 on the side stream while the bidiagonalization runs, so that W' = L3^-T W is one product
+  // at the end instead of a triangular solve on the critical path. With ||G3 - I||_F <= 1/2 the
+  // eigenvalues of G3 = L3 L3^T lie in [1/2, 3/2], so kappa(L3) <= sqrt(3) and the explicit inverse
+  // is as accurate as the solve; otherwise (the third FP64 pass still short of that) W' keeps the
+  // solve.
+  const bool l3_inverse = last_dev <= 0.5;
+  hipEvent_t ev_l3 = nullptr, ev_inv = nullptr;
+  struct EvPair {
+    hipEvent_t* a;
+    hipEvent_t* b;
+    ~EvPair() {
+      if (*a) hipEventDestroy(*a);
+      if (*b) hipEventDestroy(*b);
+    }
+  } ev_guard{&ev_l3, &ev_inv};
+  if (l3_inverse) {
+    if (!D.s2) ckh(hipStreamCreateWithFlags(&D.s2, hipStreamNonBlocking), "stream");
+    ckh(hipEventCreateWithFlags(&ev_l3, hipEventDisableTiming), "event");
+    ckh(hipEventCreateWithFlags(&ev_inv, hipEventDisableTiming), "event");
+    D.L3i.alloc(mm, s, false);
+    D.work2.alloc((exc_fact_workspace(mi, (long)m) + 7) / 8, s, false);
+    ckh(hipEventRecord(ev_l3, s), "event");
+    ckh(hipStreamWaitEvent(D.s2, ev_l3, 0), "event");
+    ck(exc_fill2d((long)m, (long)m, 0.0, D.L3i.p, (long)m, D.s2), "identity");
+    ck(exc_add_diag(mi, D.L3i.p, (long)m, 1.0, D.s2), "identity");
+    ck(exc_trsm_left(0, 0, mi, (long)m, D.L3.p, (long)m, D.L3i.p, (long)m, D.work2.p, D.work2.n * 8, D.s2),
+       "L3 inverse");
+    ckh(hipEventRecord(ev_inv, D.s2), "event");
   }
   sync();
   tf += now_seconds() - t0;
@@ -1651,7 +1689,9 @@ int DecentralizedClient::run_mixed(bool compute_u, bool compute_v) {
        "bidiagonal SVD");
     ck(exc_sign_normalize(mi, k, D.bu.p, D.bv.p, D.flip.p, s), "sign rule");
     // U = (bv U_r)^T: the left vectors of Lt; W = H (rotations back) bu: the right vectors
-    ck(exc_dgemm(0, 1, (long)m, k, (long)m, 1.0, D.U.p, (long)m, D.bv.p, (long)m, 0.0, D.Uf.p, k, nullptr, 0, s),
+    D.work.alloc(std::max(D.work.n, (exc_gemm_workspace((long)m, k, (long)m) + 7) / 8), s, false);
+    ck(exc_dgemm(0, 1, (long)m, k, (long)m, 1.0, D.U.p, (long)m, D.bv.p, (long)m, 0.0, D.Uf.p, k, D.work.p,
+                 D.work.n * 8, s),
        "U");
     ck(exc_rot_back(mi, k, D.bu.p, D.cs.p, s), "back rotation");
     {
@@ -1669,29 +1709,35 @@ int DecentralizedClient::run_mixed(bool compute_u, bool compute_v) {
                        1e-4, D.work.p, D.work.n * 8, s),
          "refinement");
     }
-    // W' = L3^-T W through the upper triangle L3^T
-    ck(exc_transpose((long)m, (long)m, D.L3.p, (long)m, D.idm.p, (long)m, s), "transpose");
-    ck(exc_trsm_left(1, 0, mi, k, D.idm.p, (long)m, D.bu.p, k, D.work.p, D.work.n * 8, s), "W'");
+    if (l3_inverse) {  // W' = (L3^-1)^T W, into D.idm, which then trades places with D.bu
+      ckh(hipStreamWaitEvent(s, ev_inv, 0), "event");
+      D.work.alloc(std::max(D.work.n, (exc_gemm_workspace((long)m, k, (long)m) + 7) / 8), s, false);
+      ck(exc_dgemm(1, 0, (long)m, k, (long)m, 1.0, D.L3i.p, (long)m, D.bu.p, k, 0.0, D.idm.p, k, D.work.p,
+                   D.work.n * 8, s),
+         "W'");
+      D.bu.swap(D.idm);
+    } else {  // W' = L3^-T W through the upper triangle L3^T
+      ck(exc_transpose((long)m, (long)m, D.L3.p, (long)m, D.idm.p, (long)m, s), "transpose");
+      ck(exc_trsm_left(1, 0, mi, k, D.idm.p, (long)m, D.bu.p, k, D.work.p, D.work.n * 8, s), "W'");
+    }
     sync();
     time_refine = now_seconds() - t0;
   }
   if (num_client > 1) {
     // FNV-1a over k, sigma and the first and last rows of U and W'
-    std::vector<double> probe((size_t)1 + 5 * k);
-    probe[0] = k;
-    ckh(hipMemcpyAsync(probe.data() + 1, D.sigma.p, (size_t)k * 8, hipMemcpyDeviceToHost, s), "d2h");
-    ckh(hipMemcpyAsync(probe.data() + 1 + k, D.Uf.p, (size_t)k * 8, hipMemcpyDeviceToHost, s), "d2h");
-    ckh(hipMemcpyAsync(probe.data() + 1 + 2 * k, D.Uf.p + (m - 1) * (size_t)k, (size_t)k * 8,
-                       hipMemcpyDeviceToHost, s),
-        "d2h");
-    ckh(hipMemcpyAsync(probe.data() + 1 + 3 * k, D.bu.p, (size_t)k * 8, hipMemcpyDeviceToHost, s), "d2h");
-    ckh(hipMemcpyAsync(probe.data() + 1 + 4 * k, D.bu.p + (m - 1) * (size_t)k, (size_t)k * 8,
-                       hipMemcpyDeviceToHost, s),
-        "d2h");
+    // the five rows gathered on the device, then one copy into pinned memory
+    const size_t np = (size_t)1 + 5 * k;
+    D.probe.alloc(np, s, false);
+    D.h_probe.alloc(np, false);
+    const double* rows[5] = {D.sigma.p, D.Uf.p, D.Uf.p + (m - 1) * (size_t)k, D.bu.p, D.bu.p + (m - 1) * (size_t)k};
+    for (int q = 0; q < 5; q++)
+      ckh(hipMemcpyAsync(D.probe.p + 1 + (size_t)q * k, rows[q], (size_t)k * 8, hipMemcpyDeviceToDevice, s), "copy");
+    ckh(hipMemcpyAsync(D.h_probe.p + 1, D.probe.p + 1, (size_t)5 * k * 8, hipMemcpyDeviceToHost, s), "d2h");
     sync();
+    D.h_probe.p[0] = k;
     uint64_t h = 1469598103934665603ull;
-    const unsigned char* bytes = (const unsigned char*)probe.data();
-    for (size_t q = 0; q < probe.size() * sizeof(double); q++) h = (h ^ bytes[q]) * 1099511628211ull;
+    const unsigned char* bytes = (const unsigned char*)D.h_probe.p;
+    for (size_t q = 0; q < np * sizeof(double); q++) h = (h ^ bytes[q]) * 1099511628211ull;
     const double tb = now_seconds();
     net.broadcast(&h, sizeof h);
     for (int j = 0; j < (int)num_client; j++) {

@@ -18,23 +18,26 @@ namespace {
 
 constexpr int kThreads = 256;
 
-template <typename T>
-__device__ __forceinline__ void rot(T* p1, T* p2, T c, T ss, T nss) {
-  const T a = *p1, b = *p2;
-  *p1 = a * c + b * ss;
-  *p2 = a * nss + b * c;
+// Workgroup barrier that orders LDS accesses only: unlike __syncthreads it does not wait for the
+// thread's outstanding global loads, so a plan prefetch stays in flight across it.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
-// Plans of at most this many pairs per pass are staged in LDS (two pass buffers): every
-// workgroup reads each pass's pairs and rotations from HBM or L2 once, one pass ahead of their
+// Plans of at most this many pairs per pass are staged in LDS (three pass buffers): every
+// workgroup reads each pass's pairs and rotations from HBM or L2 once, two passes ahead of their
 // use, instead of once per line it owns and in the shadow of nothing.
 constexpr int kStageMaxHalf = 1024;
 constexpr int kStagePerThread = kStageMaxHalf / 256;
-size_t stage_bytes(int half) { return half <= kStageMaxHalf ? (size_t)half * 48 : 0; }
+constexpr int kStageBufs = 3;
+size_t stage_bytes(int half) { return half <= kStageMaxHalf ? (size_t)half * 24 * kStageBufs : 0; }
 
 // odd_mode: 0 none, 1 the special pair is pair 0 (applied first), 2 it is the last pair.
-// STAGE: the plan of pass p + 1 is loaded into registers before the rotations of pass p and
-// stored into the other LDS plan buffer after them (stage: 2 x half x {2 int32, 2 double}).
+// STAGE: the plan of pass p + 2 is loaded into registers before the rotations of pass p and
+// stored into its LDS plan buffer after the rotations of pass p + 1, so a load has two passes to
+// arrive (stage: 3 x half x {2 int32, 2 double}, two register sets).
 template <typename T, bool LEFT, bool STAGE>
 __device__ __forceinline__ void givens_strip(T* __restrict__ x, long m, long n, int W,
                                              const int32_t* __restrict__ pairs, const double* __restrict__ cs,
@@ -59,32 +62,149 @@ __device__ __forceinline__ void givens_strip(T* __restrict__ x, long m, long n, 
   }
   __syncthreads();
   const T sgn = mask ? T(1) : T(-1);
+  // The pairs of one pass are disjoint, so the rotations of a range touch disjoint elements: a
+  // thread takes kBatch of its items at once, all their loads before any of their stores, and the
+  // LDS latencies overlap instead of chaining item after item. Every element still receives the
+  // same operations in the same order.
+  // (LDS indices fit in 32 bits: the strip is at most 160 KB.)
+  constexpr int kBatch = 8;
+  constexpr int kBatchGeneral = 4;
   auto apply_range = [&](const int32_t* pp, const double* cp, int j0, int j1) {
     const int nj = j1 - j0;
-    const long items = (long)nj * Wc;
-    for (long it = tid; it < items; it += kThreads) {
-      int j, cc;
-      if (LEFT) { j = j0 + (int)(it / Wc); cc = (int)(it % Wc); }
-      else { j = j0 + (int)(it % nj); cc = (int)(it / nj); }
-      const int r1 = pp[2 * j], r2 = pp[2 * j + 1];
-      if ((unsigned)r1 >= (unsigned long)K || (unsigned)r2 >= (unsigned long)K) continue;  // malformed plan
-      const T c = (T)cp[2 * j];
-      const T s = (T)cp[2 * j + 1];
-      const T ss = sgn * s;
-      const T nss = -sgn * s;
-      if (LEFT) rot(&buf[(long)r1 * W + cc], &buf[(long)r2 * W + cc], c, ss, nss);
-      else rot(&buf[(long)cc * n + r1], &buf[(long)cc * n + r2], c, ss, nss);
+    const int items = nj * Wc;
+    for (int it0 = tid; it0 < items; it0 += kThreads * kBatchGeneral) {
+      int i1[kBatchGeneral], i2[kBatchGeneral];
+      T c[kBatchGeneral], s[kBatchGeneral], a[kBatchGeneral], b[kBatchGeneral];
+      bool ok[kBatchGeneral];
+#pragma unroll
+      for (int u = 0; u < kBatchGeneral; u++) {
+        const int it = it0 + u * kThreads;
+        ok[u] = false;
+        if (it < items) {
+          int j, cc;
+          if (LEFT) { j = j0 + it / Wc; cc = it - (j - j0) * Wc; }
+          else { cc = it / nj; j = j0 + (it - cc * nj); }
+          const int r1 = pp[2 * j], r2 = pp[2 * j + 1];
+          // a malformed plan's out-of-range pair is skipped
+          ok[u] = (unsigned)r1 < (unsigned long)K && (unsigned)r2 < (unsigned long)K;
+          i1[u] = LEFT ? r1 * W + cc : cc * (int)n + r1;
+          i2[u] = LEFT ? r2 * W + cc : cc * (int)n + r2;
+          c[u] = (T)cp[2 * j];
+          s[u] = (T)cp[2 * j + 1];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kBatchGeneral; u++)
+        if (ok[u]) {
+          a[u] = buf[i1[u]];
+          b[u] = buf[i2[u]];
+        }
+#pragma unroll
+      for (int u = 0; u < kBatchGeneral; u++)
+        if (ok[u]) {
+          const T ss = sgn * s[u];
+          const T nss = -sgn * s[u];
+          buf[i1[u]] = a[u] * c[u] + b[u] * ss;
+          buf[i2[u]] = a[u] * nss + b[u] * c[u];
+        }
     }
   };
+  // Fast path, when a pass holds at most kBatch items per thread (the common shapes): thread tid
+  // owns items tid + u * kThreads of every pass, and the item -> (pair j, line cc) split is the
+  // same in every pass, so it is computed once here; per pass an item then costs its plan reads,
+  // two 32-bit index products, two loads, the rotation and two stores. For odd K the special pair
+  // goes in a sub-step of its own, before (unmask) or after (mask) the others, as in apply_range.
+  const int all_items = half * Wc;
+  const bool fast = all_items <= kThreads * kBatch;
+  int fj[kBatch], fbase[kBatch];
+  const int fmul = LEFT ? W : 1;
+#pragma unroll
+  for (int u = 0; u < kBatch; u++) {
+    const int it = tid + u * kThreads;
+    fj[u] = -1;
+    fbase[u] = 0;
+    if (fast && it < all_items) {
+      int cc;
+      if (LEFT) { fj[u] = it / Wc; cc = it - fj[u] * Wc; }
+      else { cc = it / half; fj[u] = it - cc * half; }
+      fbase[u] = LEFT ? cc : cc * (int)n;
+    }
+  }
+  const int special = odd_mode == 1 ? 0 : (odd_mode == 2 ? half - 1 : -1);
+  // sub = 0: every pair (no special pair); 1: the special pair only; 2: all but the special pair
+  auto apply_fast = [&](const int32_t* pp, const double* cp, int sub) {
+    int i1[kBatch], i2[kBatch];
+    T c[kBatch], sv[kBatch], a[kBatch], b[kBatch];
+    bool ok[kBatch];
+#pragma unroll
+    for (int u = 0; u < kBatch; u++) {
+      const int j = fj[u];
+      ok[u] = j >= 0 && (sub == 0 || ((j == special) == (sub == 1)));
+      if (ok[u]) {
+        const int r1 = pp[2 * j], r2 = pp[2 * j + 1];
+        ok[u] = (unsigned)r1 < (unsigned long)K && (unsigned)r2 < (unsigned long)K;  // malformed plan
+        i1[u] = r1 * fmul + fbase[u];
+        i2[u] = r2 * fmul + fbase[u];
+        c[u] = (T)cp[2 * j];
+        sv[u] = (T)cp[2 * j + 1];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kBatch; u++)
+      if (ok[u]) {
+        a[u] = buf[i1[u]];
+        b[u] = buf[i2[u]];
+      }
+#pragma unroll
+    for (int u = 0; u < kBatch; u++)
+      if (ok[u]) {
+        const T ss = sgn * sv[u];
+        const T nss = -sgn * sv[u];
+        buf[i1[u]] = a[u] * c[u] + b[u] * ss;
+        buf[i2[u]] = a[u] * nss + b[u] * c[u];
+      }
+  };
   // staged plan buffers: b holds half int2 pairs then half double2 rotations
-  int32_t* spp[2] = {nullptr, nullptr};
-  double* scp[2] = {nullptr, nullptr};
-  // registers of the next pass: pair (r1, r2) and rotation (c, s) of up to kStagePerThread pairs;
-  // scalar loads, so no alignment beyond the elements' own is assumed of the plan
-  int32_t ra[kStagePerThread], rb[kStagePerThread];
-  double rc[kStagePerThread], rs[kStagePerThread];
+  int32_t* spp[kStageBufs] = {nullptr, nullptr, nullptr};
+  double* scp[kStageBufs] = {nullptr, nullptr, nullptr};
+  // a pass's pairs (r1, r2) and rotations (c, s) of up to kStagePerThread pairs per thread, in
+  // registers between their load and their store into LDS; scalar loads, so no alignment beyond
+  // the elements' own is assumed of the plan
+  struct PlanRegs {
+    int32_t a[kStagePerThread], b[kStagePerThread];
+    double c[kStagePerThread], s[kStagePerThread];
+  };
+  PlanRegs R0, R1;
+  auto issue = [&](PlanRegs& R, int q) {  // loads of pass q's plan
+    const int32_t* gp = pairs + (long)q * half * 2;
+    const double* gc = cs + (long)q * half * 2;
+#pragma unroll
+    for (int u = 0; u < kStagePerThread; u++) {
+      const int idx = tid + u * kThreads;
+      if (idx < half) {
+        R.a[u] = gp[2 * idx];
+        R.b[u] = gp[2 * idx + 1];
+        R.c[u] = gc[2 * idx];
+        R.s[u] = gc[2 * idx + 1];
+      }
+    }
+  };
+  auto commit = [&](const PlanRegs& R, int q) {  // pass q's plan into its LDS buffer
+    int32_t* np = spp[q % kStageBufs];
+    double* nc = scp[q % kStageBufs];
+#pragma unroll
+    for (int u = 0; u < kStagePerThread; u++) {
+      const int idx = tid + u * kThreads;
+      if (idx < half) {
+        np[2 * idx] = R.a[u];
+        np[2 * idx + 1] = R.b[u];
+        nc[2 * idx] = R.c[u];
+        nc[2 * idx + 1] = R.s[u];
+      }
+    }
+  };
   if (STAGE) {
-    for (int b = 0; b < 2; b++) {
+    for (int b = 0; b < kStageBufs; b++) {
       unsigned char* base_b = stage + (size_t)b * half * 24;
       spp[b] = reinterpret_cast<int32_t*>(base_b);
       scp[b] = reinterpret_cast<double*>(base_b + (size_t)half * 8);
@@ -93,53 +213,46 @@ __device__ __forceinline__ void givens_strip(T* __restrict__ x, long m, long n, 
       spp[0][idx] = pairs[idx];
       scp[0][idx] = cs[idx];
     }
-    __syncthreads();
+    if (passes > 1) issue(R1, 1);
+    lds_barrier();
   }
-  for (int p = 0; p < passes; p++) {
-    const int32_t* pp = STAGE ? spp[p & 1] : pairs + (long)p * half * 2;
-    const double* cp = STAGE ? scp[p & 1] : cs + (long)p * half * 2;
-    if (STAGE && p + 1 < passes) {  // next pass's plan in flight while this pass rotates
-      const int32_t* gp = pairs + (long)(p + 1) * half * 2;
-      const double* gc = cs + (long)(p + 1) * half * 2;
-#pragma unroll
-      for (int q = 0; q < kStagePerThread; q++) {
-        const int idx = tid + q * kThreads;
-        if (idx < half) {
-          ra[q] = gp[2 * idx];
-          rb[q] = gp[2 * idx + 1];
-          rc[q] = gc[2 * idx];
-          rs[q] = gc[2 * idx + 1];
-        }
+  // Pass p reads buffer p % 3. At its start the loads of pass p + 2 go out into the register set
+  // of p's parity (free: it held pass p's plan, committed at the end of pass p - 1); at its end
+  // pass p + 1's plan is committed into buffer (p + 1) % 3, last read by pass p - 2, which every
+  // thread finished at the barrier that ended it.
+  auto one_pass = [&](int p, PlanRegs& Rp, PlanRegs& Rn) {
+    const int32_t* pp = STAGE ? spp[p % kStageBufs] : pairs + (long)p * half * 2;
+    const double* cp = STAGE ? scp[p % kStageBufs] : cs + (long)p * half * 2;
+    if (STAGE && p + 2 < passes) issue(Rp, p + 2);
+    if (fast) {
+      if (odd_mode == 1) {
+        apply_fast(pp, cp, 1);
+        lds_barrier();
+        apply_fast(pp, cp, 2);
+      } else if (odd_mode == 2) {
+        apply_fast(pp, cp, 2);
+        lds_barrier();
+        apply_fast(pp, cp, 1);
+      } else {
+        apply_fast(pp, cp, 0);
       }
-    }
-    if (odd_mode == 1) {
+    } else if (odd_mode == 1) {
       apply_range(pp, cp, 0, 1);
-      __syncthreads();
+      lds_barrier();
       apply_range(pp, cp, 1, half);
     } else if (odd_mode == 2) {
       apply_range(pp, cp, 0, half - 1);
-      __syncthreads();
+      lds_barrier();
       apply_range(pp, cp, half - 1, half);
     } else {
       apply_range(pp, cp, 0, half);
     }
-    if (STAGE && p + 1 < passes) {
-      // the other buffer was last read in pass p - 1, which every thread finished at the
-      // barrier that ended it
-      int32_t* np = spp[(p + 1) & 1];
-      double* nc = scp[(p + 1) & 1];
-#pragma unroll
-      for (int q = 0; q < kStagePerThread; q++) {
-        const int idx = tid + q * kThreads;
-        if (idx < half) {
-          np[2 * idx] = ra[q];
-          np[2 * idx + 1] = rb[q];
-          nc[2 * idx] = rc[q];
-          nc[2 * idx + 1] = rs[q];
-        }
-      }
-    }
-    __syncthreads();
+    if (STAGE && p + 1 < passes) commit(Rn, p + 1);
+    lds_barrier();
+  };
+  for (int p = 0; p < passes; p += 2) {
+    one_pass(p, R0, R1);
+    if (p + 1 < passes) one_pass(p + 1, R1, R0);
   }
   if (LEFT) {
     const long tot = K * Wc;
