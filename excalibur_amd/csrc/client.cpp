@@ -106,7 +106,7 @@ struct ClientDevice {
   HBuf h_masked, h_repart, h_msg, h_U;
   // the mixed-precision form: FP32 copies of the data, the exchanged Grams, the Cholesky factors
   // of the passes, and the host staging of the Grams and of the shared small SVD
-  DBuf x32, y32, g32, li32, Gm, L1, L2, L3, Lt, Uf, idm, pk, pacc, pstage, L3i, work2, probe;
+  DBuf x32, y32, g32, li32, Gm, L1, L2, L3, Lt, Uf, idm, pk, pacc, pstage, L3i, work2, probe, Pm;
   HBuf h_gram, h_tri32, h_pstage, h_probe;
   hipStream_t s2 = nullptr;  // side stream of the mixed form (work that overlaps the main chain)
   // the results as the host members hold them (pinned, so the copies run at the link's rate)
@@ -1607,21 +1607,29 @@ int DecentralizedClient::run_mixed(bool compute_u, bool compute_v) {
   // is as accurate as the solve; otherwise (the third FP64 pass still short of that) W' keeps the
   // solve.
   const bool l3_inverse = last_dev <= 0.5;
-  hipEvent_t ev_l3 = nullptr, ev_inv = nullptr;
-  struct EvPair {
-    hipEvent_t* a;
-    hipEvent_t* b;
-    ~EvPair() {
-      if (*a) hipEventDestroy(*a);
-      if (*b) hipEventDestroy(*b);
+  // events of the side stream's work (destroyed on every exit)
+  struct Events {
+    std::vector<hipEvent_t> e;
+    hipEvent_t make() {
+      hipEvent_t v = nullptr;
+      ckh(hipEventCreateWithFlags(&v, hipEventDisableTiming), "event");
+      e.push_back(v);
+      return v;
     }
-  } ev_guard{&ev_l3, &ev_inv};
+    ~Events() {
+      for (hipEvent_t v : e) hipEventDestroy(v);
+    }
+  } events;
+  if (!D.s2) ckh(hipStreamCreateWithFlags(&D.s2, hipStreamNonBlocking), "stream");
+  // the side stream's workspace, sized for both of its jobs before either runs
+  D.work2.alloc((std::max(exc_fact_workspace(mi, (long)m), exc_ormqr_house_workspace(mi, (long)m)) + 7) / 8, s,
+                false);
+  D.Pm.alloc(mm, s, false);
+  hipEvent_t ev_inv = nullptr;
   if (l3_inverse) {
-    if (!D.s2) ckh(hipStreamCreateWithFlags(&D.s2, hipStreamNonBlocking), "stream");
-    ckh(hipEventCreateWithFlags(&ev_l3, hipEventDisableTiming), "event");
-    ckh(hipEventCreateWithFlags(&ev_inv, hipEventDisableTiming), "event");
+    hipEvent_t ev_l3 = events.make();
+    ev_inv = events.make();
     D.L3i.alloc(mm, s, false);
-    D.work2.alloc((exc_fact_workspace(mi, (long)m) + 7) / 8, s, false);
     ckh(hipEventRecord(ev_l3, s), "event");
     ckh(hipStreamWaitEvent(D.s2, ev_l3, 0), "event");
     ck(exc_fill2d((long)m, (long)m, 0.0, D.L3i.p, (long)m, D.s2), "identity");
@@ -1671,6 +1679,22 @@ int DecentralizedClient::run_mixed(bool compute_u, bool compute_v) {
     sync();
     time_bi_diag = now_seconds() - t0;
     t0 = now_seconds();
+    // P = H R^T on the side stream while the bisection and the twisted factorization run: the back
+    // rotation R^T (the rotations of the lower-to-upper step) and the Householder product H both
+    // multiply the columns of bu from the left, so W = H (R^T bu) = P bu is then one product
+    // instead of a chain of them after the twisted factorization.
+    hipEvent_t ev_p = events.make();
+    {
+      hipEvent_t ev_bd = events.make();
+      ckh(hipEventRecord(ev_bd, s), "event");
+      ckh(hipStreamWaitEvent(D.s2, ev_bd, 0), "event");
+      ck(exc_fill2d((long)m, (long)m, 0.0, D.Pm.p, (long)m, D.s2), "identity");
+      ck(exc_add_diag(mi, D.Pm.p, (long)m, 1.0, D.s2), "identity");
+      ck(exc_rot_back(mi, mi, D.Pm.p, D.cs.p, D.s2), "back rotation");
+      ck(exc_ormqr_house_ln(mi, (long)m, D.house.p, D.tau_h.p, D.Pm.p, (long)m, D.work2.p, D.work2.n * 8, D.s2),
+         "ormqr house");
+      ckh(hipEventRecord(ev_p, D.s2), "event");
+    }
     DBuf& b2 = D.b2;
     b2.alloc(2, s);
     DBuf& cnt2 = D.cnt2;
@@ -1693,12 +1717,12 @@ int DecentralizedClient::run_mixed(bool compute_u, bool compute_v) {
     ck(exc_dgemm(0, 1, (long)m, k, (long)m, 1.0, D.U.p, (long)m, D.bv.p, (long)m, 0.0, D.Uf.p, k, D.work.p,
                  D.work.n * 8, s),
        "U");
-    ck(exc_rot_back(mi, k, D.bu.p, D.cs.p, s), "back rotation");
-    {
-      const size_t ow = exc_ormqr_house_workspace(mi, k);
-      D.work.alloc(std::max(D.work.n, (ow + 7) / 8), s, false);
-      ck(exc_ormqr_house_ln(mi, k, D.house.p, D.tau_h.p, D.bu.p, k, D.work.p, D.work.n * 8, s), "ormqr house");
-    }
+    // W = P bu, into D.idm, which then trades places with D.bu
+    ckh(hipStreamWaitEvent(s, ev_p, 0), "event");
+    ck(exc_dgemm(0, 0, (long)m, k, (long)m, 1.0, D.Pm.p, (long)m, D.bu.p, k, 0.0, D.idm.p, k, D.work.p,
+                 D.work.n * 8, s),
+       "W");
+    D.bu.swap(D.idm);
     sync();
     time_bsvd = now_seconds() - t0;
     t0 = now_seconds();

@@ -3,6 +3,8 @@ left/right, mask/unmask, even/odd K; round trips at full size."""
 import numpy as np
 import pytest
 
+from givens_restate import givens_numpy
+
 pytestmark = pytest.mark.gpu
 
 
@@ -48,6 +50,21 @@ def test_givens_f32_close(hip, oracle):
     got = hip.givens_apply(x, pairs, cs, True, True, f32=True)
     want = oracle.givens_mask(x, 32, True, True, 3)
     assert np.max(np.abs(got - want)) < 2e-5 * np.abs(want).max()
+
+
+@pytest.mark.parametrize("shape", [(7, 5), (37, 101), (1000, 1000), (513, 1001)])
+@pytest.mark.parametrize("is_left", [True, False])
+@pytest.mark.parametrize("mask", [True, False])
+def test_givens_f32_bit_exact(hip, shape, is_left, mask):
+    """exc_givens_apply_f32 (the mixed form's masks) against its float32 restatement, bit for bit,
+    for even and odd K."""
+    banner(f"givens f32 shape={shape} left={is_left} mask={mask}")
+    rng = np.random.default_rng(shape[0] + 3 * shape[1])
+    x = rng.standard_normal(shape).astype(np.float32)
+    K = shape[0] if is_left else shape[1]
+    pairs, cs = hip.givens_plan(K, 32, mask, 99)
+    got = hip.givens_apply(x, pairs, cs, mask, is_left, f32=True)
+    assert np.array_equal(got, givens_numpy(x, pairs, cs, mask, is_left, np.float32))
 
 
 @pytest.mark.parametrize("m,n,is_left", [(6, 10007, False), (5, 12100, False), (10501, 4, True)])

@@ -136,3 +136,20 @@ def test_loop_colocation_decision():
     assert decide(2, [63, 0], [512, 0], True) == 0
     # the fewest slots decide
     assert decide(2, [100, 100], [512, 150], True) == 0
+
+
+@pytest.mark.parametrize("shape", [(7, 5), (37, 101), (100, 37), (64, 9)])
+@pytest.mark.parametrize("is_left", [True, False])
+@pytest.mark.parametrize("mask", [True, False])
+def test_givens_restatement_matches_oracle(oracle, shape, is_left, mask):
+    """The numpy restatement that the GPU test of the FP32 sweep checks against, run in FP64, is
+    bit-identical to the oracle's sweep (pass order, the odd pair's place, the signs), so only its
+    rounding to float differs when it stands in for the FP32 sweep."""
+    from excalibur_amd import hip
+    from givens_restate import givens_numpy
+    rng = np.random.default_rng(shape[0] * 13 + shape[1])
+    x = rng.standard_normal(shape)
+    K = shape[0] if is_left else shape[1]
+    pairs, cs = hip.givens_plan(K, 32, mask, 99)
+    want = oracle.givens_mask(x, 32, mask, is_left, 99)
+    assert np.array_equal(givens_numpy(x, pairs, cs, mask, is_left, np.float64), want)
