@@ -90,6 +90,16 @@ __device__ __forceinline__ void coh_st(double* p, double v) {
 }
 __device__ __forceinline__ void coh_stores_done() { __builtin_amdgcn_s_waitcnt(0x0F70); }  // vmcnt(0)
 
+// Workgroup barrier that orders LDS accesses only. __syncthreads also waits for every outstanding
+// global load and store of the thread (vmcnt(0)); where a barrier only publishes LDS data, this one
+// lets global stores that no other thread reads before the kernel ends (and prefetches) stay in
+// flight across it.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // Live kernel timing with HIP events on the launch stream (off unless exc_prof_enable(1)).
 namespace exc_prof {
 bool enabled();

@@ -18,14 +18,6 @@ namespace {
 
 constexpr int kThreads = 256;
 
-// Workgroup barrier that orders LDS accesses only: unlike __syncthreads it does not wait for the
-// thread's outstanding global loads, so a plan prefetch stays in flight across it.
-__device__ __forceinline__ void lds_barrier() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-  __builtin_amdgcn_s_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-}
-
 // Plans of at most this many pairs per pass are staged in LDS (three pass buffers): every
 // workgroup reads each pass's pairs and rotations from HBM or L2 once, two passes ahead of their
 // use, instead of once per line it owns and in the shadow of nothing.
