@@ -324,6 +324,89 @@ __device__ void chol_tile(int t, int nb, int* i, int* j) {
 }
 __host__ __device__ inline int chol_index(int i, int j, int nb) { return j * nb - j * (j - 1) / 2 + (i - j); }
 
+
+// ---- the same Cholesky with four wavefronts per tile
+// The one-wavefront tile spends most of a diagonal step outside the factorization itself: the last
+// 64 x 64 x 64 update (256 MFMAs), staging the tiles it needs and storing the finished tile. Here
+// wavefront w holds rows 16w .. 16w + 15 of the tile (four 16 x 16 MFMA blocks), so the updates,
+// the staging and the stores split four ways; the in-register factorization and the solve, chains
+// of dependent operations on 64 rows, stay on wavefront 0. Every block receives the same MFMAs in
+// the same order as in the one-wavefront form, and the factorization and solve are the same code,
+// so the factor is bit-identical.
+constexpr int TNT4 = 256;
+
+// element v of lane l of block b of wavefront w: row 16w + (l / 16) + 4v, column 16b + l % 16
+struct Quad4 {
+  int li, lk, w;
+  __device__ Quad4() {
+    li = threadIdx.x & 15;
+    lk = (threadIdx.x & 63) >> 4;
+    w = threadIdx.x >> 6;
+  }
+  __device__ int row(int v) const { return 16 * w + lk + 4 * v; }
+  __device__ int col(int b) const { return 16 * b + li; }
+};
+
+__device__ __forceinline__ void acc_load4(d4 (&acc)[4], const Quad4& q, const double* A, long lda, long r0, long c0,
+                                          int nr, int nc) {
+#pragma unroll
+  for (int b = 0; b < 4; b++)
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const int r = q.row(v), c = q.col(b);
+      acc[b][v] = (r < nr && c < nc) ? A[(r0 + r) * lda + c0 + c] : 0.0;
+    }
+}
+
+__device__ __forceinline__ void acc_to_lds4(const d4 (&acc)[4], const Quad4& q, double* S) {
+#pragma unroll
+  for (int b = 0; b < 4; b++)
+#pragma unroll
+    for (int v = 0; v < 4; v++) S[q.row(v) * LD + q.col(b)] = acc[b][v];
+}
+
+// T (LDS) <- the nr x nc tile of A at (r0, c0), zero outside (coherent loads): thread t reads
+// column t % 64 of rows 16 (t / 64) .. 16 (t / 64) + 15, all 16 loads in flight.
+__device__ __forceinline__ void tile_to_lds4(double* T, const double* A, long lda, long r0, long c0, int nr, int nc) {
+  const int c = threadIdx.x & 63, rb = (threadIdx.x >> 6) * 16;
+  double v[16];
+#pragma unroll
+  for (int q = 0; q < 16; q++) {
+    const int r = rb + q;
+    v[q] = (r < nr && c < nc) ? coh_ld(&A[(r0 + r) * lda + c0 + c]) : 0.0;
+  }
+#pragma unroll
+  for (int q = 0; q < 16; q++) T[(rb + q) * LD + c] = v[q];
+}
+
+__device__ __forceinline__ void lds_to_tile4(const double* S, double* A, long lda, long r0, long c0, int nr, int nc,
+                                             bool lower_zero) {
+  for (int e = threadIdx.x; e < TB * TB; e += TNT4) {
+    const int r = e >> 6, c = e & 63;
+    if (r < nr && c < nc) coh_st(&A[(r0 + r) * lda + c0 + c], (lower_zero && c > r) ? 0.0 : S[r * LD + c]);
+  }
+}
+
+__device__ __forceinline__ void pad_identity4(double* S, int kb) {
+  for (int e = threadIdx.x; e < TB * TB; e += TNT4) {
+    const int r = e >> 6, c = e & 63;
+    if (r >= kb || c >= kb) S[r * LD + c] = (r == c) ? 1.0 : 0.0;
+  }
+}
+
+// acc -= X Y^T for this wavefront's row block (X, Y 64 x 64 in LDS)
+__device__ __forceinline__ void acc_update4(d4 (&acc)[4], const Quad4& q, const double* X, const double* Y) {
+#pragma unroll 2
+  for (int kk = 0; kk < TB; kk += 4) {
+    const double xa = -X[(16 * q.w + q.li) * LD + kk + q.lk];
+    double yb[4];
+#pragma unroll
+    for (int b = 0; b < 4; b++) yb[b] = Y[(16 * b + q.li) * LD + kk + q.lk];
+#pragma unroll
+    for (int b = 0; b < 4; b++) acc[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(xa, yb[b], acc[b], 0, 0, 0);
+  }
+}
+
 // Tile t of the lower triangle of A's Cholesky factor (t in column-major tile order; every tile
 // waits only on tiles of lower index, so in-order dispatch cannot deadlock).
 __device__ __forceinline__ void potrf_tile_run(int n, double* __restrict__ A, long lda, int nb, int* __restrict__ done,
@@ -337,48 +420,51 @@ __device__ __forceinline__ void potrf_tile_run(int n, double* __restrict__ A, lo
   chol_tile(t, nb, &ti, &tj);
   const long r0 = (long)ti * TB, c0 = (long)tj * TB;
   const int nr = min(TB, n - (int)r0), nc = min(TB, n - (int)c0);
-  const Quad q;
-  d4 acc[4][4];
-  acc_load(acc, q, A, lda, r0, c0, nr, nc);
+  const Quad4 q;
+  d4 acc[4];
+  acc_load4(acc, q, A, lda, r0, c0, nr, nc);
   for (int k = 0; k < tj; k++) {
     if (!wait_flag(&done[chol_index(ti, k, nb)], error, sh)) return;
     if (ti != tj && !wait_flag(&done[chol_index(tj, k, nb)], error, sh)) return;
     const int kc = min(TB, n - k * TB);
-    tile_to_lds(X, A, lda, r0, (long)k * TB, nr, kc);
-    if (ti != tj) tile_to_lds(Y, A, lda, c0, (long)k * TB, nc, kc);
+    tile_to_lds4(X, A, lda, r0, (long)k * TB, nr, kc);
+    if (ti != tj) tile_to_lds4(Y, A, lda, c0, (long)k * TB, nc, kc);
     __syncthreads();
-    acc_update<true>(acc, q, X, ti != tj ? Y : X);
+    acc_update4(acc, q, X, ti != tj ? Y : X);
     __syncthreads();
   }
-  acc_to_lds(acc, q, S);
+  acc_to_lds4(acc, q, S);
   __syncthreads();
   if (ti == tj) {
-    pad_identity(S, nr);
+    pad_identity4(S, nr);
     __syncthreads();
-    const int bad = potf2_loop(S, X, nr);
-    if (threadIdx.x == 0 && bad) atomicCAS(&error[1], 0, (int)r0 + bad);
+    if (threadIdx.x < TB) {
+      const int bad = potf2_loop(S, X, nr);
+      if (threadIdx.x == 0 && bad) atomicCAS(&error[1], 0, (int)r0 + bad);
+    }
     __syncthreads();
-    lds_to_tile(S, A, lda, r0, c0, nr, nc, true);
+    lds_to_tile4(S, A, lda, r0, c0, nr, nc, true);
   } else {
     if (!wait_flag(&done[chol_index(tj, tj, nb)], error, sh)) return;
-    tile_solve<0>(S, X, A + c0 * lda + c0, lda, nc);  // L(i, j) = A(i, j) L(j, j)^-T
+    if (threadIdx.x < TB) tile_solve<0>(S, X, A + c0 * lda + c0, lda, nc);  // L(i, j) = A(i, j) L(j, j)^-T
     __syncthreads();
-    lds_to_tile(S, A, lda, r0, c0, nr, nc, false);
+    lds_to_tile4(S, A, lda, r0, c0, nr, nc, false);
   }
   signal_flag(&done[t]);
 }
 
-__global__ __launch_bounds__(TNT) void k_potrf_tiles(int n, double* __restrict__ A, long lda, int nb,
-                                                     int* __restrict__ done, int* __restrict__ error) {
+__global__ __launch_bounds__(TNT4) void k_potrf_tiles(int n, double* __restrict__ A, long lda, int nb,
+                                                      int* __restrict__ done, int* __restrict__ error) {
   potrf_tile_run(n, A, lda, nb, done, error, blockIdx.x);
 }
 
-// Matrix blockIdx.y of the batch: its own flags (nb * nb) and error pair.
-__global__ __launch_bounds__(TNT) void k_potrf_tiles_batched(int n, double* const* __restrict__ As, long lda, int nb,
-                                                             int* __restrict__ done, int* __restrict__ error) {
+// The same over a batch of matrices of one order (blockIdx.y picks the matrix).
+__global__ __launch_bounds__(TNT4) void k_potrf_tiles_batched(int n, double* const* __restrict__ As, long lda, int nb,
+                                                              int* __restrict__ done, int* __restrict__ error) {
   const int b = blockIdx.y;
   potrf_tile_run(n, As[b], lda, nb, done + (long)b * nb * nb, error + 2 * b, blockIdx.x);
 }
+
 
 // X = L^-1 by tiles (both lower, n x n): tile (i, j), i >= j, in the column-major order of the
 // Cholesky tiles, is X(i, j) = L(i, i)^-1 (delta_ij I - sum_{k=j}^{i-1} L(i, k) X(k, j)); it waits
@@ -543,7 +629,7 @@ int potrf_tiles(int n, double* A, long lda, int* info, void* work, size_t wb, hi
     attr = true;
   }
   hipLaunchKernelGGL(k_zero_ints, dim3(exc_detail::ceil_div(nb * nb + 8, 256)), dim3(256), 0, s, done, nb * nb + 8);
-  hipLaunchKernelGGL(k_potrf_tiles, dim3(T), dim3(TNT), kTileLds, s, n, A, lda, nb, done, err);
+  hipLaunchKernelGGL(k_potrf_tiles, dim3(T), dim3(TNT4), kTileLds, s, n, A, lda, nb, done, err);
   hipLaunchKernelGGL(k_zero_upper_blocks, dim3(exc_detail::ceil_div((long)n * n, 256)), dim3(256), 0, s, n, A, lda);
   EXC_CHECK_HIP(hipGetLastError());
   int herr[2] = {0, 0};
@@ -580,7 +666,7 @@ int potrf_tiles_batched(int n, double* const* d_As, int count, long lda, int* d_
   }
   const long flags = (long)count * nb * nb + 2L * count;
   hipLaunchKernelGGL(k_zero_ints, dim3(exc_detail::ceil_div(flags, 256)), dim3(256), 0, s, done, (int)flags);
-  hipLaunchKernelGGL(k_potrf_tiles_batched, dim3(T, count), dim3(TNT), kTileLds, s, n, d_As, lda, nb, done, err);
+  hipLaunchKernelGGL(k_potrf_tiles_batched, dim3(T, count), dim3(TNT4), kTileLds, s, n, d_As, lda, nb, done, err);
   hipLaunchKernelGGL(k_zero_upper_batched, dim3(std::min<unsigned>(1024, exc_detail::ceil_div((long)n * n, 256)), count),
                      dim3(256), 0, s, n, d_As, lda);
   hipLaunchKernelGGL(k_tile_infos, dim3(exc_detail::ceil_div(count, 256)), dim3(256), 0, s, count, err, d_info);
