@@ -18,23 +18,26 @@ namespace {
 
 constexpr int kThreads = 256;
 
-// Plans of at most this many pairs per pass are staged in LDS (three pass buffers): every
-// workgroup reads each pass's pairs and rotations from HBM or L2 once, two passes ahead of their
-// use, instead of once per line it owns and in the shadow of nothing.
+// Plans of at most this many pairs per pass are staged in LDS (two or three pass buffers): every
+// workgroup reads each pass's pairs and rotations from HBM or L2 once, one or two passes ahead of
+// their use, instead of once per line it owns and in the shadow of nothing.
 constexpr int kStageMaxHalf = 1024;
 constexpr int kStagePerThread = kStageMaxHalf / 256;
-constexpr int kStageBufs = 3;
-size_t stage_bytes(int half) { return half <= kStageMaxHalf ? (size_t)half * 24 * kStageBufs : 0; }
+size_t stage_bytes(int half, int nbuf) { return half <= kStageMaxHalf ? (size_t)half * 24 * nbuf : 0; }
+// items (pair, line) per thread and pass up to which a pass takes the precomputed-index path
+constexpr int kBatch = 8;
 
 // odd_mode: 0 none, 1 the special pair is pair 0 (applied first), 2 it is the last pair.
-// STAGE: the plan of pass p + 2 is loaded into registers before the rotations of pass p and
-// stored into its LDS plan buffer after the rotations of pass p + 1, so a load has two passes to
-// arrive (stage: 3 x half x {2 int32, 2 double}, two register sets).
-template <typename T, bool LEFT, bool STAGE>
+// NBUF plan buffers in LDS (stage: NBUF x half x {2 int32, 2 double}); 0: no staging. With 2, the
+// plan of pass p + 1 is loaded into registers before the rotations of pass p and stored into the
+// other buffer after them; with 3, the plan of pass p + 2 is loaded before the rotations of pass p
+// and stored after those of pass p + 1 (two register sets), so a load has two passes to arrive.
+template <typename T, bool LEFT, int NBUF>
 __device__ __forceinline__ void givens_strip(T* __restrict__ x, long m, long n, int W,
                                              const int32_t* __restrict__ pairs, const double* __restrict__ cs,
                                              int passes, int half, int mask, int odd_mode, T* buf,
                                              unsigned char* stage) {
+  constexpr bool STAGE = NBUF > 0;
   const int tid = threadIdx.x;
   const long K = LEFT ? m : n;
   const long base = (long)blockIdx.x * W;
@@ -55,11 +58,10 @@ __device__ __forceinline__ void givens_strip(T* __restrict__ x, long m, long n, 
   __syncthreads();
   const T sgn = mask ? T(1) : T(-1);
   // The pairs of one pass are disjoint, so the rotations of a range touch disjoint elements: a
-  // thread takes kBatch of its items at once, all their loads before any of their stores, and the
-  // LDS latencies overlap instead of chaining item after item. Every element still receives the
-  // same operations in the same order.
+  // thread takes several of its items at once (kBatchGeneral here, kBatch on the fast path), all
+  // their loads before any of their stores, and the LDS latencies overlap instead of chaining item
+  // after item. Every element still receives the same operations in the same order.
   // (LDS indices fit in 32 bits: the strip is at most 160 KB.)
-  constexpr int kBatch = 8;
   constexpr int kBatchGeneral = 4;
   auto apply_range = [&](const int32_t* pp, const double* cp, int j0, int j1) {
     const int nj = j1 - j0;
@@ -157,8 +159,8 @@ __device__ __forceinline__ void givens_strip(T* __restrict__ x, long m, long n, 
       }
   };
   // staged plan buffers: b holds half int2 pairs then half double2 rotations
-  int32_t* spp[kStageBufs] = {nullptr, nullptr, nullptr};
-  double* scp[kStageBufs] = {nullptr, nullptr, nullptr};
+  int32_t* spp[3] = {nullptr, nullptr, nullptr};
+  double* scp[3] = {nullptr, nullptr, nullptr};
   // a pass's pairs (r1, r2) and rotations (c, s) of up to kStagePerThread pairs per thread, in
   // registers between their load and their store into LDS; scalar loads, so no alignment beyond
   // the elements' own is assumed of the plan
@@ -182,8 +184,8 @@ __device__ __forceinline__ void givens_strip(T* __restrict__ x, long m, long n, 
     }
   };
   auto commit = [&](const PlanRegs& R, int q) {  // pass q's plan into its LDS buffer
-    int32_t* np = spp[q % kStageBufs];
-    double* nc = scp[q % kStageBufs];
+    int32_t* np = spp[q % (STAGE ? NBUF : 1)];
+    double* nc = scp[q % (STAGE ? NBUF : 1)];
 #pragma unroll
     for (int u = 0; u < kStagePerThread; u++) {
       const int idx = tid + u * kThreads;
@@ -196,7 +198,7 @@ __device__ __forceinline__ void givens_strip(T* __restrict__ x, long m, long n, 
     }
   };
   if (STAGE) {
-    for (int b = 0; b < kStageBufs; b++) {
+    for (int b = 0; b < NBUF; b++) {
       unsigned char* base_b = stage + (size_t)b * half * 24;
       spp[b] = reinterpret_cast<int32_t*>(base_b);
       scp[b] = reinterpret_cast<double*>(base_b + (size_t)half * 8);
@@ -205,17 +207,10 @@ __device__ __forceinline__ void givens_strip(T* __restrict__ x, long m, long n, 
       spp[0][idx] = pairs[idx];
       scp[0][idx] = cs[idx];
     }
-    if (passes > 1) issue(R1, 1);
+    if (NBUF == 3 && passes > 1) issue(R1, 1);
     lds_barrier();
   }
-  // Pass p reads buffer p % 3. At its start the loads of pass p + 2 go out into the register set
-  // of p's parity (free: it held pass p's plan, committed at the end of pass p - 1); at its end
-  // pass p + 1's plan is committed into buffer (p + 1) % 3, last read by pass p - 2, which every
-  // thread finished at the barrier that ended it.
-  auto one_pass = [&](int p, PlanRegs& Rp, PlanRegs& Rn) {
-    const int32_t* pp = STAGE ? spp[p % kStageBufs] : pairs + (long)p * half * 2;
-    const double* cp = STAGE ? scp[p % kStageBufs] : cs + (long)p * half * 2;
-    if (STAGE && p + 2 < passes) issue(Rp, p + 2);
+  auto apply_pass = [&](const int32_t* pp, const double* cp) {
     if (fast) {
       if (odd_mode == 1) {
         apply_fast(pp, cp, 1);
@@ -239,7 +234,20 @@ __device__ __forceinline__ void givens_strip(T* __restrict__ x, long m, long n, 
     } else {
       apply_range(pp, cp, 0, half);
     }
-    if (STAGE && p + 1 < passes) commit(Rn, p + 1);
+  };
+  // Pass p reads buffer p % NBUF. With 3 buffers the loads of pass p + 2 go out at its start into
+  // the register set of p's parity (free: it held pass p's plan, committed at the end of pass
+  // p - 1), and pass p + 1's plan is committed at its end into buffer (p + 1) % 3, last read by
+  // pass p - 2; with 2, pass p + 1's plan is loaded at its start and committed at its end into the
+  // buffer pass p - 1 read. Either buffer was released by the barrier that ended its last pass.
+  auto one_pass = [&](int p, PlanRegs& Rp, PlanRegs& Rn) {
+    const int32_t* pp = STAGE ? spp[p % (STAGE ? NBUF : 1)] : pairs + (long)p * half * 2;
+    const double* cp = STAGE ? scp[p % (STAGE ? NBUF : 1)] : cs + (long)p * half * 2;
+    if (NBUF == 3 && p + 2 < passes) issue(Rp, p + 2);
+    if (NBUF == 2 && p + 1 < passes) issue(Rp, p + 1);
+    apply_pass(pp, cp);
+    if (NBUF == 3 && p + 1 < passes) commit(Rn, p + 1);
+    if (NBUF == 2 && p + 1 < passes) commit(Rp, p + 1);
     lds_barrier();
   };
   for (int p = 0; p < passes; p += 2) {
@@ -260,21 +268,21 @@ __device__ __forceinline__ void givens_strip(T* __restrict__ x, long m, long n, 
   }
 }
 
-// LDS: the strip (W lines, rounded up to 256 bytes), then the staged plan when STAGE.
-template <typename T, bool LEFT, bool STAGE>
+// LDS: the strip (W lines, rounded up to 256 bytes), then the NBUF staged plan buffers.
+template <typename T, bool LEFT, int NBUF>
 __global__ __launch_bounds__(kThreads) void k_givens(T* __restrict__ x, long m, long n, int W,
                                                      const int32_t* __restrict__ pairs,
                                                      const double* __restrict__ cs, int passes, int half, int mask,
                                                      int odd_mode) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const size_t strip = ((size_t)W * (LEFT ? m : n) * sizeof(T) + 255) / 256 * 256;
-  givens_strip<T, LEFT, STAGE>(x, m, n, W, pairs, cs, passes, half, mask, odd_mode, reinterpret_cast<T*>(smem_raw),
+  givens_strip<T, LEFT, NBUF>(x, m, n, W, pairs, cs, passes, half, mask, odd_mode, reinterpret_cast<T*>(smem_raw),
                                smem_raw + strip);
 }
 
 // The same sweep over a batch of matrices of one shape, each with its own plan: blockIdx.y picks
 // the matrix.
-template <typename T, bool LEFT, bool STAGE>
+template <typename T, bool LEFT, int NBUF>
 __global__ __launch_bounds__(kThreads) void k_givens_batched(T* const* __restrict__ xs, long m, long n, int W,
                                                              const int32_t* const* __restrict__ pairs,
                                                              const double* const* __restrict__ cs, int passes,
@@ -282,12 +290,38 @@ __global__ __launch_bounds__(kThreads) void k_givens_batched(T* const* __restric
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int b = blockIdx.y;
   const size_t strip = ((size_t)W * (LEFT ? m : n) * sizeof(T) + 255) / 256 * 256;
-  givens_strip<T, LEFT, STAGE>(xs[b], m, n, W, pairs[b], cs[b], passes, half, mask, odd_mode,
+  givens_strip<T, LEFT, NBUF>(xs[b], m, n, W, pairs[b], cs[b], passes, half, mask, odd_mode,
                                reinterpret_cast<T*>(smem_raw), smem_raw + strip);
 }
 
-constexpr size_t kLdsSoft = 64 * 1024;   // two workgroups per CU
+constexpr size_t kLdsSoft = 80 * 1024;   // two workgroups per CU
 constexpr size_t kLdsHard = 160 * 1024;  // one workgroup per CU
+
+// The staged plan (nbuf buffers, none when it does not fit beside one line) and the lines per
+// workgroup: w_want, or as many as fit beside the plan within the soft budget.
+void sweep_geom(int half, size_t line_bytes, int nbuf, long w_want, size_t& stage, long& w) {
+  stage = stage_bytes(half, nbuf) + 256 <= kLdsHard - line_bytes ? stage_bytes(half, nbuf) : 0;
+  const size_t budget = kLdsSoft > stage + line_bytes ? kLdsSoft - stage : line_bytes;
+  const long w_lds = std::max<long>(1, (long)(budget / line_bytes));
+  w = std::max<long>(1, std::min<long>(w_want, w_lds));
+}
+
+// Three plan buffers when the third costs no line per workgroup and the whole still leaves two
+// workgroups per CU; else two (0: the plan is not staged).
+int sweep_plan(int half, size_t line_bytes, long w_want, size_t& stage, long& w) {
+  size_t s3, s2;
+  long w3, w2;
+  sweep_geom(half, line_bytes, 3, w_want, s3, w3);
+  sweep_geom(half, line_bytes, 2, w_want, s2, w2);
+  if (s3 && w3 == w2 && ((size_t)w3 * line_bytes + 255) / 256 * 256 + s3 <= kLdsSoft) {
+    stage = s3;
+    w = w3;
+    return 3;
+  }
+  stage = s2;
+  w = w2;
+  return s2 ? 2 : 0;
+}
 
 template <typename T>
 int givens_launch(T* x, long m, long n, const int32_t* pairs, const double* cs, long passes, int mask, int is_left,
@@ -300,16 +334,12 @@ int givens_launch(T* x, long m, long n, const int32_t* pairs, const double* cs, 
   const int odd_mode = (K % 2 == 1) ? (mask ? 2 : 1) : 0;
   const size_t line_bytes = (size_t)K * sizeof(T);
   EXC_REQUIRE(line_bytes <= kLdsHard, EXC_ERR_ARG, "exc_givens_apply: a line does not fit in LDS (K too large)");
-  // the staged plan needs its LDS beside one line at least
-  const size_t stage = stage_bytes(half) + 256 <= kLdsHard - line_bytes ? stage_bytes(half) : 0;
-  const size_t budget = kLdsSoft > stage + line_bytes ? kLdsSoft - stage : line_bytes;
-  long w_lds = (long)(budget / line_bytes);
-  if (w_lds < 1) w_lds = 1;
-  // enough workgroups to spread over the 256 CUs, as long as the strip fits
-  long w = lines / 256;
-  if (w < 1) w = 1;
-  if (w > w_lds) w = w_lds;
-  if (w > 64) w = 64;
+  // enough workgroups to spread over the 256 CUs, as long as the strip fits and a pass keeps at
+  // most kBatch items per thread (the precomputed-index path)
+  const long w_want = std::max<long>(1, std::min<long>({lines / 256, 64L, (long)kThreads * kBatch / half}));
+  size_t stage = 0;
+  long w = 1;
+  const int nbuf = sweep_plan(half, line_bytes, w_want, stage, w);
   const size_t lds = ((size_t)w * line_bytes + 255) / 256 * 256 + stage;
   const unsigned grid = exc_detail::ceil_div(lines, w);
   hipEvent_t e0 = exc_prof::mark(s);
@@ -321,11 +351,13 @@ int givens_launch(T* x, long m, long n, const int32_t* pairs, const double* cs, 
     return EXC_OK;
   };
   if (is_left) {
-    if (stage) launch(k_givens<T, true, true>);
-    else launch(k_givens<T, true, false>);
+    if (nbuf == 3) launch(k_givens<T, true, 3>);
+    else if (nbuf == 2) launch(k_givens<T, true, 2>);
+    else launch(k_givens<T, true, 0>);
   } else {
-    if (stage) launch(k_givens<T, false, true>);
-    else launch(k_givens<T, false, false>);
+    if (nbuf == 3) launch(k_givens<T, false, 3>);
+    else if (nbuf == 2) launch(k_givens<T, false, 2>);
+    else launch(k_givens<T, false, 0>);
   }
   EXC_CHECK_HIP(hipGetLastError());
   if (e0) {
@@ -349,12 +381,11 @@ int givens_batched_launch(int nb, T* const* d_xs, long m, long n, const int32_t*
   const int odd_mode = (K % 2 == 1) ? (mask ? 2 : 1) : 0;
   const size_t line_bytes = (size_t)K * sizeof(T);
   EXC_REQUIRE(line_bytes <= kLdsHard, EXC_ERR_ARG, "exc_givens_apply_batched: K too large");
-  const size_t stage = stage_bytes(half) + 256 <= kLdsHard - line_bytes ? stage_bytes(half) : 0;
-  const size_t budget = kLdsSoft > stage + line_bytes ? kLdsSoft - stage : line_bytes;
-  long w_lds = (long)(budget / line_bytes);
-  if (w_lds < 1) w_lds = 1;
   // the batch already spreads the work: give each workgroup as many lines as fit in LDS
-  long w = std::max<long>(1, std::min<long>({w_lds, 64L, std::max<long>(1, lines * nb / 512)}));
+  const long w_want = std::max<long>(1, std::min<long>(64L, lines * nb / 512));
+  size_t stage = 0;
+  long w = 1;
+  const int nbuf = sweep_plan(half, line_bytes, w_want, stage, w);
   const size_t lds = ((size_t)w * line_bytes + 255) / 256 * 256 + stage;
   const dim3 grid(exc_detail::ceil_div(lines, w), nb);
   auto launch = [&](auto kernel) {
@@ -365,11 +396,13 @@ int givens_batched_launch(int nb, T* const* d_xs, long m, long n, const int32_t*
     return EXC_OK;
   };
   if (is_left) {
-    if (stage) launch(k_givens_batched<T, true, true>);
-    else launch(k_givens_batched<T, true, false>);
+    if (nbuf == 3) launch(k_givens_batched<T, true, 3>);
+    else if (nbuf == 2) launch(k_givens_batched<T, true, 2>);
+    else launch(k_givens_batched<T, true, 0>);
   } else {
-    if (stage) launch(k_givens_batched<T, false, true>);
-    else launch(k_givens_batched<T, false, false>);
+    if (nbuf == 3) launch(k_givens_batched<T, false, 3>);
+    else if (nbuf == 2) launch(k_givens_batched<T, false, 2>);
+    else launch(k_givens_batched<T, false, 0>);
   }
   EXC_CHECK_HIP(hipGetLastError());
   return EXC_OK;
