@@ -53,7 +53,6 @@ __global__ __launch_bounds__(256) void k_cvt_f64_f32_flat(long n, const double* 
   if (blockIdx.x == 0 && threadIdx.x == 0 && (n & 1)) dst[n - 1] = (float)src[n - 1];
 }
 
-// out[0] = trace(A) (sequential order, one thread: n is small).
 // out[0] = trace(A): thread t sums the diagonal entries t, t + 256, ... in order, then the 256
 // partial sums are added in a fixed tree order (the same bits on every run and every device).
 __global__ __launch_bounds__(256) void k_trace(int n, const double* __restrict__ a, long lda, double* __restrict__ out) {
@@ -205,17 +204,28 @@ __global__ __launch_bounds__(256) void k_oa_coeffs(int m, const double* __restri
   G[e] = g;
 }
 
-// flags[j] = 1 when value j has a cluster partner and lies above floor * smax.
+// flags[j] = 1 when value j has a cluster partner and lies above floor * smax. With STAGED the
+// workgroup first copies all m values into LDS (m * 8 bytes of dynamic LDS), so the scan, every
+// lane reading the same value at each step, runs on LDS broadcasts instead of one dependent global
+// load per step; the comparisons are the same.
+template <bool STAGED>
 __global__ __launch_bounds__(256) void k_oa_members(int m, const double* __restrict__ sigma,
                                                     const double* __restrict__ smax, double tau, double floor_rel,
                                                     int* __restrict__ flags) {
+  extern __shared__ double sg[];
+  const double* sv = sigma;
+  if (STAGED) {
+    for (int i = threadIdx.x; i < m; i += blockDim.x) sg[i] = sigma[i];
+    __syncthreads();
+    sv = sg;
+  }
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= m) return;
-  const double sj = sigma[j];
+  const double sj = sv[j];
   int f = 0;
   if (fabs(sj) > floor_rel * smax[0])
     for (int i = 0; i < m && !f; i++)
-      if (i != j && oa_cluster(sigma[i], sj, tau)) f = 1;
+      if (i != j && oa_cluster(sv[i], sj, tau)) f = 1;
   flags[j] = f;
 }
 
@@ -402,8 +412,12 @@ int exc_oa_refine(int m, const double* d_l, long ldl, int l_lower, double* d_u, 
     }
     hipLaunchKernelGGL(k_oa_sigma, dim3(1), dim3(1024), 0, s, m, T, UtU, WtW, sig, smax);
     hipLaunchKernelGGL(k_oa_coeffs, dim3(ge), dim3(256), 0, s, m, T, UtU, WtW, sig, tau, F, G);
-    hipLaunchKernelGGL(k_oa_members, dim3(exc_detail::ceil_div(m, 256)), dim3(256), 0, s, m, sig, smax, tau,
-                       floor_rel, flags);
+    if ((size_t)m * sizeof(double) <= 64 * 1024)
+      hipLaunchKernelGGL(k_oa_members<true>, dim3(exc_detail::ceil_div(m, 256)), dim3(256),
+                         (size_t)m * sizeof(double), s, m, sig, smax, tau, floor_rel, flags);
+    else
+      hipLaunchKernelGGL(k_oa_members<false>, dim3(exc_detail::ceil_div(m, 256)), dim3(256), 0, s, m, sig, smax, tau,
+                         floor_rel, flags);
     EXC_CHECK_HIP(hipGetLastError());
     EXC_CHECK_HIP(hipMemcpy2DAsync(Un, m * sizeof(double), d_u, ldu * sizeof(double), m * sizeof(double), m,
                                    hipMemcpyDeviceToDevice, s));
