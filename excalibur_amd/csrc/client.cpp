@@ -212,6 +212,7 @@ struct PlanJob {
   size_t pair_words = 0, count = 0;  // doubles: pairs (as int32) then c/s
   HBuf host;
   DBuf dev;
+  bool uploaded = false;  // dev holds the plan (uploaded once per run, by the first sweep)
 };
 
 class RunPlans {
@@ -258,6 +259,18 @@ class RunPlans {
   bool failed_ = false;
 };
 
+// The plan of (K, mask, seed) on the device, uploaded by the first sweep that needs it; the copy
+// is ordered before that sweep on s and overwrites the whole buffer, so it is not zeroed first.
+PlanJob& device_plan(RunPlans& plans, long K, int mask, unsigned seed, hipStream_t s) {
+  PlanJob& p = plans.get(K, mask, seed);
+  if (!p.uploaded) {
+    p.dev.alloc(p.count, s, false);
+    ckh(hipMemcpyAsync(p.dev.p, p.host.p, p.count * sizeof(double), hipMemcpyHostToDevice, s), "plan");
+    p.uploaded = true;
+  }
+  return p;
+}
+
 // block_shift_mask(m, n, 32, x, mask, is_left, seed) for K <= 10000 (bbtsvd.cpp:533-546). The
 // sweep is enqueued on s behind an asynchronous upload of the plan; the caller synchronizes.
 void givens_mask(double* d_x, long m, long n, int mask, int is_left, unsigned seed, RunPlans& plans, hipStream_t s) {
@@ -270,9 +283,7 @@ void givens_mask(double* d_x, long m, long n, int mask, int is_left, unsigned se
     ck(exc_block_shift_mask(d_x, m, n, 32, mask, is_left, seed, w.p, w.n * 8, s), "virtual-matrix mask");
     return;
   }
-  PlanJob& p = plans.get(K, mask, seed);
-  p.dev.alloc(p.count, s);
-  ckh(hipMemcpyAsync(p.dev.p, p.host.p, p.count * sizeof(double), hipMemcpyHostToDevice, s), "plan");
+  PlanJob& p = device_plan(plans, K, mask, seed, s);
   ck(exc_givens_apply(d_x, m, n, (const int32_t*)p.dev.p, p.dev.p + p.pair_words, 32, mask, is_left, s),
      "givens apply");
 }
@@ -283,9 +294,7 @@ void givens_mask_f32(float* d_x, long m, long n, int mask, int is_left, unsigned
   const long K = is_left ? m : n;
   if (K <= 1) return;
   if (K > 10000) throw std::runtime_error("Givens masks of more than 10000 lines are not available in FP32");
-  PlanJob& p = plans.get(K, mask, seed);
-  p.dev.alloc(p.count, s);
-  ckh(hipMemcpyAsync(p.dev.p, p.host.p, p.count * sizeof(double), hipMemcpyHostToDevice, s), "plan");
+  PlanJob& p = device_plan(plans, K, mask, seed, s);
   ck(exc_givens_apply_f32(d_x, m, n, (const int32_t*)p.dev.p, p.dev.p + p.pair_words, 32, mask, is_left, s),
      "givens apply");
 }
