@@ -1502,12 +1502,20 @@ int DecentralizedClient::run_mixed(bool compute_u, bool compute_v) {
   };
   // Cholesky of D.Gm + shift I into L, the shift doubled from `shift` (0: first without one)
   auto shifted_chol = [&](DBuf& L, double shift, const char* what) {
-    double tr = 0.0;
-    ck(exc_trace(mi, D.Gm.p, (long)m, D.scal.p, s), "trace");
-    ckh(hipMemcpyAsync(&tr, D.scal.p, 8, hipMemcpyDeviceToHost, s), "d2h");
-    sync();
-    const double step = u32 * tr;
-    if (shift < 0) shift = step;  // a first pass in single precision always needs one
+    // the shift step u32 * trace(G), read back only when a shift is needed (a pass that starts
+    // without one needs it only if that attempt fails)
+    double step = -1.0;
+    auto shift_step = [&]() {
+      if (step < 0.0) {
+        double tr = 0.0;
+        ck(exc_trace(mi, D.Gm.p, (long)m, D.scal.p, s), "trace");
+        ckh(hipMemcpyAsync(&tr, D.scal.p, 8, hipMemcpyDeviceToHost, s), "d2h");
+        sync();
+        step = u32 * tr;
+      }
+      return step;
+    };
+    if (shift < 0) shift = shift_step();  // a first pass in single precision always needs one
     for (int attempt = 0; attempt < 60; attempt++) {
       ckh(hipMemcpyAsync(L.p, D.Gm.p, mm * 8, hipMemcpyDeviceToDevice, s), "copy");
       if (shift > 0) ck(exc_add_diag(mi, L.p, (long)m, shift, s), "shift");
@@ -1519,7 +1527,7 @@ int DecentralizedClient::run_mixed(bool compute_u, bool compute_v) {
         if (shift > 0) log(std::string(what) + ": shift " + std::to_string(shift));
         return;
       }
-      shift = shift > 0 ? 2 * shift : step;
+      shift = shift > 0 ? 2 * shift : shift_step();
     }
     throw std::runtime_error(std::string(what) + ": no shift made the Gram matrix positive definite");
   };
