@@ -21,6 +21,8 @@
 #include <vector>
 #include <tuple>
 #include <thread>
+#include <functional>
+#include <exception>
 #include <memory>
 
 #include "exc_common.hpp"
@@ -1638,6 +1640,39 @@ int DecentralizedClient::run_mixed(bool compute_u, bool compute_v) {
     }
   } events;
   if (!D.s2) ckh(hipStreamCreateWithFlags(&D.s2, hipStreamNonBlocking), "stream");
+  // Side-stream work is enqueued from a host thread of its own: its tens of launches would
+  // otherwise hold back the main stream's next launch by the time they take on the host. The
+  // thread makes the main thread's device current first (the current device is per thread); a
+  // failure in it is rethrown by join(), which comes before any wait on the thread's events.
+  int device = 0;
+  ckh(hipGetDevice(&device), "device");
+  struct SideThread {
+    std::thread th;
+    std::exception_ptr err;
+    int device = 0;
+    void run(std::function<void()> f) {
+      th = std::thread([this, f]() {
+        try {
+          ckh(hipSetDevice(device), "device");
+          f();
+        } catch (...) {
+          err = std::current_exception();
+        }
+      });
+    }
+    void join() {
+      if (th.joinable()) th.join();
+      if (err) {
+        std::exception_ptr e = err;
+        err = nullptr;
+        std::rethrow_exception(e);
+      }
+    }
+    ~SideThread() {
+      if (th.joinable()) th.join();
+    }
+  } side;
+  side.device = device;
   // the side stream's workspace, sized for both of its jobs before either runs
   D.work2.alloc((std::max(exc_fact_workspace(mi, (long)m), exc_ormqr_house_workspace(mi, (long)m)) + 7) / 8, s,
                 false);
@@ -1648,12 +1683,13 @@ int DecentralizedClient::run_mixed(bool compute_u, bool compute_v) {
     ev_inv = events.make();
     D.L3i.alloc(mm, s, false);
     ckh(hipEventRecord(ev_l3, s), "event");
-    ckh(hipStreamWaitEvent(D.s2, ev_l3, 0), "event");
-    ck(exc_fill2d((long)m, (long)m, 0.0, D.L3i.p, (long)m, D.s2), "identity");
-    ck(exc_add_diag(mi, D.L3i.p, (long)m, 1.0, D.s2), "identity");
-    ck(exc_trsm_left(0, 0, mi, (long)m, D.L3.p, (long)m, D.L3i.p, (long)m, D.work2.p, D.work2.n * 8, D.s2),
-       "L3 inverse");
-    ckh(hipEventRecord(ev_inv, D.s2), "event");
+    side.run([&D, ev_l3, ev_inv, mi, m = (long)m]() {
+      ckh(hipStreamWaitEvent(D.s2, ev_l3, 0), "event");
+      ck(exc_fill2d(m, m, 0.0, D.L3i.p, m, D.s2), "identity");
+      ck(exc_add_diag(mi, D.L3i.p, m, 1.0, D.s2), "identity");
+      ck(exc_trsm_left(0, 0, mi, m, D.L3.p, m, D.L3i.p, m, D.work2.p, D.work2.n * 8, D.s2), "L3 inverse");
+      ckh(hipEventRecord(ev_inv, D.s2), "event");
+    });
   }
   sync();
   tf += now_seconds() - t0;
@@ -1704,13 +1740,16 @@ int DecentralizedClient::run_mixed(bool compute_u, bool compute_v) {
     {
       hipEvent_t ev_bd = events.make();
       ckh(hipEventRecord(ev_bd, s), "event");
-      ckh(hipStreamWaitEvent(D.s2, ev_bd, 0), "event");
-      ck(exc_fill2d((long)m, (long)m, 0.0, D.Pm.p, (long)m, D.s2), "identity");
-      ck(exc_add_diag(mi, D.Pm.p, (long)m, 1.0, D.s2), "identity");
-      ck(exc_rot_back(mi, mi, D.Pm.p, D.cs.p, D.s2), "back rotation");
-      ck(exc_ormqr_house_ln(mi, (long)m, D.house.p, D.tau_h.p, D.Pm.p, (long)m, D.work2.p, D.work2.n * 8, D.s2),
-         "ormqr house");
-      ckh(hipEventRecord(ev_p, D.s2), "event");
+      side.join();  // the L3 inverse's launches are all on the side stream before these
+      side.run([&D, ev_bd, ev_p, mi, m = (long)m]() {
+        ckh(hipStreamWaitEvent(D.s2, ev_bd, 0), "event");
+        ck(exc_fill2d(m, m, 0.0, D.Pm.p, m, D.s2), "identity");
+        ck(exc_add_diag(mi, D.Pm.p, m, 1.0, D.s2), "identity");
+        ck(exc_rot_back(mi, mi, D.Pm.p, D.cs.p, D.s2), "back rotation");
+        ck(exc_ormqr_house_ln(mi, m, D.house.p, D.tau_h.p, D.Pm.p, m, D.work2.p, D.work2.n * 8, D.s2),
+           "ormqr house");
+        ckh(hipEventRecord(ev_p, D.s2), "event");
+      });
     }
     DBuf& b2 = D.b2;
     b2.alloc(2, s);
@@ -1735,6 +1774,7 @@ int DecentralizedClient::run_mixed(bool compute_u, bool compute_v) {
                  D.work.n * 8, s),
        "U");
     // W = P bu, into D.idm, which then trades places with D.bu
+    side.join();
     ckh(hipStreamWaitEvent(s, ev_p, 0), "event");
     ck(exc_dgemm(0, 0, (long)m, k, (long)m, 1.0, D.Pm.p, (long)m, D.bu.p, k, 0.0, D.idm.p, k, D.work.p,
                  D.work.n * 8, s),
@@ -1751,6 +1791,7 @@ int DecentralizedClient::run_mixed(bool compute_u, bool compute_v) {
          "refinement");
     }
     if (l3_inverse) {  // W' = (L3^-1)^T W, into D.idm, which then trades places with D.bu
+      side.join();
       ckh(hipStreamWaitEvent(s, ev_inv, 0), "event");
       D.work.alloc(std::max(D.work.n, (exc_gemm_workspace((long)m, k, (long)m) + 7) / 8), s, false);
       ck(exc_dgemm(1, 0, (long)m, k, (long)m, 1.0, D.L3i.p, (long)m, D.bu.p, k, 0.0, D.idm.p, k, D.work.p,
