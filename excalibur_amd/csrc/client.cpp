@@ -361,7 +361,9 @@ DecentralizedClient::DecentralizedClient(int cid, int nclient, int mode, size_t 
   }
 }
 
-DecentralizedClient::~DecentralizedClient() = default;
+// The network goes first: its reader threads are joined before the device state (whose pinned
+// staging a timed-out direct receive may still be filling) is freed.
+DecentralizedClient::~DecentralizedClient() { net_.reset(); }
 
 void DecentralizedClient::log(const std::string& line) {
   std::ofstream f(log_dir + "/Client" + std::to_string(client_id) + ".log", std::ios::app);
@@ -1438,9 +1440,9 @@ int DecentralizedClient::run_mixed(bool compute_u, bool compute_v) {
   // Every peer's packed triangle (this peer's in D.pk, FP32 or FP64) summed in FP64 on the device
   // in peer order 0..P-1 (exc_acc_packed), so every peer holds the same bits in D.pacc. The own
   // triangle, already in pinned memory at h_own, goes to every peer over TCP; each other peer's
-  // frame is copied into a pinned staging slot and uploaded in four pieces, the host copy of a
-  // piece overlapping the upload of the one before. The two slots alternate over the senders; a
-  // slot is refilled only after its last upload has completed.
+  // frame is read from the socket into a pinned staging slot (PeerNet::receive_direct) and
+  // uploaded from there. The two slots alternate over the senders; a slot is refilled only after
+  // its upload has completed.
   hipEvent_t slot_free[2] = {nullptr, nullptr};
   for (auto& e : slot_free) ckh(hipEventCreateWithFlags(&e, hipEventDisableTiming), "event");
   struct EventGuard {
@@ -1458,22 +1460,13 @@ int DecentralizedClient::run_mixed(bool compute_u, bool compute_v) {
     for (int j = 0; j < (int)num_client; j++) {
       const void* part = D.pk.p;
       if (j != cid) {
-        std::vector<char> frame = net.receive(j);
-        if (frame.size() != bytes)
-          throw std::runtime_error("Gram exchange: frame of " + std::to_string(frame.size()) + " bytes from peer " +
-                                   std::to_string(j) + ", expected " + std::to_string(bytes));
         const int slot = used & 1;
         if (used >= 2) ckh(hipEventSynchronize(slot_free[slot]), "staging slot");
         char* hs = (char*)(D.h_pstage.p + (size_t)slot * n_tri);
         char* ds = (char*)(D.pstage.p + (size_t)slot * n_tri);
-        const size_t piece = (bytes + 3) / 4;
-        for (size_t o = 0; o < bytes; o += piece) {
-          const size_t len = std::min(piece, bytes - o);
-          memcpy(hs + o, frame.data() + o, len);
-          ckh(hipMemcpyAsync(ds + o, hs + o, len, hipMemcpyHostToDevice, s), "h2d");
-        }
+        net.receive_direct(j, hs, bytes);  // read from the socket straight into the pinned slot
+        ckh(hipMemcpyAsync(ds, hs, bytes, hipMemcpyHostToDevice, s), "h2d");
         ckh(hipEventRecord(slot_free[slot], s), "event");
-        net.recycle(j, std::move(frame));
         part = ds;
         used++;
       }

@@ -176,12 +176,29 @@ void PeerNet::reader_loop(int fd, int from) {
     uint64_t size = 0;
     if (!read_spin(fd, &size, sizeof size, hot_until, stop_)) break;
     std::vector<char> msg;
+    char* direct = nullptr;
     {
       std::lock_guard<std::mutex> g(box.mu);
-      if (!box.pool.empty()) {
-        msg = std::move(box.pool.back());
-        box.pool.pop_back();
+      if (box.posted && box.q.empty() && box.posted_bytes == size) {  // the next frame, posted for
+        direct = box.posted;
+        box.posted = nullptr;
+      } else {
+        box.reading = true;
+        if (!box.pool.empty()) {
+          msg = std::move(box.pool.back());
+          box.pool.pop_back();
+        }
       }
+    }
+    if (direct) {
+      if (size && !read_spin(fd, direct, size, now_seconds() + 1.0, stop_)) break;
+      hot_until = now_seconds() + kHotSeconds;
+      {
+        std::lock_guard<std::mutex> g(box.mu);
+        box.posted_done = true;
+      }
+      box.cv.notify_all();
+      continue;
     }
     msg.resize(size);  // a pooled buffer at least this large only changes its length
     if (size && !read_spin(fd, msg.data(), size, now_seconds() + 1.0, stop_)) break;
@@ -190,8 +207,9 @@ void PeerNet::reader_loop(int fd, int from) {
       std::lock_guard<std::mutex> g(box.mu);
       box.q.push_back(std::move(msg));
       box.n++;
+      box.reading = false;
     }
-    box.cv.notify_one();
+    box.cv.notify_all();
   }
   std::lock_guard<std::mutex> g(box.mu);
   box.closed = true;
@@ -267,6 +285,37 @@ void PeerNet::recycle(int peer, std::vector<char>&& m) {
   Inbox& box = *inbox_[peer];
   std::lock_guard<std::mutex> g(box.mu);
   if (box.pool.size() < 4) box.pool.push_back(std::move(m));
+}
+
+void PeerNet::receive_direct(int peer, void* buf, size_t bytes) {
+  Inbox& box = *inbox_[peer];
+  std::unique_lock<std::mutex> lk(box.mu);
+  if (!box.q.empty() || box.reading || box.closed) {  // the frame is (being) queued: copy it
+    lk.unlock();
+    receive_into(peer, buf, bytes);
+    return;
+  }
+  box.posted = (char*)buf;
+  box.posted_bytes = bytes;
+  box.posted_done = false;
+  // done when the reader filled buf; a frame that went to the queue instead (another size) or a
+  // closed connection ends the wait too
+  auto ready = [&] { return box.posted_done || !box.q.empty() || box.closed; };
+  bool ok = true;
+  if (timeout_s_ > 0) ok = box.cv.wait_for(lk, std::chrono::duration<double>(timeout_s_), ready);
+  else box.cv.wait(lk, ready);
+  if (box.posted_done) {
+    box.posted_done = false;
+    return;
+  }
+  box.posted = nullptr;  // not taken yet: withdrawn
+  if (!ok) throw std::runtime_error("receive from peer " + std::to_string(peer) + " timed out");
+  if (!box.q.empty()) {
+    lk.unlock();
+    receive_into(peer, buf, bytes);  // throws on the size mismatch that queued it
+    return;
+  }
+  throw std::runtime_error("peer " + std::to_string(peer) + " disconnected");
 }
 
 void PeerNet::receive_into(int peer, void* buf, size_t bytes) {

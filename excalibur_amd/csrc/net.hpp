@@ -35,6 +35,12 @@ class PeerNet {
   void receive_into(int peer, void* buf, size_t bytes);
   // Returns a consumed frame's buffer (from receive) to the pool of the peer it came from.
   void recycle(int peer, std::vector<char>&& m);
+  // The next frame from `peer` (exactly `bytes` bytes) into buf. When no part of it has arrived
+  // yet, buf is posted to the reader, which reads the payload from the socket straight into it
+  // (no copy through a pool buffer: for large frames into pinned memory, that copy is the cost);
+  // otherwise the queued frame is copied. Throws on a size mismatch, timeout or disconnect; after
+  // a timeout the reader may still be filling buf, so buf must outlive this PeerNet.
+  void receive_direct(int peer, void* buf, size_t bytes);
   void broadcast(const void* buf, size_t bytes);
 
   // client.cpp:313-356: below P*10000 values a broadcast and a sum of the others' buffers in
@@ -70,6 +76,13 @@ class PeerNet {
     // frame buffers handed back after use: the reader fills them again without zeroing or
     // faulting in fresh pages (large frames otherwise cost a memset and a page fault per 4 KB)
     std::vector<std::vector<char>> pool;
+    // receive_direct: a caller buffer posted for the next frame, which the reader then reads
+    // straight from the socket into it (posted_done when complete); reading: a frame is being
+    // read into a pool buffer
+    char* posted = nullptr;
+    size_t posted_bytes = 0;
+    bool posted_done = false;
+    bool reading = false;
   };
   void accept_loop();
   void reader_loop(int fd, int from);

@@ -2,9 +2,11 @@
 // fills buffers from a seeded generator, runs the collectives the mixed form uses and prints a
 // digest of each result, so that the caller can check that every peer holds the same bits and
 // that they equal the sum in peer order.
+#include <chrono>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
+#include <thread>
 #include <vector>
 
 #include "net.hpp"
@@ -45,8 +47,25 @@ int main(int argc, char** argv) {
     for (int j = 0; j < P; j++) acc = acc + value(j, q);
     if (o[q] != acc) bad++;
   }
-  printf("%016llx %016llx %ld\n", (unsigned long long)fnv(d.data(), n * 8), (unsigned long long)fnv(o.data(), n * 8),
-         bad);
+  // receive_direct: every peer sends its values to every other peer twice; the first round is
+  // received at once (the buffer is posted before the frame arrives, for large n), the second
+  // after a pause (the frame is already queued and is copied)
+  long bad_direct = 0;
+  for (int round = 0; round < 2; round++) {
+    std::vector<double> mine(n), got(n);
+    for (long q = 0; q < n; q++) mine[q] = value(cid, q) + round;
+    net.barrier();
+    net.broadcast(mine.data(), n * sizeof(double));
+    if (round == 1) std::this_thread::sleep_for(std::chrono::milliseconds(200));
+    for (int j = 0; j < P; j++) {
+      if (j == cid) continue;
+      net.receive_direct(j, got.data(), n * sizeof(double));
+      for (long q = 0; q < n; q++)
+        if (got[q] != value(j, q) + round) bad_direct++;
+    }
+  }
+  printf("%016llx %016llx %ld %ld\n", (unsigned long long)fnv(d.data(), n * 8),
+         (unsigned long long)fnv(o.data(), n * 8), bad, bad_direct);
   net.barrier();
   return 0;
 }

@@ -1,7 +1,9 @@
 """The host TCP transport (excalibur_amd/csrc/net.cpp) without a GPU: P peer processes of a small
 driver (tests/cpp/net_main.cpp) run all_reduce_sum_same (ring above P * 10000 values, in peer order
 below) and all_reduce_sum_same_f32; every peer must hold the same bits, and the FP32 form must
-equal the FP64 sum in peer order of the widened values."""
+equal the FP64 sum in peer order of the widened values. receive_direct must deliver every frame
+intact, both when the buffer is posted before the frame arrives and when the frame is queued
+first."""
 import os
 import socket
 import subprocess
@@ -45,3 +47,4 @@ def test_sums_identical_on_every_peer(driver, P, n):
         outs.append(out.split())
     assert all(o[:2] == outs[0][:2] for o in outs)
     assert all(o[2] == "0" for o in outs)
+    assert all(o[3] == "0" for o in outs)  # receive_direct delivered every frame intact
