@@ -15,7 +15,7 @@ refinement); the plain FP64 form (the reference's algorithm: local QR, Givens ma
 per-column all-reduce bidiagonalization, bisection + twisted factorization) is measured beside it
 on the same input, and both again on the short-wide 2 x 2,000 x 100,000 workload with a
 clustered spectrum. Accuracy against the generator's exact singular values is reported for
-every run. Prints one JSON line. --gpus N > 1 exits: this build measures one GPU.
+every run. Prints one JSON line. --gpus N > 1 exits: the peers share one GPU, and only that case is measured.
 """
 import argparse
 import json
@@ -407,7 +407,7 @@ def main():
                     help="only copy MIB mebibytes device to device once (HBM traffic calibration for PMC runs)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: every GPU box of this build has one MI355X and the peers share it; "
+        print(f"bench.py measures one GPU: all peers of the workload share one MI355X; "
               f"--gpus {args.gpus} is not supported", file=sys.stderr)
         sys.exit(2)
     ensure_built()

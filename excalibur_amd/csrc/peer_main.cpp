@@ -1,5 +1,5 @@
 // Peer executable: the reference's main.cpp command line (main.cpp:16-33, 58-104) in front of the
-// GPU DecentralizedClient, plus the long options of this build (--form, --refine-iters, --dump,
+// GPU DecentralizedClient, plus long options of its own (--form, --refine-iters, --dump,
 // --recv-timeout, --repeat, --profile, --ralha-loop, --device-exchange).
 #include <getopt.h>
 #include <unistd.h>

@@ -2,7 +2,7 @@
 clipping, the reference's 14-argument constructor with host results, and the full-size
 2 x 50,000 x 1,000 workload with its accuracy bars.
 
-Mode 0 is the reference's LAPACKE_dbdsdc branch (client.cpp:975-980); this build routes it through
+Mode 0 is the reference's LAPACKE_dbdsdc branch (client.cpp:975-980); this library routes it through
 the same bisection + twisted factorization as mode 1, so its results equal mode 1's bit for bit.
 Mode 2 computes only U and mode 3 only V (client.cpp:1002, 1119); the shared stages are the same
 kernels on the same data, so what they compute equals mode 1's bit for bit.
