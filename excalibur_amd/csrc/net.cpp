@@ -182,6 +182,10 @@ void PeerNet::reader_loop(int fd, int from) {
       if (box.posted && box.q.empty() && box.posted_bytes == size) {  // the next frame, posted for
         direct = box.posted;
         box.posted = nullptr;
+        // a receive that gives up (time-out) while this read is in flight leaves it running; the
+        // flag then sends the next receive to the queue instead of posting another buffer that
+        // this read's completion would be taken for
+        box.reading = true;
       } else {
         box.reading = true;
         if (!box.pool.empty()) {
@@ -196,6 +200,7 @@ void PeerNet::reader_loop(int fd, int from) {
       {
         std::lock_guard<std::mutex> g(box.mu);
         box.posted_done = true;
+        box.reading = false;
       }
       box.cv.notify_all();
       continue;
