@@ -73,27 +73,39 @@ def run_peers(xs, seed=0, mode=1, topk=-1, opt=3, form="fp64", dump=True, workdi
     write_peer_files(xs, dpath)
     if ground_truth is not None:
         write_ground_truth(ground_truth[0], ground_truth[1], dpath)
-    port = free_port(P + 1)
-    procs = []
-    for cid in range(P):
-        cmd = [PEER_PATH, "-l", "127.0.0.1", "-p", str(port), "-m", str(m), "-n", str(xs[cid].shape[1]),
-               "-d", dpath, "-g", logdir, "-k", str(topk), "-o", str(mode), "-t", str(P), "-c", str(cid),
-               "-e", "1" if evaluate else "0", "-f", "1" if memmap else "0", "-s", str(seed), "-a", str(opt), "--form", form,
-               "--dump", "1" if dump else "0", "--recv-timeout", str(recv_timeout)] + list(extra or [])
-        out = open(os.path.join(logdir, f"peer{cid}.out"), "w")
-        procs.append((subprocess.Popen(cmd, stdout=out, stderr=subprocess.STDOUT), out))
-    t0 = time.time()
-    rcs = []
-    for p, out in procs:
-        left = max(1.0, timeout - (time.time() - t0))
-        try:
-            rcs.append(p.wait(timeout=left))
-        except subprocess.TimeoutExpired:
-            for q, _ in procs:
-                if q.poll() is None:
-                    q.kill()
-            rcs.append(-9)
-        out.close()
+    def launch_once():
+        port = free_port(P + 1)
+        procs = []
+        for cid in range(P):
+            cmd = [PEER_PATH, "-l", "127.0.0.1", "-p", str(port), "-m", str(m), "-n", str(xs[cid].shape[1]),
+                   "-d", dpath, "-g", logdir, "-k", str(topk), "-o", str(mode), "-t", str(P), "-c", str(cid),
+                   "-e", "1" if evaluate else "0", "-f", "1" if memmap else "0", "-s", str(seed), "-a", str(opt),
+                   "--form", form, "--dump", "1" if dump else "0", "--recv-timeout", str(recv_timeout)] + list(extra or [])
+            out = open(os.path.join(logdir, f"peer{cid}.out"), "w")
+            procs.append((subprocess.Popen(cmd, stdout=out, stderr=subprocess.STDOUT), out))
+        t0 = time.time()
+        rcs = []
+        for p, out in procs:
+            left = max(1.0, timeout - (time.time() - t0))
+            try:
+                rcs.append(p.wait(timeout=left))
+            except subprocess.TimeoutExpired:
+                for q, _ in procs:
+                    if q.poll() is None:
+                        q.kill()
+                rcs.append(-9)
+            out.close()
+        return rcs
+
+    # another process can take a probed port before a peer binds it: the peers then start again
+    # on other ports (once)
+    for attempt in range(2):
+        rcs = launch_once()
+        if all(rc == 0 for rc in rcs) or attempt == 1:
+            break
+        logs = [open(os.path.join(logdir, f"peer{cid}.out")).read() for cid in range(P)]
+        if not any("bind failed" in text for text in logs):
+            break
     res = {"workdir": workdir, "rc": rcs, "peers": []}
     for cid in range(P):
         text = open(os.path.join(logdir, f"peer{cid}.out")).read()
